@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
+
+    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,d3q27] [--no-cpu]
+
+A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
+HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
+u = (0.1, 0, 0), tau 0.55, uint8 flags) with FP32 population storage; further legs run the same
+geometry with FP16 shifted-population storage and with D3Q27.  Prints progress on stderr and ONE
+JSON line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
+(apps/cfd-bench.cpp:111-117); the roofline uses algorithmic HBM bytes per launch (each fluid
+population read and written once, plus one read of the flag array) over the kernel time measured
+with HIP events on the solver's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "D3Q19 MLUPS fwd FP32/FP16-store + adjoint, % of 8 TB/s HBM roofline"
+PEAK_GBS = 8000.0
+BYTES_PER_POP = {"fp32": 4, "fp16s": 2, "fp64": 8}
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_stream_collide_d3q19_256_cavity.json"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cavity_flags(nx, ny, nz):
+    """Lid y = ny-1 moving (wall-velocity slot 1); walls x = 0, nx-1, y = 0, z = 0, nz-1 at rest.
+    Side walls are written after the lid, as apps/cfd-solve.cpp:258-265 does in 2-D."""
+    f = np.zeros((nz, ny, nx), dtype=np.uint8)
+    f[:, -1, :] = 0x81
+    f[:, :, 0] = f[:, :, -1] = 0x80
+    f[:, 0, :] = 0x80
+    f[0, :, :] = f[-1, :, :] = 0x80
+    return f.reshape(-1)
+
+
+def run_leg(hip, name, lattice, storage, n, tau, steps, warmup):
+    log(f"leg {name}: {lattice} {n}^3 {storage} cavity, warmup {warmup}, steps {steps}")
+    q = {"d3q19": 19, "d3q27": 27}[lattice]
+    s = hip.EsotericPullHipSolver(lattice, n, n, n, tau, storage=storage)
+    flags = cavity_flags(n, n, n)
+    s.set_flags(flags)
+    s.set_wall_velocity(1, (0.1, 0.0, 0.0))
+    s.initialize_uniform()
+    mass0 = s.total_mass()
+    s.step(warmup)
+    s.wait()
+    t0 = time.perf_counter()
+    gpu_ms = s.step_timed(steps)  # events on the solver's stream; synchronises at the end
+    s.wait()
+    wall = time.perf_counter() - t0
+    mass1, ke, vmax = s.diagnostics()
+    cells = n ** 3
+    fluid = int(np.count_nonzero(flags == 0))
+    bytes_launch = fluid * 2 * q * BYTES_PER_POP[storage] + cells * 1
+    launch_s = gpu_ms * 1e-3 / steps
+    gbs = bytes_launch / launch_s / 1e9
+    res = {
+        "lattice": lattice.upper(), "storage": storage, "n": n, "cells": cells, "fluid_cells": fluid,
+        "steps": steps, "warmup": warmup, "seconds": wall, "ms_per_step": wall * 1e3 / steps,
+        "mlups": cells * steps / wall / 1e6, "kernel_ms": gpu_ms / steps,
+        "kernel_mlups": cells / launch_s / 1e6, "algorithmic_bytes_per_launch": bytes_launch,
+        "bytes_per_fluid_cell_update": 2 * q * BYTES_PER_POP[storage],
+        "achieved_gbs": gbs, "roofline_frac": gbs / PEAK_GBS,
+        "mass_rel_drift": abs(mass1 - mass0) / mass0, "max_speed": vmax,
+        "finite": bool(np.isfinite(mass1) and np.isfinite(vmax)),
+    }
+    log(f"leg {name}: {res['mlups']:.0f} MLUPS, kernel {res['kernel_ms']:.3f} ms, {gbs:.0f} GB/s "
+        f"({100 * gbs / PEAK_GBS:.1f}% of 8 TB/s), mass drift {res['mass_rel_drift']:.2e}")
+    s.close()
+    return res
+
+
+def copy_probe():
+    import ctypes as C
+
+    lib = C.CDLL(str(ROOT / "cfd_solvers_amd" / "libhbm_probe.so"))
+    lib.hbm_probe_copy.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_float)]
+    ms = C.c_float()
+    nbytes = 1 << 31
+    if lib.hbm_probe_copy(nbytes, 20, C.byref(ms)) != 0:
+        return None
+    gbs = 2 * nbytes / (ms.value * 1e-3) / 1e9
+    log(f"copy probe: 2 GiB copy {ms.value:.3f} ms -> {gbs:.0f} GB/s (read + write)")
+    return {"bytes_per_copy": 2 * nbytes, "ms": ms.value, "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS}
+
+
+def cpu_baseline(target_seconds=15.0):
+    """The reference's EsotericPullSolver<D3Q19> (oracle/_ref, built from the reference sources
+    like its Release build) on the host cores: 256^3 periodic Taylor-Green, cfd-bench protocol
+    (apps/cfd-bench.cpp:174-191) on a bounded number of steps.  Falls back to the CPU
+    restatement (kind "port") where the reference build is absent."""
+    threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16")))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    import oracle
+
+    n = 256
+    if oracle.ref_available(native=True):
+        kind = "reference"
+        s = oracle.RefEsoteric(19, n, n, n, 0.6, native=True)
+        s.init_taylor_green(0.02)
+    else:
+        kind = "port"
+        s = oracle.Oracle(19, n, n, n, 0.6)
+        s.init_taylor_green(0.02)
+    s.mass()
+    t0 = time.perf_counter()
+    s.step(1)  # warmup, also sizes the sample
+    one = time.perf_counter() - t0
+    steps = int(max(1, min(50, target_seconds / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    s.step(steps)
+    dt = time.perf_counter() - t0
+    s.mass()
+    mlups = n ** 3 * steps / dt / 1e6
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
+    except OSError:
+        pass
+    log(f"cpu baseline ({kind}): {mlups:.1f} MLUPS on {threads} threads, {steps} steps in {dt:.1f} s")
+    return {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": kind,
+            "sample": f"D3Q19 256^3 periodic Taylor-Green (A 0.02, tau 0.6), 1 warmup + {steps} timed steps, "
+                      f"FP32, {threads} OpenMP threads, {cpu}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--legs", default="fp32,fp16,d3q27")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-probe", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
+              "sharding of the lattice is a design (DESIGN.md), not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import cfd_solvers_amd as hip
+
+    legs = {}
+    wanted = [l.strip() for l in args.legs.split(",") if l.strip()]
+    table = {"fp32": ("d3q19", "fp32"), "fp16": ("d3q19", "fp16s"), "d3q27": ("d3q27", "fp32"),
+             "fp64": ("d3q19", "fp64")}
+    for name in wanted:
+        lattice, storage = table[name]
+        legs[name] = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
+    probe = None if args.no_probe else copy_probe()
+    head = legs.get("fp32") or next(iter(legs.values()))
+
+    traffic = None
+    if TRAFFIC_FILE.exists() and args.n == 256:
+        try:
+            traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    result = {
+        "metric": METRIC,
+        "value": head["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"D3Q19 {args.n}^3 lid-driven cavity, walls on 5 faces, lid u=(0.1,0,0), "
+                               f"tau 0.55, FP32 populations, uint8 flags, esoteric-pull in place",
+                   "lattice": "D3Q19", "nx": args.n, "ny": args.n, "nz": args.n, "storage": "fp32",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": head["achieved_gbs"] / PEAK_GBS, "traffic": traffic,
+                     "kernel": "stream_collide<19, float, bounded>",
+                     "bytes_per_launch": head["algorithmic_bytes_per_launch"]},
+        "legs": legs,
+        "copy_probe": probe,
+    }
+    if "fp16" in legs and "fp32" in legs:
+        result["fp16_over_fp32"] = legs["fp16"]["mlups"] / legs["fp32"]["mlups"]
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

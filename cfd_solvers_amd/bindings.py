@@ -1,0 +1,290 @@
+"""ctypes bindings for the gfx950 LBM C ABI (include/cfd_lbm/c_api.h).
+
+Mirrors the reference's dependency-free ctypes layer (python/cfd_solvers/bindings.py:5-30):
+``load(path=None)`` honours ``$CFD_SOLVERS_LIBRARY`` and raises ``RuntimeError`` with the C
+ABI's error text whenever an entry point returns non-zero.  When no path is given the in-tree
+``libcfd_lbm_hip.so`` is loaded; if it has not been built, loading fails loudly -- there is no
+CPU fallback.
+
+``EsotericPullHipSolver`` follows the method names of the reference's solver classes
+(include/cfd/solvers/lbm/esoteric_pull.hpp:104-278, esoteric_pull_sycl.hpp:18-475), with numpy
+arrays of shape (nz, ny, nx) for fields and (q, cells) for populations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+DEFAULT_LIBRARY = PKG / "libcfd_lbm_hip.so"
+
+LATTICES = {"d2q9": 9, "d3q19": 19, "d3q27": 27}
+STORAGE = {"fp32": 0, "fp16s": 1, "fp64": 2}
+FLUID, VELOCITY_INLET, PRESSURE_OUTLET, WALL = 0x00, 0x01, 0x02, 0x80
+
+_ERR = 1024
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+# Every exported symbol and its argument types (the error buffer pair is appended).
+_SIGNATURES = {
+    "cfd_lbm_create": [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_float, C.c_float,
+                       C.c_float, C.c_int, C.POINTER(C.c_void_p)],
+    "cfd_lbm_destroy": [C.c_void_p],
+    "cfd_lbm_initialize_uniform": [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float],
+    "cfd_lbm_initialize_taylor_green": [C.c_void_p, C.c_float, C.c_float, C.c_uint64],
+    "cfd_lbm_initialize_fields": [C.c_void_p, _f32p, _f32p, _f32p, _f32p],
+    "cfd_lbm_upload_populations": [C.c_void_p, _f32p],
+    "cfd_lbm_download_populations": [C.c_void_p, _f32p],
+    "cfd_lbm_set_flags": [C.c_void_p, C.c_void_p],
+    "cfd_lbm_set_wall_velocity": [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float],
+    "cfd_lbm_set_porosity": [C.c_void_p, C.c_void_p],
+    "cfd_lbm_set_inlet_outlet": [C.c_void_p, C.c_float, C.c_float],
+    "cfd_lbm_set_local_acceleration": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "cfd_lbm_step": [C.c_void_p, C.c_uint64],
+    "cfd_lbm_step_timed": [C.c_void_p, C.c_uint64, C.POINTER(C.c_float)],
+    "cfd_lbm_synchronize": [C.c_void_p],
+    "cfd_lbm_download_macroscopic": [C.c_void_p, _f32p, _f32p, _f32p, _f32p],
+    "cfd_lbm_download_macroscopic_f64": [C.c_void_p, _f64p, _f64p, _f64p, _f64p],
+    "cfd_lbm_refresh_device_macroscopic": [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4,
+    "cfd_lbm_total_mass": [C.c_void_p, C.POINTER(C.c_double)],
+    "cfd_lbm_diagnostics": [C.c_void_p] + [C.POINTER(C.c_double)] * 3,
+    "cfd_lbm_time_step": [C.c_void_p, C.POINTER(C.c_uint64)],
+    "cfd_lbm_cells": [C.c_void_p, C.POINTER(C.c_size_t)],
+    "cfd_lbm_population_bytes": [C.c_void_p, C.POINTER(C.c_size_t)],
+    "cfd_lbm_resident_bytes": [C.c_void_p, C.POINTER(C.c_size_t)],
+    "cfd_lbm_device_name": [C.c_void_p, C.c_char_p, C.c_size_t],
+    "cfd_lbm_stream": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "cfd_lbm_transfer_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.c_int],
+    "cfd_lbm_index_map": [C.c_void_p, C.c_int, _i64p, _i64p],
+}
+
+
+class SolverLibrary:
+    """Loaded C ABI; each call raises RuntimeError on a non-zero return code."""
+
+    def __init__(self, path):
+        self.path = Path(path)
+        self._lib = C.CDLL(os.fspath(self.path))
+        self._lib.cfd_lbm_version.restype = C.c_char_p
+        for name, args in _SIGNATURES.items():
+            fn = getattr(self._lib, name)
+            fn.argtypes = list(args) + [C.c_char_p, C.c_size_t]
+            fn.restype = C.c_int
+
+    @property
+    def version(self) -> str:
+        return self._lib.cfd_lbm_version().decode()
+
+    def call(self, name, *args):
+        err = C.create_string_buffer(_ERR)
+        rc = getattr(self._lib, name)(*args, err, _ERR)
+        if rc:
+            raise RuntimeError(f"{name}: {err.value.decode('utf-8', errors='replace')}")
+
+
+_loaded: dict[str, SolverLibrary] = {}
+
+
+def load(path=None) -> SolverLibrary:
+    path = path or os.environ.get("CFD_SOLVERS_LIBRARY") or DEFAULT_LIBRARY
+    path = Path(path)
+    if not path.exists():
+        raise RuntimeError(f"HIP LBM library not found at {path}; build it with `python -m cfd_solvers_amd.build`")
+    key = os.fspath(path.resolve())
+    if key not in _loaded:
+        _loaded[key] = SolverLibrary(path)
+    return _loaded[key]
+
+
+@dataclass
+class MacroscopicFields:
+    rho: np.ndarray
+    ux: np.ndarray
+    uy: np.ndarray
+    uz: np.ndarray
+
+
+class EsotericPullHipSolver:
+    """Device-resident esoteric-pull solver (EsotericPullSolver / EsotericPullSyclSolver API)."""
+
+    def __init__(self, lattice="d3q19", nx=64, ny=64, nz=64, tau=0.6, acceleration=(0.0, 0.0, 0.0),
+                 storage="fp32", library=None):
+        self.lib = load(library)
+        self.q = LATTICES[lattice] if isinstance(lattice, str) else int(lattice)
+        self.storage = storage
+        nz = 1 if self.q == 9 else nz
+        self.shape = (nz, ny, nx)
+        h = C.c_void_p()
+        self.lib.call("cfd_lbm_create", self.q, nx, ny, nz, tau, *map(float, acceleration),
+                      STORAGE[storage], C.byref(h))
+        self._h = h
+        self.cells = nx * ny * nz
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.call("cfd_lbm_destroy", self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # initial conditions -------------------------------------------------------------
+    def initialize_uniform(self, rho=1.0, ux=0.0, uy=0.0, uz=0.0):
+        self.lib.call("cfd_lbm_initialize_uniform", self._h, rho, ux, uy, uz)
+
+    def initialize_taylor_green(self, amplitude=0.03, noise=0.0, seed=0):
+        self.lib.call("cfd_lbm_initialize_taylor_green", self._h, amplitude, noise, seed)
+
+    def initialize_fields(self, rho, ux, uy, uz):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (rho, ux, uy, uz)]
+        self.lib.call("cfd_lbm_initialize_fields", self._h, *arrs)
+
+    def upload_populations(self, logical):
+        a = np.ascontiguousarray(logical, dtype=np.float32).reshape(-1)
+        if a.size != self.q * self.cells:
+            raise ValueError("populations must have q*cells values")
+        self.lib.call("cfd_lbm_upload_populations", self._h, a)
+
+    def logical_populations(self):
+        out = np.empty(self.q * self.cells, dtype=np.float32)
+        self.lib.call("cfd_lbm_download_populations", self._h, out)
+        return out.reshape(self.q, self.cells)
+
+    # boundaries and forcing ---------------------------------------------------------
+    def set_flags(self, flags):
+        if flags is None:
+            self.lib.call("cfd_lbm_set_flags", self._h, None)
+            return
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        if f.size != self.cells:
+            raise ValueError("LBM flag field size mismatch")
+        self.lib.call("cfd_lbm_set_flags", self._h, f.ctypes.data)
+
+    def set_solid_mask(self, mask):
+        m = np.asarray(mask).reshape(-1)
+        self.set_flags(np.where(m != 0, WALL, FLUID).astype(np.uint8))
+
+    def clear_solid_mask(self):
+        self.set_flags(None)
+
+    def set_wall_velocity(self, index, u):
+        self.lib.call("cfd_lbm_set_wall_velocity", self._h, index, *map(float, u))
+
+    def set_porosity(self, gamma):
+        if gamma is None:
+            self.lib.call("cfd_lbm_set_porosity", self._h, None)
+            return
+        g = np.ascontiguousarray(gamma, dtype=np.float32).reshape(-1)
+        if g.size != self.cells:
+            raise ValueError("porosity field size mismatch")
+        self.lib.call("cfd_lbm_set_porosity", self._h, g.ctypes.data)
+
+    def set_inlet_outlet(self, u_peak, rho_out=1.0):
+        self.lib.call("cfd_lbm_set_inlet_outlet", self._h, u_peak, rho_out)
+
+    def set_local_body_acceleration(self, ax, ay, az):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (ax, ay, az)]
+        if any(a.size != self.cells for a in arrs):
+            raise ValueError("local LBM acceleration field size mismatch")
+        self._lacc = arrs
+        self.lib.call("cfd_lbm_set_local_acceleration", self._h, *(a.ctypes.data for a in arrs))
+
+    def clear_local_body_acceleration(self):
+        self.lib.call("cfd_lbm_set_local_acceleration", self._h, None, None, None)
+
+    # stepping -----------------------------------------------------------------------
+    def step(self, count=1):
+        self.lib.call("cfd_lbm_step", self._h, count)
+
+    def step_timed(self, count=1) -> float:
+        ms = C.c_float()
+        self.lib.call("cfd_lbm_step_timed", self._h, count, C.byref(ms))
+        return ms.value
+
+    def wait(self):
+        self.lib.call("cfd_lbm_synchronize", self._h)
+
+    # results ------------------------------------------------------------------------
+    def download_macroscopic(self, f64=False) -> MacroscopicFields:
+        dt = np.float64 if f64 else np.float32
+        out = [np.empty(self.cells, dtype=dt) for _ in range(4)]
+        self.lib.call("cfd_lbm_download_macroscopic_f64" if f64 else "cfd_lbm_download_macroscopic",
+                      self._h, *out)
+        return MacroscopicFields(*(o.reshape(self.shape) for o in out))
+
+    compute_macroscopic = download_macroscopic
+
+    def refresh_device_macroscopic(self):
+        ptrs = [C.c_void_p() for _ in range(4)]
+        self.lib.call("cfd_lbm_refresh_device_macroscopic", self._h, *(C.byref(p) for p in ptrs))
+        return tuple(p.value for p in ptrs)
+
+    def total_mass(self) -> float:
+        m = C.c_double()
+        self.lib.call("cfd_lbm_total_mass", self._h, C.byref(m))
+        return m.value
+
+    mass = total_mass
+
+    def diagnostics(self):
+        m, ke, vmax = C.c_double(), C.c_double(), C.c_double()
+        self.lib.call("cfd_lbm_diagnostics", self._h, C.byref(m), C.byref(ke), C.byref(vmax))
+        return m.value, ke.value, vmax.value
+
+    def kinetic_energy(self) -> float:
+        return self.diagnostics()[1]
+
+    def max_speed(self) -> float:
+        return self.diagnostics()[2]
+
+    @property
+    def time_step(self) -> int:
+        v = C.c_uint64()
+        self.lib.call("cfd_lbm_time_step", self._h, C.byref(v))
+        return v.value
+
+    def population_bytes(self) -> int:
+        v = C.c_size_t()
+        self.lib.call("cfd_lbm_population_bytes", self._h, C.byref(v))
+        return v.value
+
+    def two_lattice_population_bytes(self) -> int:
+        return 2 * self.population_bytes()
+
+    def resident_bytes(self) -> int:
+        v = C.c_size_t()
+        self.lib.call("cfd_lbm_resident_bytes", self._h, C.byref(v))
+        return v.value
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        self.lib.call("cfd_lbm_device_name", self._h, buf, 256)
+        return buf.value.decode()
+
+    def stream(self) -> int:
+        p = C.c_void_p()
+        self.lib.call("cfd_lbm_stream", self._h, C.byref(p))
+        return p.value or 0
+
+    def transfer_stats(self, reset=False):
+        v = (C.c_uint64 * 4)()
+        self.lib.call("cfd_lbm_transfer_stats", self._h, v, int(reset))
+        return {"host_to_device_bytes": v[0], "device_to_host_bytes": v[1],
+                "device_to_device_bytes": v[2], "synchronization_points": v[3]}
+
+    def index_map(self, parity):
+        load = np.empty(self.q * self.cells, dtype=np.int64)
+        store = np.empty(self.q * self.cells, dtype=np.int64)
+        self.lib.call("cfd_lbm_index_map", self._h, parity, load, store)
+        return load.reshape(self.q, self.cells), store.reshape(self.q, self.cells)

@@ -1,0 +1,680 @@
+// lbm_capi.hip -- device-resident solver object behind the C ABI of include/cfd_lbm/c_api.h.
+//
+// Mirrors EsotericPullSyclSolver<D> (include/cfd/solvers/lbm/esoteric_pull_sycl.hpp:18-475):
+// the solver owns one padded q-plane population array in HBM, an in-order HIP stream, lazily
+// allocated device scratch for macroscopic fields and reductions, and a host/device transfer
+// ledger (include/cfd/core/device_residency.hpp:11-42).  Error convention of
+// src/python/c_api.cpp:8-15: 0 ok, 1 std::exception, 2 unknown, message in a caller buffer.
+#include "../../include/cfd_lbm/c_api.h"
+#include "lbm_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using cfd_hip::Geometry;
+using cfd_hip::KernelOps;
+using cfd_hip::LaunchState;
+using cfd_hip::Physics;
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct TransferStats {
+    uint64_t h2d = 0, d2h = 0, d2d = 0, syncs = 0;
+};
+
+template <class T> struct DeviceBuffer {
+    T* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { reset(); }
+    void allocate(size_t n_bytes) {
+        reset();
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, n_bytes), "hipMalloc");
+        ptr = static_cast<T*>(p);
+        bytes = n_bytes;
+    }
+    void reset() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    explicit operator bool() const { return ptr != nullptr; }
+};
+
+}  // namespace
+
+struct cfd_lbm_solver {
+    int q = 19, storage = 0, device = 0;
+    size_t nx = 0, ny = 0, nz = 0, cells = 0;
+    float tau = 0.6F, ax = 0, ay = 0, az = 0;
+    size_t elem = 4, real = 4;
+    Geometry geo{};
+    hipStream_t stream = nullptr;
+    const KernelOps* ops = nullptr;
+    DeviceBuffer<unsigned char> f, flags, solid_rho, gamma, lacc, macro, partial, reduce_out;
+    float wall_u[8][3] = {};
+    float u_peak = 0.0F, rho_out = 1.0F;
+    uint64_t t = 0;
+    mutable TransferStats stats;
+    // Host copy of the Physics<Real> passed by value to every launch.
+    alignas(16) unsigned char physics[sizeof(Physics<double>)] = {};
+
+    cfd_lbm_solver(int lattice, size_t nx_, size_t ny_, size_t nz_, float tau_, float ax_, float ay_, float az_,
+                   int storage_) {
+        if (lattice != 9 && lattice != 19 && lattice != 27)
+            throw std::invalid_argument("lattice must be D2Q9 (9), D3Q19 (19) or D3Q27 (27)");
+        if (storage_ < 0 || storage_ > 2) throw std::invalid_argument("storage must be FP32 (0), FP16S (1) or FP64 (2)");
+        if (nx_ < 2 || ny_ < 2) throw std::invalid_argument("LBM grid requires nx, ny >= 2");
+        if (lattice == 9) nz_ = 1;
+        else if (nz_ < 2) throw std::invalid_argument("3-D LBM grid requires nz >= 2");
+        if (!(tau_ > 0.5F)) throw std::invalid_argument("LBM tau must be > 0.5 for positive viscosity");
+        q = lattice;
+        storage = storage_;
+        nx = nx_;
+        ny = ny_;
+        nz = nz_;
+        cells = nx * ny * nz;
+        if (cells >= (size_t{1} << 32) || nz > 65535U * 256U)
+            throw std::invalid_argument("lattice exceeds 2^32 cells (32-bit cell indices)");
+        tau = tau_;
+        ax = ax_;
+        ay = ay_;
+        az = az_;
+        elem = storage == 0 ? 4 : (storage == 1 ? 2 : 8);
+        real = storage == 2 ? 8 : 4;
+
+        // Launch geometry: 256-thread tiles, bx a power of two in [32, 256] covering x.
+        uint32_t bx = 32;
+        while (bx < nx && bx < 256) bx <<= 1;
+        geo.nx = static_cast<uint32_t>(nx);
+        geo.ny = static_cast<uint32_t>(ny);
+        geo.nz = static_cast<uint32_t>(nz);
+        geo.bx = bx;
+        geo.by = 256U / bx;
+        geo.tiles_x = static_cast<uint32_t>((nx + bx - 1) / bx);
+        geo.tiles_y = static_cast<uint32_t>((ny + geo.by - 1) / geo.by);
+        const uint64_t tiles = static_cast<uint64_t>(geo.tiles_x) * geo.tiles_y * nz;
+        if (tiles >= (uint64_t{1} << 31)) throw std::invalid_argument("lattice too large for one launch");
+        geo.tiles = static_cast<uint32_t>(tiles);
+        geo.xcd_remap = (geo.tiles % 8U == 0U && geo.tiles >= 64U) ? 1U : 0U;
+        geo.stride = (static_cast<uint64_t>(cells) + 255U) / 256U * 256U;  // 1 KiB-aligned fp32 planes
+
+        ops = cfd_hip::kernel_ops(q, storage);
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (count < 1) throw std::runtime_error("no HIP device available");
+        hip_check(hipGetDevice(&device), "hipGetDevice");
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        try {
+            f.allocate(static_cast<size_t>(q) * geo.stride * elem);
+            initialize_uniform(1.0F, 0.0F, 0.0F, 0.0F);
+        } catch (...) {
+            (void)hipStreamDestroy(stream);
+            throw;
+        }
+    }
+
+    ~cfd_lbm_solver() {
+        (void)hipSetDevice(device);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void use_device() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
+    bool odd() const { return (t & 1U) != 0U; }
+    bool bounded() const { return static_cast<bool>(flags); }
+
+    template <class Real> void build_physics() {
+        Physics<Real> p{};
+        p.omega = Real(1) / static_cast<Real>(tau);
+        p.ax = static_cast<Real>(ax);
+        p.ay = static_cast<Real>(ay);
+        p.az = static_cast<Real>(az);
+        p.u_peak = static_cast<Real>(u_peak);
+        p.rho_out = static_cast<Real>(rho_out);
+        for (int k = 0; k < 8; ++k)
+            for (int c = 0; c < 3; ++c) p.wall_u[k][c] = static_cast<Real>(wall_u[k][c]);
+        p.flags = flags.ptr;
+        p.gamma = reinterpret_cast<const Real*>(gamma.ptr);
+        p.lax = reinterpret_cast<const Real*>(lacc.ptr);
+        p.solid_rho = reinterpret_cast<const Real*>(solid_rho.ptr);
+        p.cells = static_cast<uint32_t>(cells);
+        std::memcpy(physics, &p, sizeof(p));
+    }
+
+    LaunchState state() {
+        if (storage == 2) build_physics<double>();
+        else build_physics<float>();
+        return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells)};
+    }
+
+    void sync() const {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        ++stats.syncs;
+    }
+
+    void h2d(void* dst, const void* src, size_t bytes) {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+        stats.h2d += bytes;
+    }
+    void d2h(void* dst, const void* src, size_t bytes) {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+        stats.d2h += bytes;
+    }
+
+    // Host float array -> device Real array (converted on the host; float -> double is exact).
+    void upload_real(DeviceBuffer<unsigned char>& dst, const float* const* parts, size_t nparts) {
+        const size_t n = cells * nparts;
+        if (!dst || dst.bytes != n * real) dst.allocate(n * real);
+        if (real == 4) {
+            for (size_t k = 0; k < nparts; ++k) h2d(dst.ptr + k * cells * 4, parts[k], cells * 4);
+            sync();
+        } else {
+            std::vector<double> tmp(n);
+            for (size_t k = 0; k < nparts; ++k)
+                for (size_t i = 0; i < cells; ++i) tmp[k * cells + i] = static_cast<double>(parts[k][i]);
+            h2d(dst.ptr, tmp.data(), n * 8);
+            sync();
+        }
+    }
+
+    // After the plain layout has been (re)written: freeze wall densities, prepare bounce-back slots.
+    void prepare_walls() {
+        if (!bounded()) return;
+        const LaunchState st = state();
+        ops->freeze(st, odd(), flags.ptr, nullptr, solid_rho.ptr);
+        ops->bounce_back(st, odd(), true, flags.ptr);
+    }
+
+    void run_init(int mode, double rho, double ux, double uy, double uz, double amplitude, double noise,
+                  uint64_t seed, const void* tables, const void* fields) {
+        t = 0;
+        unsigned char args[256];
+        if (cfd_hip::init_args_size(storage) > sizeof(args)) throw std::logic_error("init args buffer too small");
+        cfd_hip::fill_init_args(storage, args, mode, rho, ux, uy, uz, amplitude, noise, seed, tables, fields);
+        const LaunchState st = state();
+        ops->initialize(st, args, flags.ptr);
+        prepare_walls();
+    }
+
+    void initialize_uniform(float rho, float ux, float uy, float uz) {
+        if (!(rho > 0.0F) || !std::isfinite(rho) || !std::isfinite(ux) || !std::isfinite(uy) || !std::isfinite(uz))
+            throw std::invalid_argument("LBM uniform state must be finite with positive density");
+        use_device();
+        run_init(0, rho, ux, uy, uz, 0.0, 0.0, 0, nullptr, nullptr);
+    }
+
+    template <class Real> std::vector<Real> taylor_green_tables() const {
+        // esoteric_pull.hpp:121-137: the trigonometric factors of x, y and z, evaluated on the
+        // host with the same libm calls as the reference; the device multiplies them.
+        const Real two_pi = Real(2) * static_cast<Real>(3.14159265358979323846);
+        std::vector<Real> tab(2 * nx + 2 * ny + nz);
+        for (size_t x = 0; x < nx; ++x) {
+            const Real xf = (static_cast<Real>(x) + Real(0.5)) / static_cast<Real>(nx);
+            tab[x] = std::sin(two_pi * xf);
+            tab[nx + x] = std::cos(two_pi * xf);
+        }
+        for (size_t y = 0; y < ny; ++y) {
+            const Real yf = (static_cast<Real>(y) + Real(0.5)) / static_cast<Real>(ny);
+            tab[2 * nx + y] = std::sin(two_pi * yf);
+            tab[2 * nx + ny + y] = std::cos(two_pi * yf);
+        }
+        for (size_t z = 0; z < nz; ++z) {
+            if (q == 9) {
+                tab[2 * nx + 2 * ny + z] = Real(1);
+            } else {
+                const Real zf = (static_cast<Real>(z) + Real(0.5)) / static_cast<Real>(nz);
+                tab[2 * nx + 2 * ny + z] = std::cos(two_pi * zf);
+            }
+        }
+        return tab;
+    }
+
+    void initialize_taylor_green(float amplitude, float noise, uint64_t seed) {
+        if (!std::isfinite(amplitude) || !std::isfinite(noise))
+            throw std::invalid_argument("Taylor-Green amplitude and noise must be finite");
+        use_device();
+        DeviceBuffer<unsigned char> tables;
+        if (real == 4) {
+            const auto tab = taylor_green_tables<float>();
+            tables.allocate(tab.size() * 4);
+            h2d(tables.ptr, tab.data(), tab.size() * 4);
+        } else {
+            const auto tab = taylor_green_tables<double>();
+            tables.allocate(tab.size() * 8);
+            h2d(tables.ptr, tab.data(), tab.size() * 8);
+        }
+        run_init(1, 1.0, 0.0, 0.0, 0.0, amplitude, noise, seed, tables.ptr, nullptr);
+        sync();
+    }
+
+    void initialize_fields(const float* rho, const float* ux, const float* uy, const float* uz) {
+        if (!rho || !ux || !uy || !uz) throw std::invalid_argument("initialize_fields requires rho, ux, uy, uz");
+        use_device();
+        DeviceBuffer<unsigned char> fields;
+        const float* parts[4] = {rho, ux, uy, uz};
+        upload_real(fields, parts, 4);
+        run_init(2, 0, 0, 0, 0, 0, 0, 0, nullptr, fields.ptr);
+        sync();
+    }
+
+    void upload_populations(const float* logical) {
+        if (!logical) throw std::invalid_argument("upload_populations requires q*cells values");
+        use_device();
+        DeviceBuffer<unsigned char> buf;
+        std::vector<const float*> parts(q);
+        for (int d = 0; d < q; ++d) parts[d] = logical + static_cast<size_t>(d) * cells;
+        upload_real(buf, parts.data(), q);
+        const LaunchState st = state();
+        ops->upload(st, odd(), buf.ptr);
+        prepare_walls();
+        sync();
+    }
+
+    void download_populations(float* logical) {
+        if (!logical) throw std::invalid_argument("download_populations requires q*cells values");
+        use_device();
+        DeviceBuffer<unsigned char> buf;
+        buf.allocate(static_cast<size_t>(q) * cells * real);
+        const LaunchState st = state();
+        ops->download(st, odd(), bounded(), buf.ptr);
+        const size_t n = static_cast<size_t>(q) * cells;
+        if (real == 4) {
+            d2h(logical, buf.ptr, n * 4);
+            sync();
+        } else {
+            std::vector<double> tmp(n);
+            d2h(tmp.data(), buf.ptr, n * 8);
+            sync();
+            for (size_t i = 0; i < n; ++i) logical[i] = static_cast<float>(tmp[i]);
+        }
+    }
+
+    void set_flags(const uint8_t* host_flags) {
+        use_device();
+        DeviceBuffer<unsigned char> fresh;
+        if (host_flags) {
+            for (size_t i = 0; i < cells; ++i) {
+                const uint8_t v = host_flags[i];
+                if (v != 0x00 && v != 0x01 && v != 0x02 && (v & 0x80U) == 0U)
+                    throw std::invalid_argument("cell flags must be 0 (fluid), 1 (inlet), 2 (outlet) or 0x80|k (wall)");
+                if ((v & 0x80U) != 0U && (v & 0x78U) != 0U)
+                    throw std::invalid_argument("wall flags use bits 0-2 for the wall-velocity index only");
+            }
+            fresh.allocate(cells);
+            h2d(fresh.ptr, host_flags, cells);
+        }
+        LaunchState st = state();
+        if (bounded()) {
+            ops->bounce_back(st, odd(), false, flags.ptr);  // back to the plain layout
+            ops->thaw(st, odd(), flags.ptr, fresh.ptr, solid_rho.ptr);
+        }
+        if (!host_flags) {
+            sync();
+            flags.reset();
+            solid_rho.reset();
+            return;
+        }
+        if (!solid_rho) solid_rho.allocate(cells * real);
+        ops->freeze(st, odd(), fresh.ptr, flags.ptr, solid_rho.ptr);
+        ops->bounce_back(st, odd(), true, fresh.ptr);
+        sync();
+        std::swap(flags.ptr, fresh.ptr);
+        std::swap(flags.bytes, fresh.bytes);
+    }
+
+    void set_porosity(const float* g) {
+        use_device();
+        if (!g) {
+            sync();
+            gamma.reset();
+            return;
+        }
+        for (size_t i = 0; i < cells; ++i)
+            if (!(g[i] >= 0.0F && g[i] <= 1.0F)) throw std::invalid_argument("porosity gamma must lie in [0, 1]");
+        const float* parts[1] = {g};
+        upload_real(gamma, parts, 1);
+    }
+
+    void set_local_acceleration(const float* lx, const float* ly, const float* lz) {
+        use_device();
+        if (!lx && !ly && !lz) {
+            sync();
+            lacc.reset();
+            return;
+        }
+        if (!lx || !ly || !lz) throw std::invalid_argument("local LBM acceleration field size mismatch");
+        const float* parts[3] = {lx, ly, lz};
+        upload_real(lacc, parts, 3);
+    }
+
+    void step(uint64_t count) {
+        use_device();
+        const bool b = bounded();
+        LaunchState st = state();
+        for (uint64_t k = 0; k < count; ++k) {
+            ops->step(st, odd(), b);
+            ++t;
+        }
+    }
+
+    float step_timed(uint64_t count) {
+        use_device();
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        float ms = 0.0F;
+        try {
+            hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+            step(count);
+            hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            ++stats.syncs;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return ms;
+    }
+
+    void refresh_macroscopic() {
+        use_device();
+        if (!macro) macro.allocate(4 * cells * real);
+        const LaunchState st = state();
+        ops->macroscopic(st, odd(), bounded(), macro.ptr);
+    }
+
+    template <class Out> void download_macroscopic(Out* rho, Out* ux, Out* uy, Out* uz) {
+        if (!rho || !ux || !uy || !uz) throw std::invalid_argument("download_macroscopic requires four arrays");
+        refresh_macroscopic();
+        Out* outs[4] = {rho, ux, uy, uz};
+        if (real == sizeof(Out)) {
+            for (int k = 0; k < 4; ++k) d2h(outs[k], macro.ptr + k * cells * real, cells * real);
+            sync();
+            return;
+        }
+        if (real == 4) {
+            std::vector<float> tmp(4 * cells);
+            d2h(tmp.data(), macro.ptr, 4 * cells * 4);
+            sync();
+            for (int k = 0; k < 4; ++k)
+                for (size_t i = 0; i < cells; ++i) outs[k][i] = static_cast<Out>(tmp[k * cells + i]);
+        } else {
+            std::vector<double> tmp(4 * cells);
+            d2h(tmp.data(), macro.ptr, 4 * cells * 8);
+            sync();
+            for (int k = 0; k < 4; ++k)
+                for (size_t i = 0; i < cells; ++i) outs[k][i] = static_cast<Out>(tmp[k * cells + i]);
+        }
+    }
+
+    void diagnostics(double* out3) {
+        use_device();
+        if (!partial) partial.allocate(static_cast<size_t>(geo.tiles) * 3 * sizeof(double));
+        if (!reduce_out) reduce_out.allocate(3 * sizeof(double));
+        const LaunchState st = state();
+        ops->diagnostics(st, odd(), bounded(), reinterpret_cast<double*>(partial.ptr),
+                         reinterpret_cast<double*>(reduce_out.ptr));
+        d2h(out3, reduce_out.ptr, 3 * sizeof(double));
+        sync();
+    }
+
+    void index_map(int parity, int64_t* load, int64_t* store) {
+        use_device();
+        DeviceBuffer<unsigned char> buf;
+        const size_t n = static_cast<size_t>(q) * cells;
+        buf.allocate(2 * n * sizeof(int64_t));
+        const LaunchState st = state();
+        ops->index_map(st, (parity & 1) != 0, bounded(), flags.ptr, reinterpret_cast<int64_t*>(buf.ptr),
+                       reinterpret_cast<int64_t*>(buf.ptr) + n);
+        d2h(load, buf.ptr, n * sizeof(int64_t));
+        d2h(store, buf.ptr + n * sizeof(int64_t), n * sizeof(int64_t));
+        sync();
+    }
+
+    size_t population_bytes() const { return static_cast<size_t>(q) * cells * elem; }
+    size_t resident_bytes() const {
+        return f.bytes + flags.bytes + solid_rho.bytes + gamma.bytes + lacc.bytes + macro.bytes + partial.bytes +
+               reduce_out.bytes;
+    }
+};
+
+namespace {
+
+void copy_error(char* out, size_t cap, const char* text) {
+    if (!out || cap == 0U) return;
+    const size_t n = std::min(cap - 1U, std::strlen(text));
+    std::memcpy(out, text, n);
+    out[n] = '\0';
+}
+
+template <class F> int guarded(char* error, size_t cap, F&& fn) {
+    try {
+        fn();
+        return 0;
+    } catch (const std::exception& e) {
+        copy_error(error, cap, e.what());
+        return 1;
+    } catch (...) {
+        copy_error(error, cap, "unknown C++ exception");
+        return 2;
+    }
+}
+
+cfd_lbm_solver& need(cfd_lbm_solver* s) {
+    if (!s) throw std::invalid_argument("null solver handle");
+    return *s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cfd_lbm_version(void) { return "cfd_lbm_hip 0.1.0 (gfx950)"; }
+
+int cfd_lbm_create(int lattice, size_t nx, size_t ny, size_t nz, float tau, float ax, float ay, float az, int storage,
+                   cfd_lbm_solver** out, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (!out) throw std::invalid_argument("null output handle");
+        *out = nullptr;
+        *out = new cfd_lbm_solver(lattice, nx, ny, nz, tau, ax, ay, az, storage);
+    });
+}
+
+int cfd_lbm_destroy(cfd_lbm_solver* s, char* error, size_t cap) {
+    return guarded(error, cap, [&] { delete s; });
+}
+
+int cfd_lbm_initialize_uniform(cfd_lbm_solver* s, float rho, float ux, float uy, float uz, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        need(s).initialize_uniform(rho, ux, uy, uz);
+        s->sync();
+    });
+}
+
+int cfd_lbm_initialize_taylor_green(cfd_lbm_solver* s, float amplitude, float noise, uint64_t seed, char* error,
+                                    size_t cap) {
+    return guarded(error, cap, [&] { need(s).initialize_taylor_green(amplitude, noise, seed); });
+}
+
+int cfd_lbm_initialize_fields(cfd_lbm_solver* s, const float* rho, const float* ux, const float* uy, const float* uz,
+                              char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).initialize_fields(rho, ux, uy, uz); });
+}
+
+int cfd_lbm_upload_populations(cfd_lbm_solver* s, const float* logical, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).upload_populations(logical); });
+}
+
+int cfd_lbm_download_populations(cfd_lbm_solver* s, float* logical, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).download_populations(logical); });
+}
+
+int cfd_lbm_set_flags(cfd_lbm_solver* s, const uint8_t* flags, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).set_flags(flags); });
+}
+
+int cfd_lbm_set_wall_velocity(cfd_lbm_solver* s, int index, float ux, float uy, float uz, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (index < 1 || index > 7) throw std::invalid_argument("wall velocity index must be 1..7");
+        need(s).wall_u[index][0] = ux;
+        s->wall_u[index][1] = uy;
+        s->wall_u[index][2] = uz;
+    });
+}
+
+int cfd_lbm_set_porosity(cfd_lbm_solver* s, const float* gamma, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).set_porosity(gamma); });
+}
+
+int cfd_lbm_set_inlet_outlet(cfd_lbm_solver* s, float u_peak, float rho_out, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (!std::isfinite(u_peak) || !(rho_out > 0.0F)) throw std::invalid_argument("need finite u_peak and rho_out > 0");
+        if (need(s).ny < 3 || (s->q != 9 && s->nz < 3))
+            throw std::invalid_argument("inlet profile needs at least one interior cell between walls");
+        s->u_peak = u_peak;
+        s->rho_out = rho_out;
+    });
+}
+
+int cfd_lbm_set_local_acceleration(cfd_lbm_solver* s, const float* ax, const float* ay, const float* az, char* error,
+                                   size_t cap) {
+    return guarded(error, cap, [&] { need(s).set_local_acceleration(ax, ay, az); });
+}
+
+int cfd_lbm_step(cfd_lbm_solver* s, uint64_t count, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).step(count); });
+}
+
+int cfd_lbm_step_timed(cfd_lbm_solver* s, uint64_t count, float* ms, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        const float v = need(s).step_timed(count);
+        if (ms) *ms = v;
+    });
+}
+
+int cfd_lbm_synchronize(cfd_lbm_solver* s, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).sync(); });
+}
+
+int cfd_lbm_download_macroscopic(cfd_lbm_solver* s, float* rho, float* ux, float* uy, float* uz, char* error,
+                                 size_t cap) {
+    return guarded(error, cap, [&] { need(s).download_macroscopic<float>(rho, ux, uy, uz); });
+}
+
+int cfd_lbm_download_macroscopic_f64(cfd_lbm_solver* s, double* rho, double* ux, double* uy, double* uz, char* error,
+                                     size_t cap) {
+    return guarded(error, cap, [&] { need(s).download_macroscopic<double>(rho, ux, uy, uz); });
+}
+
+int cfd_lbm_refresh_device_macroscopic(cfd_lbm_solver* s, void** rho, void** ux, void** uy, void** uz, char* error,
+                                       size_t cap) {
+    return guarded(error, cap, [&] {
+        need(s).refresh_macroscopic();
+        unsigned char* base = s->macro.ptr;
+        const size_t plane = s->cells * s->real;
+        if (rho) *rho = base;
+        if (ux) *ux = base + plane;
+        if (uy) *uy = base + 2 * plane;
+        if (uz) *uz = base + 3 * plane;
+    });
+}
+
+int cfd_lbm_total_mass(cfd_lbm_solver* s, double* mass, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        double out[3];
+        need(s).diagnostics(out);
+        if (mass) *mass = out[0];
+    });
+}
+
+int cfd_lbm_diagnostics(cfd_lbm_solver* s, double* mass, double* ke, double* vmax, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        double out[3];
+        need(s).diagnostics(out);
+        if (mass) *mass = out[0];
+        if (ke) *ke = out[1];
+        if (vmax) *vmax = out[2];
+    });
+}
+
+int cfd_lbm_time_step(cfd_lbm_solver* s, uint64_t* out, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (out) *out = need(s).t;
+    });
+}
+
+int cfd_lbm_cells(cfd_lbm_solver* s, size_t* out, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (out) *out = need(s).cells;
+    });
+}
+
+int cfd_lbm_population_bytes(cfd_lbm_solver* s, size_t* out, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (out) *out = need(s).population_bytes();
+    });
+}
+
+int cfd_lbm_resident_bytes(cfd_lbm_solver* s, size_t* out, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (out) *out = need(s).resident_bytes();
+    });
+}
+
+int cfd_lbm_device_name(cfd_lbm_solver* s, char* name, size_t name_cap, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, need(s).device), "hipGetDeviceProperties");
+        std::string text = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+        copy_error(name, name_cap, text.c_str());
+    });
+}
+
+int cfd_lbm_stream(cfd_lbm_solver* s, void** stream, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (stream) *stream = need(s).stream;
+    });
+}
+
+int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int reset, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        auto& st = need(s).stats;
+        if (stats4) {
+            stats4[0] = st.h2d;
+            stats4[1] = st.d2h;
+            stats4[2] = st.d2d;
+            stats4[3] = st.syncs;
+        }
+        if (reset) st = TransferStats{};
+    });
+}
+
+int cfd_lbm_index_map(cfd_lbm_solver* s, int parity, int64_t* load, int64_t* store, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (!load || !store) throw std::invalid_argument("index_map requires two q*cells arrays");
+        need(s).index_map(parity, load, store);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,382 @@
+// lbm_device.hpp -- device building blocks of the gfx950 esoteric-pull stream-collide.
+//
+// Everything here is written for one thread per cell, x fastest across the 64-lane wavefront,
+// populations in structure-of-arrays planes F[d][n] (d-major, one padded plane per direction).
+// Per-cell state lives in registers: the q loaded populations, the 9 neighbour row bases and 3
+// wrapped x coordinates; no LDS, no division by grid sizes in the cell body.
+//
+// Reference behaviour followed (paths relative to the xsession/cfd_solvers root):
+//   velocity sets ............. include/cfd/solvers/lbm/descriptors.hpp:31-146
+//   index map (load/store) .... include/cfd/solvers/lbm/esoteric_pull.hpp:349-382
+//   equilibrium / Guo ......... esoteric_pull.hpp:48-70
+//   macroscopic accumulation .. esoteric_pull.hpp:384-406
+//   BGK collision ............. esoteric_pull.hpp:408-432
+// Arithmetic keeps the reference's association order; the library is compiled with
+// -ffp-contract=off, so an FP32 step reproduces the reference's FP32 step bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+#include <utility>
+
+namespace cfd_hip {
+
+// ------------------------------------------------------------------ velocity sets
+template <int Q> struct Lattice;
+template <> struct Lattice<9> {
+    static constexpr int dims = 2;
+    static constexpr int cx[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
+    static constexpr int cy[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+    static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    static constexpr float w[9] = {4.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F, 1.0F / 9.0F,
+                                   1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+};
+template <> struct Lattice<19> {
+    static constexpr int dims = 3;
+    static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+    static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+    static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+    static constexpr float w[19] = {1.0F / 3.0F,  1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 18.0F,
+                                    1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
+                                    1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
+                                    1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+};
+template <> struct Lattice<27> {
+    static constexpr int dims = 3;
+    static constexpr int cx[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0,
+                                   1, -1, 1, -1, 1, -1, 1, -1};
+    static constexpr int cy[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1,
+                                   1, -1, 1, -1, -1, 1, -1, 1};
+    static constexpr int cz[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1,
+                                   1, -1, -1, 1, 1, -1, -1, 1};
+    static constexpr float w[27] = {
+        8.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,
+        1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,
+        1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 216.0F, 1.0F / 216.0F,
+        1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F};
+};
+
+constexpr int opposite(int i) { return i == 0 ? 0 : ((i & 1) != 0 ? i + 1 : i - 1); }
+
+// ------------------------------------------------------------------ compile-time loops
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// ------------------------------------------------------------------ flags
+constexpr uint8_t kFluid = 0x00;
+constexpr uint8_t kInlet = 0x01;
+constexpr uint8_t kOutlet = 0x02;
+constexpr uint8_t kWall = 0x80;
+__device__ __forceinline__ bool is_wall(uint8_t f) { return (f & kWall) != 0; }
+
+// ------------------------------------------------------------------ population storage
+// fp32: the value itself.  fp16s: binary16 of (f - w_d) * 4096 (round to nearest even, the
+// hardware conversion); the shift keeps small deviations from equilibrium in the normal range.
+// fp64: verification build with double collision arithmetic.
+constexpr float kHalfScale = 4096.0F;
+constexpr float kHalfInvScale = 1.0F / 4096.0F;
+
+template <class T> struct Storage;
+template <> struct Storage<float> {
+    using Real = float;
+    template <int Q, int D> __device__ __forceinline__ static float decode(float v) { return v; }
+    template <int Q, int D> __device__ __forceinline__ static float encode(float v) { return v; }
+};
+template <> struct Storage<_Float16> {
+    using Real = float;
+    template <int Q, int D> __device__ __forceinline__ static float decode(_Float16 h) {
+        return static_cast<float>(h) * kHalfInvScale + Lattice<Q>::w[D];
+    }
+    template <int Q, int D> __device__ __forceinline__ static _Float16 encode(float v) {
+        return static_cast<_Float16>((v - Lattice<Q>::w[D]) * kHalfScale);
+    }
+};
+template <> struct Storage<double> {
+    using Real = double;
+    template <int Q, int D> __device__ __forceinline__ static double decode(double v) { return v; }
+    template <int Q, int D> __device__ __forceinline__ static double encode(double v) { return v; }
+};
+
+// ------------------------------------------------------------------ launch geometry
+// A block is a bx*by = 256-thread tile of one z plane (bx spans x, multiple of 32).  Tiles are
+// numbered x-fastest; with xcd_remap the hardware's round-robin block->XCD dispatch is undone so
+// each XCD (and its L2) walks one contiguous slab of tiles.
+struct Geometry {
+    uint32_t nx, ny, nz;
+    uint32_t bx, by;
+    uint32_t tiles_x, tiles_y, tiles;
+    uint32_t xcd_remap;
+    uint64_t stride;  // elements between consecutive direction planes
+};
+
+__device__ __forceinline__ bool cell_coords(const Geometry& g, uint32_t& x, uint32_t& y, uint32_t& z) {
+    uint32_t b = blockIdx.x;
+    if (g.xcd_remap) b = (b & 7U) * (g.tiles >> 3) + (b >> 3);
+    const uint32_t tx = b % g.tiles_x;
+    const uint32_t r = b / g.tiles_x;
+    const uint32_t ty = r % g.tiles_y;
+    z = r / g.tiles_y;
+    x = tx * g.bx + threadIdx.x;
+    y = ty * g.by + threadIdx.y;
+    return x < g.nx && y < g.ny;
+}
+
+// Periodic neighbourhood of (x, y, z): 3 wrapped x coordinates and 9 wrapped row bases.
+struct Neighbourhood {
+    uint32_t xs[3];
+    uint32_t rows[3][3];  // [dy + 1][dz + 1]
+    __device__ __forceinline__ uint32_t cell(int dx, int dy, int dz) const { return rows[dy + 1][dz + 1] + xs[dx + 1]; }
+};
+
+__device__ __forceinline__ Neighbourhood neighbourhood(const Geometry& g, uint32_t x, uint32_t y, uint32_t z) {
+    Neighbourhood h;
+    h.xs[0] = x == 0U ? g.nx - 1U : x - 1U;
+    h.xs[1] = x;
+    h.xs[2] = x + 1U == g.nx ? 0U : x + 1U;
+    const uint32_t ys[3] = {y == 0U ? g.ny - 1U : y - 1U, y, y + 1U == g.ny ? 0U : y + 1U};
+    const uint32_t zs[3] = {z == 0U ? g.nz - 1U : z - 1U, z, z + 1U == g.nz ? 0U : z + 1U};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) h.rows[a][b] = (zs[b] * g.ny + ys[a]) * g.nx;
+    return h;
+}
+
+// ------------------------------------------------------------------ esoteric slots
+// For the opposite pair (i, i+1) of cell n, with up = n - c_i and dn = n + c_i:
+//   even parity: fin[i] <- F[i][up],   fin[i+1] <- F[i+1][n];  fout[i] -> F[i+1][n], fout[i+1] -> F[i][up]
+//   odd parity:  fin[i] <- F[i+1][up], fin[i+1] <- F[i][n];    fout[i] -> F[i][n],   fout[i+1] -> F[i+1][up]
+// (esoteric_pull.hpp:349-382).  Halfway bounce-back in place: when the upstream cell of a
+// population is a wall, the population is read from the slot of the opposite parity, where this
+// cell stored the opposite population one step earlier.  Wall cells do nothing, and the two slots
+// of a pair at a wall cell are touched only by the one fluid cell facing it, so the update stays
+// race-free with a single pass over the lattice.
+template <int Q> struct PairSlots {
+    uint32_t up[Q];        // index by i (odd)
+    uint8_t wall_up[Q];    // upstream wall flag of direction i (cell up)
+    uint8_t wall_dn[Q];    // upstream wall flag of direction i+1 (cell dn)
+    uint8_t flag_up[Q];    // full flags of up / dn (wall velocity index)
+    uint8_t flag_dn[Q];
+};
+
+// Direction plane holding fin[i] (a) and fin[i+1] (b) of pair i.
+template <bool Odd> __device__ __forceinline__ int dir_a(int i, bool wall_up) { return (Odd != wall_up) ? i + 1 : i; }
+template <bool Odd> __device__ __forceinline__ int dir_b(int i, bool wall_dn) { return (Odd != wall_dn) ? i : i + 1; }
+
+// ------------------------------------------------------------------ collision arithmetic
+// c_d . (ux, uy, uz) with the reference's (a + b) + c association; zero terms are skipped and
+// unit products folded, which leaves every result bit-identical.
+template <int Q, int D, class Real> __device__ __forceinline__ Real cdot(Real ux, Real uy, Real uz) {
+    using L = Lattice<Q>;
+    Real s = Real(0);
+    bool any = false;
+    if constexpr (L::cx[D] != 0) {
+        s = L::cx[D] > 0 ? ux : -ux;
+        any = true;
+    }
+    if constexpr (L::cy[D] != 0) {
+        if (any) s = L::cy[D] > 0 ? s + uy : s - uy;
+        else s = L::cy[D] > 0 ? uy : -uy;
+        any = true;
+    }
+    if constexpr (L::cz[D] != 0) {
+        if (any) s = L::cz[D] > 0 ? s + uz : s - uz;
+        else s = L::cz[D] > 0 ? uz : -uz;
+    }
+    return s;
+}
+
+template <int Q, int D, class Real>
+__device__ __forceinline__ Real equilibrium(Real rho, Real ux, Real uy, Real uz, Real uu) {
+    const Real cu = Real(3) * cdot<Q, D>(ux, uy, uz);
+    return static_cast<Real>(Lattice<Q>::w[D]) * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
+}
+
+template <int Q, int D, class Real>
+__device__ __forceinline__ Real guo(Real ux, Real uy, Real uz, Real fx, Real fy, Real fz, Real uf, Real wfac) {
+    const Real cu = cdot<Q, D>(ux, uy, uz);
+    const Real cf = cdot<Q, D>(fx, fy, fz);
+    return static_cast<Real>(Lattice<Q>::w[D]) * wfac * (Real(3) * (cf - uf) + Real(9) * cu * cf);
+}
+
+// Density and momentum in direction order 0..q-1 (esoteric_pull.hpp:384-406).
+template <int Q, class Real>
+__device__ __forceinline__ void moments(const Real (&f)[Q], Real& rho, Real& mx, Real& my, Real& mz) {
+    using L = Lattice<Q>;
+    rho = mx = my = mz = Real(0);
+    static_for<Q>([&](auto DD) {
+        constexpr int d = decltype(DD)::value;
+        const Real v = f[d];
+        rho += v;
+        if constexpr (L::cx[d] > 0) mx += v;
+        if constexpr (L::cx[d] < 0) mx -= v;
+        if constexpr (L::cy[d] > 0) my += v;
+        if constexpr (L::cy[d] < 0) my -= v;
+        if constexpr (L::cz[d] > 0) mz += v;
+        if constexpr (L::cz[d] < 0) mz -= v;
+    });
+}
+
+template <class Real> struct Physics {
+    Real omega;
+    Real ax, ay, az;
+    Real u_peak, rho_out;
+    Real wall_u[8][3];
+    const uint8_t* flags;       // null: fully periodic lattice
+    const Real* gamma;          // null: no partial bounce-back
+    const Real* lax;            // null: no local body acceleration (lay = lax + cells, laz = lay + cells)
+    const Real* solid_rho;      // density frozen in wall cells
+    uint32_t cells;
+};
+
+// Sum of the populations that reach an x-boundary cell from inside the domain.
+template <int Q, int SignX, class Real> __device__ __forceinline__ Real sum_cx(const Real (&f)[Q]) {
+    Real s = Real(0);
+    static_for<Q>([&](auto DD) {
+        constexpr int d = decltype(DD)::value;
+        if constexpr (Lattice<Q>::cx[d] == SignX) s += f[d];
+    });
+    return s;
+}
+
+template <class Real>
+__device__ __forceinline__ Real inlet_velocity(const Physics<Real>& p, const Geometry& g, uint32_t y, uint32_t z,
+                                               bool three_d) {
+    const Real eta = (static_cast<Real>(y) - Real(0.5)) / static_cast<Real>(g.ny - 2U);
+    Real u = p.u_peak * (Real(4) * eta * (Real(1) - eta));
+    if (three_d) {
+        const Real zeta = (static_cast<Real>(z) - Real(0.5)) / static_cast<Real>(g.nz - 2U);
+        u = u * (Real(4) * zeta * (Real(1) - zeta));
+    }
+    return u;
+}
+
+// Macroscopic state a non-wall cell reports (esoteric_pull.hpp:180-209 for fluid cells).
+template <int Q, class Real>
+__device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
+                                           uint32_t y, uint32_t z, const Real (&f)[Q], Real& rho, Real& ux, Real& uy,
+                                           Real& uz) {
+    if (flag == kInlet) {
+        const Real uin = inlet_velocity(p, g, y, z, Lattice<Q>::dims == 3);
+        rho = (sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, -1>(f)) / (Real(1) - uin);
+        ux = uin;
+        uy = uz = Real(0);
+        return;
+    }
+    if (flag == kOutlet) {
+        rho = p.rho_out;
+        ux = Real(-1) + (sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, 1>(f)) / p.rho_out;
+        uy = uz = Real(0);
+        return;
+    }
+    Real mx, my, mz;
+    moments<Q>(f, rho, mx, my, mz);
+    if (rho > Real(0)) {
+        ux = mx / rho;
+        uy = my / rho;
+        uz = mz / rho;
+    } else {
+        ux = uy = uz = Real(0);
+    }
+    Real ax = p.ax, ay = p.ay, az = p.az;
+    if (p.lax) {
+        ax = ax + p.lax[n];
+        ay = ay + p.lax[p.cells + n];
+        az = az + p.lax[2U * p.cells + n];
+    }
+    ux = ux + Real(0.5) * ax;
+    uy = uy + Real(0.5) * ay;
+    uz = uz + Real(0.5) * az;
+}
+
+// Collision of one non-wall cell in registers.  Returns the density used for moving-wall
+// momentum corrections (src/lbm/d2q9.cpp:217-223 uses the fluid cell's density).
+template <int Q, class Real>
+__device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
+                                        uint32_t y, uint32_t z, Real (&f)[Q]) {
+    if (flag == kInlet || flag == kOutlet) {
+        Real rho, ux, uy, uz;
+        cell_state<Q>(p, g, flag, n, y, z, f, rho, ux, uy, uz);
+        const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            f[d] = equilibrium<Q, d>(rho, ux, uy, uz, uu);
+        });
+        return rho;
+    }
+    Real rho, mx, my, mz;
+    moments<Q>(f, rho, mx, my, mz);
+    Real ux, uy, uz;
+    if (rho > Real(0)) {
+        ux = mx / rho;
+        uy = my / rho;
+        uz = mz / rho;
+    } else {
+        ux = uy = uz = Real(0);
+    }
+    Real ax = p.ax, ay = p.ay, az = p.az;
+    if (p.lax) {
+        ax = ax + p.lax[n];
+        ay = ay + p.lax[p.cells + n];
+        az = az + p.lax[2U * p.cells + n];
+    }
+    ux = ux + Real(0.5) * ax;
+    uy = uy + Real(0.5) * ay;
+    uz = uz + Real(0.5) * az;
+    const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
+    const Real omega = p.omega;
+    Real fin_copy[Q];
+    if (p.gamma) {
+        static_for<Q>([&](auto DD) { fin_copy[decltype(DD)::value] = f[decltype(DD)::value]; });
+    }
+    if (ax == Real(0) && ay == Real(0) && az == Real(0)) {
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            f[d] = f[d] - omega * (f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu));
+        });
+    } else {
+        const Real fx = rho * ax, fy = rho * ay, fz = rho * az;
+        const Real uf = ux * fx + uy * fy + uz * fz;
+        const Real wfac = Real(1) - Real(0.5) * omega;
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            const Real post = f[d] - omega * (f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu));
+            f[d] = post + guo<Q, d>(ux, uy, uz, fx, fy, fz, uf, wfac);
+        });
+    }
+    if (p.gamma) {
+        // Partial bounce-back (Walsh, Burwinkle & Saar 2009): blend the BGK result with the
+        // pre-collision population of the opposite direction.
+        const Real gm = p.gamma[n];
+        const Real keep = Real(1) - gm;
+        Real mixed[Q];
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            mixed[d] = keep * f[d] + gm * fin_copy[opposite(d)];
+        });
+        static_for<Q>([&](auto DD) { f[decltype(DD)::value] = mixed[decltype(DD)::value]; });
+    }
+    return rho;
+}
+
+// ------------------------------------------------------------------ seeded noise (SplitMix64)
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+template <class Real>
+__host__ __device__ __forceinline__ Real uniform01(uint64_t seed_hash, uint64_t cell, uint64_t component) {
+    const uint64_t h = splitmix64(seed_hash + 4ULL * cell + component);
+    return static_cast<Real>(h >> 40) * static_cast<Real>(0x1p-24);
+}
+
+}  // namespace cfd_hip

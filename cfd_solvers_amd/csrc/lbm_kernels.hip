@@ -1,0 +1,616 @@
+// lbm_kernels.hip -- gfx950 kernels of the esoteric-pull LBM backend and their launchers.
+//
+// Hot kernel: stream_collide<Q, T, Odd, Bounded>, one fused in-place pass per time step
+// (the single-grid scheme of include/cfd/solvers/lbm/esoteric_pull.hpp:494-514; the reference's
+// device sibling is the SYCL kernel at esoteric_pull_sycl.hpp:87-211).  Everything else here runs
+// once per setup or per diagnostic: initialisation, bounce-back slot preparation, macroscopic
+// fields, FP64 reductions, the index-map probe.
+#include "lbm_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace cfd_hip {
+
+namespace {
+
+template <class T> __device__ __forceinline__ T* plane(void* base, const Geometry& g, int d) {
+    return static_cast<T*>(base) + static_cast<uint64_t>(d) * g.stride;
+}
+template <class T> __device__ __forceinline__ const T* plane(const void* base, const Geometry& g, int d) {
+    return static_cast<const T*>(base) + static_cast<uint64_t>(d) * g.stride;
+}
+
+// Loads the logical populations of a non-wall cell (bounce-back aware) and records the slots.
+template <int Q, class T, bool Odd, bool Bounded>
+__device__ __forceinline__ void load_cell(const void* f, const Geometry& g, const uint8_t* flags,
+                                          const Neighbourhood& h, uint32_t n,
+                                          typename Storage<T>::Real (&fin)[Q], PairSlots<Q>& s) {
+    using L = Lattice<Q>;
+    using S = Storage<T>;
+    fin[0] = S::template decode<Q, 0>(plane<T>(f, g, 0)[n]);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+        s.up[i] = up;
+        bool wup = false, wdn = false;
+        if constexpr (Bounded) {
+            const uint32_t dn = h.cell(L::cx[i], L::cy[i], L::cz[i]);
+            s.flag_up[i] = flags[up];
+            s.flag_dn[i] = flags[dn];
+            wup = is_wall(s.flag_up[i]);
+            wdn = is_wall(s.flag_dn[i]);
+            s.wall_up[i] = wup;
+            s.wall_dn[i] = wdn;
+        }
+        const T* pa = (Odd != wup) ? plane<T>(f, g, i + 1) : plane<T>(f, g, i);
+        const T* pb = (Odd != wdn) ? plane<T>(f, g, i) : plane<T>(f, g, i + 1);
+        // Decoding uses the logical direction's shift: the slot of either plane of the pair
+        // holds a population of the direction it is loaded as.
+        fin[i] = S::template decode<Q, i>(pa[up]);
+        fin[i + 1] = S::template decode<Q, i + 1>(pb[n]);
+    });
+}
+
+template <int Q, class T, bool Odd, bool Bounded>
+__device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Physics<typename Storage<T>::Real>& p,
+                                           uint32_t n, typename Storage<T>::Real rho,
+                                           typename Storage<T>::Real (&fout)[Q], const PairSlots<Q>& s) {
+    using L = Lattice<Q>;
+    using S = Storage<T>;
+    using Real = typename Storage<T>::Real;
+    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(fout[0]);
+    const Real rho_wall = rho > Real(1.0e-8) ? rho : Real(1.0e-8);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        Real a = fout[i];      // travels to dn = n + c_i
+        Real b = fout[i + 1];  // travels to up = n - c_i
+        if constexpr (Bounded) {
+            // Moving wall: the bounced population loses 6 w rho_w (c . u_w) (push form of
+            // src/lbm/d2q9.cpp:217-223).
+            const int ka = s.flag_dn[i] & 7;
+            if (s.wall_dn[i] && ka != 0) {
+                const Real cu = static_cast<Real>(L::cx[i]) * p.wall_u[ka][0] +
+                                static_cast<Real>(L::cy[i]) * p.wall_u[ka][1] +
+                                static_cast<Real>(L::cz[i]) * p.wall_u[ka][2];
+                a = a - Real(6) * static_cast<Real>(L::w[i]) * rho_wall * cu;
+            }
+            const int kb = s.flag_up[i] & 7;
+            if (s.wall_up[i] && kb != 0) {
+                const Real cu = static_cast<Real>(L::cx[i + 1]) * p.wall_u[kb][0] +
+                                static_cast<Real>(L::cy[i + 1]) * p.wall_u[kb][1] +
+                                static_cast<Real>(L::cz[i + 1]) * p.wall_u[kb][2];
+                b = b - Real(6) * static_cast<Real>(L::w[i + 1]) * rho_wall * cu;
+            }
+        }
+        if constexpr (Odd) {
+            plane<T>(f, g, i)[n] = S::template encode<Q, i>(a);
+            plane<T>(f, g, i + 1)[s.up[i]] = S::template encode<Q, i + 1>(b);
+        } else {
+            plane<T>(f, g, i + 1)[n] = S::template encode<Q, i>(a);
+            plane<T>(f, g, i)[s.up[i]] = S::template encode<Q, i + 1>(b);
+        }
+    });
+}
+
+// ------------------------------------------------------------------ hot kernel
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void stream_collide(void* __restrict__ f, const Geometry g,
+                                                      const Physics<typename Storage<T>::Real> p) {
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    uint8_t flag = kFluid;
+    if constexpr (Bounded) {
+        flag = p.flags[n];
+        if (is_wall(flag)) return;
+    }
+    Real fin[Q];
+    PairSlots<Q> s;
+    load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
+    const Real rho = collide<Q>(p, g, flag, n, y, z, fin);
+    store_cell<Q, T, Odd, Bounded>(f, g, p, n, rho, fin, s);
+}
+
+// ------------------------------------------------------------------ initialisation
+template <class Real> struct InitArgs {
+    int mode;  // 0 uniform, 1 Taylor-Green tables, 2 per-cell fields
+    Real rho, ux, uy, uz;
+    Real amplitude, noise;
+    uint64_t seed_hash;
+    const Real* tables;  // sx[nx], cx[nx], sy[ny], cy[ny], zf[nz]
+    const Real* fields;  // rho, ux, uy, uz planes of `cells`
+};
+
+// esoteric_pull.hpp:312-333: f = f_eq(rho, u) written in the even layout; walls at rest.
+template <int Q, class T>
+__global__ __launch_bounds__(256) void initialize(void* __restrict__ f, const Geometry g,
+                                                  const InitArgs<typename Storage<T>::Real> a,
+                                                  const uint8_t* __restrict__ flags, uint32_t cells) {
+    using L = Lattice<Q>;
+    using S = Storage<T>;
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    Real rho = a.rho, ux = a.ux, uy = a.uy, uz = a.uz;
+    if (a.mode == 1) {
+        const Real* sx = a.tables;
+        const Real* cx = sx + g.nx;
+        const Real* sy = cx + g.nx;
+        const Real* cy = sy + g.ny;
+        const Real* zf = cy + g.ny;
+        rho = Real(1);
+        ux = a.amplitude * sx[x] * cy[y] * zf[z];
+        uy = -a.amplitude * cx[x] * sy[y] * zf[z];
+        uz = Real(0);
+        if (a.noise != Real(0)) {
+            ux = ux + a.noise * (Real(2) * uniform01<Real>(a.seed_hash, n, 0) - Real(1));
+            uy = uy + a.noise * (Real(2) * uniform01<Real>(a.seed_hash, n, 1) - Real(1));
+            if constexpr (L::dims == 3) uz = uz + a.noise * (Real(2) * uniform01<Real>(a.seed_hash, n, 2) - Real(1));
+        }
+    } else if (a.mode == 2) {
+        rho = a.fields[n];
+        ux = a.fields[cells + n];
+        uy = a.fields[2U * cells + n];
+        uz = a.fields[3U * cells + n];
+    }
+    if (flags && is_wall(flags[n])) ux = uy = uz = Real(0);
+    const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
+    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium<Q, 0>(rho, ux, uy, uz, uu));
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+        plane<T>(f, g, i)[up] = S::template encode<Q, i>(equilibrium<Q, i>(rho, ux, uy, uz, uu));
+        plane<T>(f, g, i + 1)[n] = S::template encode<Q, i + 1>(equilibrium<Q, i + 1>(rho, ux, uy, uz, uu));
+    });
+}
+
+// Writes caller-supplied logical populations (q planes of `cells`) at the given parity.
+template <int Q, class T, bool Odd>
+__global__ __launch_bounds__(256) void upload_logical(void* __restrict__ f, const Geometry g,
+                                                      const typename Storage<T>::Real* __restrict__ in,
+                                                      uint32_t cells) {
+    using L = Lattice<Q>;
+    using S = Storage<T>;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(in[n]);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+        const auto va = in[static_cast<uint64_t>(i) * cells + n];
+        const auto vb = in[static_cast<uint64_t>(i + 1) * cells + n];
+        if constexpr (Odd) {
+            plane<T>(f, g, i + 1)[up] = S::template encode<Q, i + 1>(va);
+            plane<T>(f, g, i)[n] = S::template encode<Q, i>(vb);
+        } else {
+            plane<T>(f, g, i)[up] = S::template encode<Q, i>(va);
+            plane<T>(f, g, i + 1)[n] = S::template encode<Q, i + 1>(vb);
+        }
+    });
+}
+
+// Bounce-back slot preparation for a change of flags at a given parity.
+//   ToBounceBack = true : copy normal slot -> opposite-parity slot for every wall link of a
+//                         non-wall cell (after initialisation or upload under `flags`).
+//   ToBounceBack = false: copy opposite-parity slot -> normal slot (before `flags` are replaced),
+//                         which restores the plain esoteric layout of the fluid populations.
+template <int Q, class T, bool Odd, bool ToBounceBack>
+__global__ __launch_bounds__(256) void bounce_back_slots(void* __restrict__ f, const Geometry g,
+                                                         const uint8_t* __restrict__ flags) {
+    using L = Lattice<Q>;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    if (is_wall(flags[n])) return;
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+        const uint32_t dn = h.cell(L::cx[i], L::cy[i], L::cz[i]);
+        if (is_wall(flags[up])) {
+            T* normal = Odd ? plane<T>(f, g, i + 1) : plane<T>(f, g, i);
+            T* other = Odd ? plane<T>(f, g, i) : plane<T>(f, g, i + 1);
+            if (ToBounceBack) other[up] = normal[up];
+            else normal[up] = other[up];
+        }
+        if (is_wall(flags[dn])) {
+            T* normal = Odd ? plane<T>(f, g, i) : plane<T>(f, g, i + 1);
+            T* other = Odd ? plane<T>(f, g, i + 1) : plane<T>(f, g, i);
+            if (ToBounceBack) other[n] = normal[n];
+            else normal[n] = other[n];
+        }
+    });
+}
+
+// Density of the cells that become walls, from their plain (normal-slot) populations; cells
+// that already were walls keep their frozen density.
+template <int Q, class T, bool Odd>
+__global__ __launch_bounds__(256) void freeze_wall_density(const void* __restrict__ f, const Geometry g,
+                                                           const uint8_t* __restrict__ flags,
+                                                           const uint8_t* __restrict__ old_flags,
+                                                           typename Storage<T>::Real* __restrict__ solid_rho) {
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    if (!is_wall(flags[n])) return;
+    if (old_flags && is_wall(old_flags[n])) return;
+    Real fin[Q];
+    PairSlots<Q> s;
+    load_cell<Q, T, Odd, false>(f, g, nullptr, h, n, fin, s);
+    Real rho = Real(0);
+    static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
+    solid_rho[n] = rho;
+}
+
+// Cells that stop being walls restart from rest equilibrium at their frozen density, written in
+// the plain layout (wall cells keep no populations of their own while they are walls).
+template <int Q, class T, bool Odd>
+__global__ __launch_bounds__(256) void thaw_walls(void* __restrict__ f, const Geometry g,
+                                                  const uint8_t* __restrict__ old_flags,
+                                                  const uint8_t* __restrict__ flags,
+                                                  const typename Storage<T>::Real* __restrict__ solid_rho) {
+    using L = Lattice<Q>;
+    using S = Storage<T>;
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    if (!is_wall(old_flags[n]) || (flags && is_wall(flags[n]))) return;
+    const Real rho = solid_rho[n];
+    const Real zero = Real(0);
+    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium<Q, 0>(rho, zero, zero, zero, zero));
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+        const Real va = equilibrium<Q, i>(rho, zero, zero, zero, zero);
+        const Real vb = equilibrium<Q, i + 1>(rho, zero, zero, zero, zero);
+        if constexpr (Odd) {
+            plane<T>(f, g, i + 1)[up] = S::template encode<Q, i + 1>(va);
+            plane<T>(f, g, i)[n] = S::template encode<Q, i>(vb);
+        } else {
+            plane<T>(f, g, i)[up] = S::template encode<Q, i>(va);
+            plane<T>(f, g, i + 1)[n] = S::template encode<Q, i + 1>(vb);
+        }
+    });
+}
+
+// ------------------------------------------------------------------ macroscopic fields
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void macroscopic(const void* __restrict__ f, const Geometry g,
+                                                   const Physics<typename Storage<T>::Real> p,
+                                                   typename Storage<T>::Real* __restrict__ out) {
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t cells = p.cells;
+    uint8_t flag = kFluid;
+    if constexpr (Bounded) {
+        flag = p.flags[n];
+        if (is_wall(flag)) {
+            out[n] = p.solid_rho[n];
+            out[cells + n] = Real(0);
+            out[2U * cells + n] = Real(0);
+            out[3U * cells + n] = Real(0);
+            return;
+        }
+    }
+    Real fin[Q];
+    PairSlots<Q> s;
+    load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
+    Real rho, ux, uy, uz;
+    cell_state<Q>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
+    out[n] = rho;
+    out[cells + n] = ux;
+    out[2U * cells + n] = uy;
+    out[3U * cells + n] = uz;
+}
+
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void download_logical(const void* __restrict__ f, const Geometry g,
+                                                        const Physics<typename Storage<T>::Real> p,
+                                                        typename Storage<T>::Real* __restrict__ out) {
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    if constexpr (Bounded) {
+        if (is_wall(p.flags[n])) {
+            const Real rho = p.solid_rho[n];
+            static_for<Q>([&](auto DD) {
+                constexpr int d = decltype(DD)::value;
+                out[static_cast<uint64_t>(d) * p.cells + n] = static_cast<Real>(Lattice<Q>::w[d]) * rho;
+            });
+            return;
+        }
+    }
+    Real fin[Q];
+    PairSlots<Q> s;
+    load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
+    static_for<Q>([&](auto DD) {
+        constexpr int d = decltype(DD)::value;
+        out[static_cast<uint64_t>(d) * p.cells + n] = fin[d];
+    });
+}
+
+// ------------------------------------------------------------------ FP64 reductions
+// Per block: mass (sum of per-cell population sums; wall cells contribute their frozen density),
+// kinetic energy 0.5 rho |u|^2 and max |u| over non-wall cells.  Fixed tree order: deterministic.
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void diagnostics_partial(const void* __restrict__ f, const Geometry g,
+                                                           const Physics<typename Storage<T>::Real> p,
+                                                           double* __restrict__ partial) {
+    using Real = typename Storage<T>::Real;
+    __shared__ double sm[3][256];
+    const uint32_t tid = threadIdx.y * blockDim.x + threadIdx.x;
+    double mass = 0.0, ke = 0.0, vmax = 0.0;
+    uint32_t x, y, z;
+    if (cell_coords(g, x, y, z)) {
+        const Neighbourhood h = neighbourhood(g, x, y, z);
+        const uint32_t n = h.cell(0, 0, 0);
+        uint8_t flag = kFluid;
+        bool wall = false;
+        if constexpr (Bounded) {
+            flag = p.flags[n];
+            wall = is_wall(flag);
+        }
+        if (wall) {
+            mass = static_cast<double>(p.solid_rho[n]);
+        } else {
+            Real fin[Q];
+            PairSlots<Q> s;
+            load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
+            Real sum = Real(0);
+            static_for<Q>([&](auto DD) { sum += fin[decltype(DD)::value]; });
+            mass = static_cast<double>(sum);
+            Real rho, ux, uy, uz;
+            cell_state<Q>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
+            ke = static_cast<double>(rho) * (static_cast<double>(ux) * ux + static_cast<double>(uy) * uy +
+                                             static_cast<double>(uz) * uz);
+            vmax = static_cast<double>(sqrt(ux * ux + uy * uy + uz * uz));
+        }
+    }
+    sm[0][tid] = mass;
+    sm[1][tid] = ke;
+    sm[2][tid] = vmax;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            sm[0][tid] += sm[0][tid + w];
+            sm[1][tid] += sm[1][tid + w];
+            sm[2][tid] = fmax(sm[2][tid], sm[2][tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        partial[3U * blockIdx.x + 0] = sm[0][0];
+        partial[3U * blockIdx.x + 1] = sm[1][0];
+        partial[3U * blockIdx.x + 2] = sm[2][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void diagnostics_final(const double* __restrict__ partial, uint32_t count,
+                                                         double* __restrict__ out) {
+    __shared__ double sm[3][256];
+    const uint32_t tid = threadIdx.x;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t k = tid; k < count; k += 256U) {
+        a += partial[3U * k + 0];
+        b += partial[3U * k + 1];
+        c = fmax(c, partial[3U * k + 2]);
+    }
+    sm[0][tid] = a;
+    sm[1][tid] = b;
+    sm[2][tid] = c;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            sm[0][tid] += sm[0][tid + w];
+            sm[1][tid] += sm[1][tid + w];
+            sm[2][tid] = fmax(sm[2][tid], sm[2][tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = sm[0][0];
+        out[1] = 0.5 * sm[1][0];
+        out[2] = sm[2][0];
+    }
+}
+
+// ------------------------------------------------------------------ index-map probe
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void index_map(const Geometry g, const uint8_t* __restrict__ flags, uint32_t cells,
+                                                 int64_t* __restrict__ load, int64_t* __restrict__ store) {
+    using L = Lattice<Q>;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    auto put = [&](int64_t* out, int d, int plane_dir, uint32_t cell) {
+        out[static_cast<uint64_t>(d) * cells + n] = static_cast<int64_t>(plane_dir) * cells + cell;
+    };
+    if (Bounded && is_wall(flags[n])) {
+        for (int d = 0; d < Q; ++d) {
+            load[static_cast<uint64_t>(d) * cells + n] = -1;
+            store[static_cast<uint64_t>(d) * cells + n] = -1;
+        }
+        return;
+    }
+    put(load, 0, 0, n);
+    put(store, 0, 0, n);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+        const uint32_t dn = h.cell(L::cx[i], L::cy[i], L::cz[i]);
+        const bool wup = Bounded && is_wall(flags[up]);
+        const bool wdn = Bounded && is_wall(flags[dn]);
+        put(load, i, dir_a<Odd>(i, wup), up);
+        put(load, i + 1, dir_b<Odd>(i, wdn), n);
+        put(store, i, Odd ? i : i + 1, n);
+        put(store, i + 1, Odd ? i + 1 : i, up);
+    });
+}
+
+// ------------------------------------------------------------------ launchers
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline dim3 block_of(const Geometry& g) { return dim3(g.bx, g.by, 1); }
+inline dim3 grid_of(const Geometry& g) { return dim3(g.tiles, 1, 1); }
+
+template <int Q, class T> struct Ops final : KernelOps {
+    using Real = typename Storage<T>::Real;
+
+    template <bool Odd, bool Bounded> static void step_t(const LaunchState& st) {
+        hipLaunchKernelGGL((stream_collide<Q, T, Odd, Bounded>), grid_of(st.geo), block_of(st.geo), 0, st.stream,
+                           st.f, st.geo, *static_cast<const Physics<Real>*>(st.physics));
+    }
+    void step(const LaunchState& st, bool odd, bool bounded) const override {
+        if (odd) bounded ? step_t<true, true>(st) : step_t<true, false>(st);
+        else bounded ? step_t<false, true>(st) : step_t<false, false>(st);
+        check(hipGetLastError(), "stream_collide launch");
+    }
+    void initialize(const LaunchState& st, const void* init_args, const uint8_t* flags) const override {
+        hipLaunchKernelGGL((cfd_hip::initialize<Q, T>), grid_of(st.geo), block_of(st.geo), 0, st.stream, st.f, st.geo,
+                           *static_cast<const InitArgs<Real>*>(init_args), flags, st.cells);
+        check(hipGetLastError(), "initialize launch");
+    }
+    void upload(const LaunchState& st, bool odd, const void* logical) const override {
+        if (odd)
+            hipLaunchKernelGGL((upload_logical<Q, T, true>), grid_of(st.geo), block_of(st.geo), 0, st.stream, st.f,
+                               st.geo, static_cast<const Real*>(logical), st.cells);
+        else
+            hipLaunchKernelGGL((upload_logical<Q, T, false>), grid_of(st.geo), block_of(st.geo), 0, st.stream, st.f,
+                               st.geo, static_cast<const Real*>(logical), st.cells);
+        check(hipGetLastError(), "upload launch");
+    }
+    void bounce_back(const LaunchState& st, bool odd, bool to_bounce_back, const uint8_t* flags) const override {
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid_of(st.geo), block_of(st.geo), 0, st.stream, st.f, st.geo, flags);
+        };
+        if (odd) to_bounce_back ? go(bounce_back_slots<Q, T, true, true>) : go(bounce_back_slots<Q, T, true, false>);
+        else to_bounce_back ? go(bounce_back_slots<Q, T, false, true>) : go(bounce_back_slots<Q, T, false, false>);
+        check(hipGetLastError(), "bounce_back_slots launch");
+    }
+    void freeze(const LaunchState& st, bool odd, const uint8_t* flags, const uint8_t* old_flags,
+                void* solid_rho) const override {
+        if (odd)
+            hipLaunchKernelGGL((freeze_wall_density<Q, T, true>), grid_of(st.geo), block_of(st.geo), 0, st.stream,
+                               st.f, st.geo, flags, old_flags, static_cast<Real*>(solid_rho));
+        else
+            hipLaunchKernelGGL((freeze_wall_density<Q, T, false>), grid_of(st.geo), block_of(st.geo), 0, st.stream,
+                               st.f, st.geo, flags, old_flags, static_cast<Real*>(solid_rho));
+        check(hipGetLastError(), "freeze_wall_density launch");
+    }
+    void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
+              const void* solid_rho) const override {
+        if (odd)
+            hipLaunchKernelGGL((thaw_walls<Q, T, true>), grid_of(st.geo), block_of(st.geo), 0, st.stream, st.f, st.geo,
+                               old_flags, flags, static_cast<const Real*>(solid_rho));
+        else
+            hipLaunchKernelGGL((thaw_walls<Q, T, false>), grid_of(st.geo), block_of(st.geo), 0, st.stream, st.f,
+                               st.geo, old_flags, flags, static_cast<const Real*>(solid_rho));
+        check(hipGetLastError(), "thaw_walls launch");
+    }
+    template <class K> static void per_parity(const LaunchState& st, bool odd, bool bounded, K k) {
+        if (odd) bounded ? k(std::true_type{}, std::true_type{}) : k(std::true_type{}, std::false_type{});
+        else bounded ? k(std::false_type{}, std::true_type{}) : k(std::false_type{}, std::false_type{});
+    }
+    void macroscopic(const LaunchState& st, bool odd, bool bounded, void* out) const override {
+        per_parity(st, odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((cfd_hip::macroscopic<Q, T, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
+                               block_of(st.geo), 0, st.stream, st.f, st.geo,
+                               *static_cast<const Physics<Real>*>(st.physics), static_cast<Real*>(out));
+        });
+        check(hipGetLastError(), "macroscopic launch");
+    }
+    void download(const LaunchState& st, bool odd, bool bounded, void* out) const override {
+        per_parity(st, odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((download_logical<Q, T, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
+                               block_of(st.geo), 0, st.stream, st.f, st.geo,
+                               *static_cast<const Physics<Real>*>(st.physics), static_cast<Real*>(out));
+        });
+        check(hipGetLastError(), "download_logical launch");
+    }
+    void diagnostics(const LaunchState& st, bool odd, bool bounded, double* partial, double* out) const override {
+        per_parity(st, odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((diagnostics_partial<Q, T, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
+                               block_of(st.geo), 0, st.stream, st.f, st.geo,
+                               *static_cast<const Physics<Real>*>(st.physics), partial);
+        });
+        check(hipGetLastError(), "diagnostics_partial launch");
+        hipLaunchKernelGGL(diagnostics_final, dim3(1), dim3(256), 0, st.stream, partial, st.geo.tiles, out);
+        check(hipGetLastError(), "diagnostics_final launch");
+    }
+    void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, int64_t* load,
+                   int64_t* store) const override {
+        per_parity(st, odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((cfd_hip::index_map<Q, T, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
+                               block_of(st.geo), 0, st.stream, st.geo, flags, st.cells, load, store);
+        });
+        check(hipGetLastError(), "index_map launch");
+    }
+};
+
+template <int Q> const KernelOps* ops_for_storage(int storage) {
+    static const Ops<Q, float> f32;
+    static const Ops<Q, _Float16> f16;
+    static const Ops<Q, double> f64;
+    switch (storage) {
+    case 0: return &f32;
+    case 1: return &f16;
+    case 2: return &f64;
+    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1) or FP64 (2)");
+    }
+}
+
+}  // namespace
+
+const KernelOps* kernel_ops(int q, int storage) {
+    switch (q) {
+    case 9: return ops_for_storage<9>(storage);
+    case 19: return ops_for_storage<19>(storage);
+    case 27: return ops_for_storage<27>(storage);
+    default: throw std::invalid_argument("lattice must be D2Q9 (9), D3Q19 (19) or D3Q27 (27)");
+    }
+}
+
+size_t init_args_size(int storage) { return storage == 2 ? sizeof(InitArgs<double>) : sizeof(InitArgs<float>); }
+
+void fill_init_args(int storage, void* out, int mode, double rho, double ux, double uy, double uz, double amplitude,
+                    double noise, uint64_t seed, const void* tables, const void* fields) {
+    auto fill = [&](auto* a) {
+        using Real = std::remove_reference_t<decltype(a->rho)>;
+        a->mode = mode;
+        a->rho = static_cast<Real>(rho);
+        a->ux = static_cast<Real>(ux);
+        a->uy = static_cast<Real>(uy);
+        a->uz = static_cast<Real>(uz);
+        a->amplitude = static_cast<Real>(amplitude);
+        a->noise = static_cast<Real>(noise);
+        a->seed_hash = splitmix64(seed);
+        a->tables = static_cast<const Real*>(tables);
+        a->fields = static_cast<const Real*>(fields);
+    };
+    if (storage == 2) fill(static_cast<InitArgs<double>*>(out));
+    else fill(static_cast<InitArgs<float>*>(out));
+}
+
+}  // namespace cfd_hip

@@ -1,0 +1,45 @@
+// lbm_kernels.hpp -- host-side interface to the kernel templates in lbm_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "lbm_device.hpp"
+
+namespace cfd_hip {
+
+struct LaunchState {
+    void* f;                 // population planes
+    Geometry geo;
+    const void* physics;     // Physics<float> or Physics<double> (host copy, passed by value)
+    hipStream_t stream;
+    uint32_t cells;
+};
+
+// One instance per (lattice, storage); every launch is asynchronous on st.stream.
+struct KernelOps {
+    virtual ~KernelOps() = default;
+    virtual void step(const LaunchState& st, bool odd, bool bounded) const = 0;
+    virtual void initialize(const LaunchState& st, const void* init_args, const uint8_t* flags) const = 0;
+    virtual void upload(const LaunchState& st, bool odd, const void* logical) const = 0;
+    virtual void bounce_back(const LaunchState& st, bool odd, bool to_bounce_back, const uint8_t* flags) const = 0;
+    virtual void freeze(const LaunchState& st, bool odd, const uint8_t* flags, const uint8_t* old_flags,
+                        void* solid_rho) const = 0;
+    virtual void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
+                      const void* solid_rho) const = 0;
+    virtual void macroscopic(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
+    virtual void download(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
+    virtual void diagnostics(const LaunchState& st, bool odd, bool bounded, double* partial, double* out) const = 0;
+    virtual void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, int64_t* load,
+                           int64_t* store) const = 0;
+};
+
+const KernelOps* kernel_ops(int q, int storage);
+
+size_t init_args_size(int storage);
+void fill_init_args(int storage, void* out, int mode, double rho, double ux, double uy, double uz, double amplitude,
+                    double noise, uint64_t seed, const void* tables, const void* fields);
+
+}  // namespace cfd_hip

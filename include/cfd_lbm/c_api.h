@@ -1,0 +1,150 @@
+/*
+ * cfd_lbm/c_api.h -- C ABI of the MI355X (gfx950) esoteric-pull lattice-Boltzmann backend.
+ *
+ * This is the drop-in boundary for the reference's LBM hot path.  Each entry point replaces
+ * a member of the reference's solver classes, cited as path:line in the reference tree:
+ *   EsotericPullSolver<D>      include/cfd/solvers/lbm/esoteric_pull.hpp:104-278
+ *   EsotericPullSyclSolver<D>  include/cfd/solvers/lbm/esoteric_pull_sycl.hpp:18-475
+ * and follows the reference's C-ABI convention (include/cfd/python/c_api.h:1-15,
+ * src/python/c_api.cpp:8-15):
+ *   - every function returns int: 0 = ok, 1 = std::exception (message copied),
+ *     2 = unknown exception;
+ *   - the last two parameters are a caller-owned error buffer and its capacity; the
+ *     message is NUL-terminated and truncated to fit;
+ *   - arrays are caller-owned host pointers with the element counts stated below.
+ *
+ * Cells are numbered n = (z*ny + y)*nx + x (x fastest).  Logical population arrays are
+ * direction-major: value (d, n) at index d*cells + n.  Each solver owns its device memory and
+ * one HIP stream; calls on one solver are not thread-safe (the reference's host caller is
+ * single-threaded too).  Work is asynchronous on that stream until a call that returns host
+ * data or cfd_lbm_synchronize().
+ */
+#ifndef CFD_LBM_C_API_H
+#define CFD_LBM_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define CFD_LBM_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cfd_lbm_solver cfd_lbm_solver;
+
+/* Lattice ids are the velocity-set sizes q (descriptors.hpp:31-146). */
+enum { CFD_LBM_D2Q9 = 9, CFD_LBM_D3Q19 = 19, CFD_LBM_D3Q27 = 27 };
+
+/* Population storage.  Collision arithmetic is FP32 for FP32 and FP16S, FP64 for FP64. */
+enum {
+    CFD_LBM_STORE_FP32 = 0,  /* 4 B / population (the reference's storage)                */
+    CFD_LBM_STORE_FP16S = 1, /* 2 B / population: binary16 of (f - w_d) * 4096, RNE       */
+    CFD_LBM_STORE_FP64 = 2   /* 8 B / population: verification build of the same kernels  */
+};
+
+/* uint8 cell flags (cfd_lbm_set_flags). */
+enum {
+    CFD_LBM_FLUID = 0x00,
+    CFD_LBM_VELOCITY_INLET = 0x01, /* x = 0 plane: parabolic inlet, Zou-He density          */
+    CFD_LBM_PRESSURE_OUTLET = 0x02,/* x = nx-1 plane: fixed density, Zou-He normal velocity */
+    CFD_LBM_WALL = 0x80            /* halfway bounce-back; bits 0-2 select a wall velocity  */
+};
+
+CFD_LBM_API const char* cfd_lbm_version(void);
+
+/* EsotericPullSolver(InPlaceLbmConfig) -- esoteric_pull.hpp:109-115,292-302.  Throws (returns 1)
+ * if tau <= 0.5, nx or ny < 2, nz < 2 for 3-D lattices.  nz is forced to 1 for D2Q9.  The
+ * lattice starts at rest with rho = 1 (initialize_uniform). */
+CFD_LBM_API int cfd_lbm_create(int lattice, size_t nx, size_t ny, size_t nz, float tau, float ax, float ay,
+                               float az, int storage, cfd_lbm_solver** out, char* error, size_t error_capacity);
+/* ~EsotericPullSyclSolver -- esoteric_pull_sycl.hpp:35-45: waits, frees, swallows async errors. */
+CFD_LBM_API int cfd_lbm_destroy(cfd_lbm_solver* solver, char* error, size_t error_capacity);
+
+/* initialize_uniform -- esoteric_pull.hpp:117-119. */
+CFD_LBM_API int cfd_lbm_initialize_uniform(cfd_lbm_solver* s, float rho, float ux, float uy, float uz, char* error,
+                                           size_t error_capacity);
+/* initialize_taylor_green -- esoteric_pull.hpp:121-137, plus seeded uniform noise of amplitude
+ * noise_amplitude on each velocity component (SplitMix64 of (seed, cell, component)).
+ * noise_amplitude = 0 reproduces the reference exactly. */
+CFD_LBM_API int cfd_lbm_initialize_taylor_green(cfd_lbm_solver* s, float amplitude, float noise_amplitude,
+                                                uint64_t seed, char* error, size_t error_capacity);
+/* initialize(initializer) -- esoteric_pull.hpp:312-333 with caller-supplied fields (cells each). */
+CFD_LBM_API int cfd_lbm_initialize_fields(cfd_lbm_solver* s, const float* rho, const float* ux, const float* uy,
+                                          const float* uz, char* error, size_t error_capacity);
+/* Replaces the populations with caller-supplied logical values (q*cells) at the current parity. */
+CFD_LBM_API int cfd_lbm_upload_populations(cfd_lbm_solver* s, const float* logical, char* error,
+                                           size_t error_capacity);
+/* logical_populations -- esoteric_pull.hpp:257-267 (q*cells).  Wall cells report w_d * rho_wall. */
+CFD_LBM_API int cfd_lbm_download_populations(cfd_lbm_solver* s, float* logical, char* error,
+                                             size_t error_capacity);
+
+/* set_solid_mask / clear_solid_mask -- esoteric_pull.hpp:163-175, generalised to uint8 flags
+ * (cells values; NULL clears).  Mask value 1 in the reference maps to CFD_LBM_WALL. */
+CFD_LBM_API int cfd_lbm_set_flags(cfd_lbm_solver* s, const uint8_t* flags, char* error, size_t error_capacity);
+/* Wall velocity for flag CFD_LBM_WALL|index (index 1..7); src/lbm/d2q9.cpp:109-115,217-223. */
+CFD_LBM_API int cfd_lbm_set_wall_velocity(cfd_lbm_solver* s, int index, float ux, float uy, float uz, char* error,
+                                          size_t error_capacity);
+/* Partial bounce-back porosity gamma in [0,1] per cell (cells values; NULL clears). */
+CFD_LBM_API int cfd_lbm_set_porosity(cfd_lbm_solver* s, const float* gamma, char* error, size_t error_capacity);
+/* Inlet peak velocity and outlet density for CFD_LBM_VELOCITY_INLET / CFD_LBM_PRESSURE_OUTLET cells. */
+CFD_LBM_API int cfd_lbm_set_inlet_outlet(cfd_lbm_solver* s, float u_peak, float rho_out, char* error,
+                                         size_t error_capacity);
+/* set_local_body_acceleration / clear -- esoteric_pull.hpp:144-159 (cells each; NULL clears). */
+CFD_LBM_API int cfd_lbm_set_local_acceleration(cfd_lbm_solver* s, const float* ax, const float* ay,
+                                               const float* az, char* error, size_t error_capacity);
+
+/* step(count) -- esoteric_pull.hpp:139-142 / esoteric_pull_sycl.hpp:87-211: one kernel launch per
+ * time step, enqueued on the solver's stream. */
+CFD_LBM_API int cfd_lbm_step(cfd_lbm_solver* s, uint64_t count, char* error, size_t error_capacity);
+/* step(count) bracketed by HIP events on the solver's stream; *milliseconds = elapsed GPU time. */
+CFD_LBM_API int cfd_lbm_step_timed(cfd_lbm_solver* s, uint64_t count, float* milliseconds, char* error,
+                                   size_t error_capacity);
+/* wait() -- esoteric_pull_sycl.hpp:213-216. */
+CFD_LBM_API int cfd_lbm_synchronize(cfd_lbm_solver* s, char* error, size_t error_capacity);
+
+/* download_macroscopic -- esoteric_pull_sycl.hpp:309-325 (exactly 4*cells floats cross PCIe).
+ * Fluid u = m/rho + a/2; wall cells report (rho_wall, 0, 0, 0) -- esoteric_pull.hpp:180-209. */
+CFD_LBM_API int cfd_lbm_download_macroscopic(cfd_lbm_solver* s, float* rho, float* ux, float* uy, float* uz,
+                                             char* error, size_t error_capacity);
+/* Same in double precision (exact for FP64 storage). */
+CFD_LBM_API int cfd_lbm_download_macroscopic_f64(cfd_lbm_solver* s, double* rho, double* ux, double* uy, double* uz,
+                                                 char* error, size_t error_capacity);
+/* refresh_device_macroscopic -- esoteric_pull_sycl.hpp:229-307: device pointers to rho, ux, uy, uz
+ * (cells each, element type float, or double for FP64 storage), valid until the next refresh. */
+CFD_LBM_API int cfd_lbm_refresh_device_macroscopic(cfd_lbm_solver* s, void** rho, void** ux, void** uy, void** uz,
+                                                   char* error, size_t error_capacity);
+/* total_mass -- esoteric_pull_sycl.hpp:327-339, accumulated in FP64 (the reference uses FP32). */
+CFD_LBM_API int cfd_lbm_total_mass(cfd_lbm_solver* s, double* mass, char* error, size_t error_capacity);
+/* mass / kinetic_energy / max_speed -- esoteric_pull.hpp:211-255 in one device pass (FP64 sums;
+ * wall cells excluded from energy and speed). */
+CFD_LBM_API int cfd_lbm_diagnostics(cfd_lbm_solver* s, double* mass, double* kinetic_energy, double* max_speed,
+                                    char* error, size_t error_capacity);
+
+CFD_LBM_API int cfd_lbm_time_step(cfd_lbm_solver* s, uint64_t* time_step, char* error, size_t error_capacity);
+CFD_LBM_API int cfd_lbm_cells(cfd_lbm_solver* s, size_t* cells, char* error, size_t error_capacity);
+/* population_bytes -- esoteric_pull.hpp:272-274 (q*cells*bytes per population). */
+CFD_LBM_API int cfd_lbm_population_bytes(cfd_lbm_solver* s, size_t* bytes, char* error, size_t error_capacity);
+/* resident_bytes -- esoteric_pull_sycl.hpp:368-372 (all device allocations of this solver). */
+CFD_LBM_API int cfd_lbm_resident_bytes(cfd_lbm_solver* s, size_t* bytes, char* error, size_t error_capacity);
+/* device_name -- esoteric_pull_sycl.hpp:362-364. */
+CFD_LBM_API int cfd_lbm_device_name(cfd_lbm_solver* s, char* name, size_t name_capacity, char* error,
+                                    size_t error_capacity);
+/* queue() -- esoteric_pull_sycl.hpp:358-359: the solver's hipStream_t. */
+CFD_LBM_API int cfd_lbm_stream(cfd_lbm_solver* s, void** stream, char* error, size_t error_capacity);
+/* transfer_stats / reset_transfer_stats -- esoteric_pull_sycl.hpp:373-374,
+ * include/cfd/core/device_residency.hpp:11-42: {h2d, d2h, d2d bytes, synchronizations}. */
+CFD_LBM_API int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int reset, char* error,
+                                       size_t error_capacity);
+
+/* Debug: the slots (d'*cells + n') read (load) and written (store) for every (d, n) by the
+ * step kernel at the given parity, computed by the kernel's own index code (q*cells each).
+ * Wall cells report -1.  Periodic lattices follow esoteric_pull.hpp:349-382. */
+CFD_LBM_API int cfd_lbm_index_map(cfd_lbm_solver* s, int parity, int64_t* load_slots, int64_t* store_slots,
+                                  char* error, size_t error_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFD_LBM_C_API_H */

@@ -1,0 +1,439 @@
+"""ctypes access to the CPU checker libraries.  TEST INFRASTRUCTURE ONLY.
+
+``Oracle`` drives ``oracle/liblbm_oracle.so`` (the CPU restatement of the reference's
+esoteric-pull path, ``lbm_oracle.cpp``); ``Ref*`` classes drive ``oracle/_ref/libref_lbm*.so``,
+built from the reference's own sources by ``oracle/Makefile``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liblbm_oracle.so"
+REF_SO = HERE / "_ref" / "libref_lbm.so"
+REF_NATIVE_SO = HERE / "_ref" / "libref_lbm_native.so"
+
+STORAGE = {"fp32": 0, "fp16s": 1, "fp64": 2}
+FLUID, INLET, OUTLET, WALL = 0x00, 0x01, 0x02, 0x80
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build(reference: str | None = None, quiet: bool = True) -> None:
+    """Run oracle/Makefile (the restatement always; _ref only where the reference tree exists)."""
+    cmd = ["make", "-C", str(HERE), "all"]
+    if reference:
+        cmd.append(f"REF={reference}")
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL if quiet else None)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            build()
+        L = C.CDLL(str(ORACLE_SO))
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_create.argtypes = [C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
+                                 C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_flags.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_wall_velocity.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_set_gamma.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_local_acceleration.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_inlet_outlet.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_init_fields.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_init_uniform.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_init_taylor_green.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint64]
+        L.orc_step.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_macroscopic.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        for name in ("orc_mass", "orc_kinetic_energy", "orc_max_speed"):
+            getattr(L, name).argtypes = [C.c_void_p]
+            getattr(L, name).restype = C.c_double
+        L.orc_logical.argtypes = [C.c_void_p, _dp]
+        L.orc_set_logical.argtypes = [C.c_void_p, _dp]
+        L.orc_raw.argtypes = [C.c_void_p, _dp]
+        for name in ("orc_load_map", "orc_store_map", "orc_bb_load_map"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int, _i64p]
+        L.orc_time_step.argtypes = [C.c_void_p]
+        L.orc_time_step.restype = C.c_uint64
+        L.orc_set_time_step.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_cells.argtypes = [C.c_void_p]
+        L.orc_cells.restype = C.c_size_t
+        L.orc_f32_to_f16.argtypes = [C.c_float]
+        L.orc_f32_to_f16.restype = C.c_uint16
+        L.orc_f16_to_f32.argtypes = [C.c_uint16]
+        L.orc_f16_to_f32.restype = C.c_float
+        L.orc_uniform01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_uniform01.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, L=None, prefix="orc") -> None:
+    if rc != 0:
+        L = L or lib()
+        raise RuntimeError(getattr(L, f"{prefix}_last_error")().decode())
+
+
+class Oracle:
+    """CPU restatement of EsotericPullSolver<D> (fp32 / fp16-shifted / fp64 storage)."""
+
+    def __init__(self, q, nx, ny, nz, tau, accel=(0.0, 0.0, 0.0), storage="fp32"):
+        self.L = lib()
+        self.q = q
+        self.shape = (nz if q != 9 else 1, ny, nx)
+        h = C.c_void_p()
+        _check(self.L.orc_create(q, STORAGE[storage], nx, ny, nz, tau, *accel, C.byref(h)))
+        self.h = h
+        self.cells = int(self.L.orc_cells(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def set_flags(self, flags):
+        if flags is None:
+            _check(self.L.orc_set_flags(self.h, None))
+            return
+        self._flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        assert self._flags.size == self.cells
+        _check(self.L.orc_set_flags(self.h, self._flags.ctypes.data))
+
+    def set_wall_velocity(self, index, u):
+        _check(self.L.orc_set_wall_velocity(self.h, index, *map(float, u)))
+
+    def set_gamma(self, gamma):
+        if gamma is None:
+            _check(self.L.orc_set_gamma(self.h, None))
+            return
+        g = np.ascontiguousarray(gamma, dtype=np.float64).reshape(-1)
+        _check(self.L.orc_set_gamma(self.h, g.ctypes.data))
+
+    def set_local_acceleration(self, ax, ay=None, az=None):
+        if ax is None:
+            _check(self.L.orc_set_local_acceleration(self.h, None, None, None))
+            return
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (ax, ay, az)]
+        _check(self.L.orc_set_local_acceleration(self.h, *(a.ctypes.data for a in arrs)))
+
+    def set_inlet_outlet(self, u_peak, rho_out=1.0):
+        _check(self.L.orc_set_inlet_outlet(self.h, u_peak, rho_out))
+
+    def init_fields(self, rho, ux, uy, uz):
+        a = [np.ascontiguousarray(v, dtype=np.float64).reshape(-1) for v in (rho, ux, uy, uz)]
+        _check(self.L.orc_init_fields(self.h, *a))
+
+    def init_uniform(self, rho=1.0, ux=0.0, uy=0.0, uz=0.0):
+        _check(self.L.orc_init_uniform(self.h, rho, ux, uy, uz))
+
+    def init_taylor_green(self, amplitude=0.03, noise=0.0, seed=0):
+        _check(self.L.orc_init_taylor_green(self.h, amplitude, noise, seed))
+
+    def step(self, count=1):
+        _check(self.L.orc_step(self.h, count))
+
+    def macroscopic(self):
+        out = [np.empty(self.cells) for _ in range(4)]
+        _check(self.L.orc_macroscopic(self.h, *out))
+        return [o.reshape(self.shape) for o in out]
+
+    def mass(self):
+        return self.L.orc_mass(self.h)
+
+    def kinetic_energy(self):
+        return self.L.orc_kinetic_energy(self.h)
+
+    def max_speed(self):
+        return self.L.orc_max_speed(self.h)
+
+    def logical(self):
+        out = np.empty(self.q * self.cells)
+        _check(self.L.orc_logical(self.h, out))
+        return out.reshape(self.q, self.cells)
+
+    def set_logical(self, pops):
+        a = np.ascontiguousarray(pops, dtype=np.float64).reshape(-1)
+        _check(self.L.orc_set_logical(self.h, a))
+
+    def raw(self):
+        out = np.empty(self.q * self.cells)
+        _check(self.L.orc_raw(self.h, out))
+        return out.reshape(self.q, self.cells)
+
+    def _map(self, fn, parity):
+        out = np.empty(self.q * self.cells, dtype=np.int64)
+        _check(fn(self.h, parity, out))
+        return out.reshape(self.q, self.cells)
+
+    def load_map(self, parity):
+        return self._map(self.L.orc_load_map, parity)
+
+    def store_map(self, parity):
+        return self._map(self.L.orc_store_map, parity)
+
+    def bb_load_map(self, parity):
+        return self._map(self.L.orc_bb_load_map, parity)
+
+    @property
+    def time_step(self):
+        return int(self.L.orc_time_step(self.h))
+
+    @time_step.setter
+    def time_step(self, t):
+        self.L.orc_set_time_step(self.h, t)
+
+
+def f32_to_f16_bits(values):
+    L = lib()
+    return np.array([L.orc_f32_to_f16(float(v)) for v in np.asarray(values, dtype=np.float32)], dtype=np.uint16)
+
+
+def f16_bits_to_f32(bits):
+    L = lib()
+    return np.array([L.orc_f16_to_f32(int(b)) for b in np.asarray(bits, dtype=np.uint16)], dtype=np.float32)
+
+
+def uniform01(seed, cell, comp):
+    return lib().orc_uniform01(seed, cell, comp)
+
+
+# ---------------------------------------------------------------- compiled reference (oracle/_ref)
+_ref = {}
+
+
+def ref_available(native=False) -> bool:
+    return (REF_NATIVE_SO if native else REF_SO).exists()
+
+
+def ref_lib(native=False):
+    key = "native" if native else "exact"
+    if key not in _ref:
+        path = REF_NATIVE_SO if native else REF_SO
+        if not path.exists():
+            raise FileNotFoundError(f"{path} is not built (needs the reference tree; see oracle/Makefile)")
+        L = C.CDLL(str(path))
+        L.ref_last_error.restype = C.c_char_p
+        L.ref_eso_create.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_float,
+                                     C.c_float, C.c_float, C.POINTER(C.c_void_p)]
+        L.ref_eso_destroy.argtypes = [C.c_void_p]
+        L.ref_eso_init_tg.argtypes = [C.c_void_p, C.c_float]
+        L.ref_eso_init_uniform.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.ref_eso_step.argtypes = [C.c_void_p, C.c_size_t]
+        L.ref_eso_macroscopic.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp]
+        L.ref_eso_mass.argtypes = [C.c_void_p]
+        L.ref_eso_mass.restype = C.c_double
+        L.ref_eso_kinetic_energy.argtypes = [C.c_void_p]
+        L.ref_eso_kinetic_energy.restype = C.c_double
+        L.ref_eso_max_speed.argtypes = [C.c_void_p]
+        L.ref_eso_max_speed.restype = C.c_float
+        L.ref_eso_logical.argtypes = [C.c_void_p, _fp]
+        L.ref_eso_raw.argtypes = [C.c_void_p, _fp]
+        L.ref_eso_set_raw.argtypes = [C.c_void_p, _fp]
+        L.ref_eso_set_mask.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_eso_time_step.argtypes = [C.c_void_p]
+        L.ref_eso_time_step.restype = C.c_uint64
+        L.ref_pull_create.argtypes = L.ref_eso_create.argtypes
+        L.ref_pull_destroy.argtypes = [C.c_void_p]
+        L.ref_pull_init_tg.argtypes = [C.c_void_p, C.c_float]
+        L.ref_pull_init_uniform.argtypes = [C.c_void_p]
+        L.ref_pull_step.argtypes = [C.c_void_p, C.c_size_t]
+        L.ref_pull_macroscopic.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp]
+        L.ref_prec_create.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
+                                      C.POINTER(C.c_void_p)]
+        L.ref_prec_destroy.argtypes = [C.c_void_p]
+        L.ref_prec_init_tg.argtypes = [C.c_void_p, C.c_double]
+        L.ref_prec_step.argtypes = [C.c_void_p, C.c_size_t]
+        L.ref_prec_macroscopic.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.ref_d2q9_create.argtypes = [C.c_size_t, C.c_size_t, C.c_float, C.c_float, C.c_float,
+                                      C.POINTER(C.c_void_p)]
+        L.ref_d2q9_destroy.argtypes = [C.c_void_p]
+        L.ref_d2q9_set_solid.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t]
+        L.ref_d2q9_set_wall_velocity.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_float, C.c_float]
+        L.ref_d2q9_init_uniform.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
+        L.ref_d2q9_init_tg.argtypes = [C.c_void_p, C.c_float]
+        L.ref_d2q9_step.argtypes = [C.c_void_p, C.c_size_t]
+        L.ref_d2q9_fields.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        L.ref_d2q9_mass.argtypes = [C.c_void_p]
+        L.ref_d2q9_mass.restype = C.c_double
+        _ref[key] = L
+    return _ref[key]
+
+
+class RefEsoteric:
+    """The reference's EsotericPullSolver<D>, compiled from its sources."""
+
+    def __init__(self, q, nx, ny, nz, tau, accel=(0.0, 0.0, 0.0), native=False):
+        self.L = ref_lib(native)
+        self.q = q
+        h = C.c_void_p()
+        _check(self.L.ref_eso_create(q, nx, ny, nz, tau, *accel, C.byref(h)), self.L, "ref")
+        self.h = h
+        self.cells = nx * ny * (nz if q != 9 else 1)
+        self.shape = (nz if q != 9 else 1, ny, nx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_eso_destroy(self.h)
+            self.h = None
+
+    def init_taylor_green(self, a):
+        _check(self.L.ref_eso_init_tg(self.h, a), self.L, "ref")
+
+    def init_uniform(self, rho=1.0, ux=0.0, uy=0.0, uz=0.0):
+        _check(self.L.ref_eso_init_uniform(self.h, rho, ux, uy, uz), self.L, "ref")
+
+    def step(self, n=1):
+        _check(self.L.ref_eso_step(self.h, n), self.L, "ref")
+
+    def macroscopic(self):
+        out = [np.empty(self.cells, dtype=np.float32) for _ in range(4)]
+        _check(self.L.ref_eso_macroscopic(self.h, *out), self.L, "ref")
+        return [o.reshape(self.shape) for o in out]
+
+    def mass(self):
+        return self.L.ref_eso_mass(self.h)
+
+    def kinetic_energy(self):
+        return self.L.ref_eso_kinetic_energy(self.h)
+
+    def max_speed(self):
+        return self.L.ref_eso_max_speed(self.h)
+
+    def logical(self):
+        out = np.empty(self.q * self.cells, dtype=np.float32)
+        _check(self.L.ref_eso_logical(self.h, out), self.L, "ref")
+        return out.reshape(self.q, self.cells)
+
+    def raw(self):
+        out = np.empty(self.q * self.cells, dtype=np.float32)
+        _check(self.L.ref_eso_raw(self.h, out), self.L, "ref")
+        return out.reshape(self.q, self.cells)
+
+    def set_raw(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        assert a.size == self.q * self.cells
+        _check(self.L.ref_eso_set_raw(self.h, a), self.L, "ref")
+
+    def set_mask(self, mask):
+        if mask is None:
+            _check(self.L.ref_eso_set_mask(self.h, None), self.L, "ref")
+            return
+        self._mask = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        _check(self.L.ref_eso_set_mask(self.h, self._mask.ctypes.data), self.L, "ref")
+
+    @property
+    def time_step(self):
+        return int(self.L.ref_eso_time_step(self.h))
+
+
+class RefPull:
+    """The reference's two-grid OneStepPullSolver<D>."""
+
+    def __init__(self, q, nx, ny, nz, tau, accel=(0.0, 0.0, 0.0)):
+        self.L = ref_lib()
+        h = C.c_void_p()
+        _check(self.L.ref_pull_create(q, nx, ny, nz, tau, *accel, C.byref(h)), self.L, "ref")
+        self.h = h
+        self.cells = nx * ny * (nz if q != 9 else 1)
+        self.shape = (nz if q != 9 else 1, ny, nx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_pull_destroy(self.h)
+            self.h = None
+
+    def init_taylor_green(self, a):
+        _check(self.L.ref_pull_init_tg(self.h, a), self.L, "ref")
+
+    def init_uniform(self):
+        _check(self.L.ref_pull_init_uniform(self.h), self.L, "ref")
+
+    def step(self, n=1):
+        _check(self.L.ref_pull_step(self.h, n), self.L, "ref")
+
+    def macroscopic(self):
+        out = [np.empty(self.cells, dtype=np.float32) for _ in range(4)]
+        _check(self.L.ref_pull_macroscopic(self.h, *out), self.L, "ref")
+        return [o.reshape(self.shape) for o in out]
+
+
+class RefPrecision:
+    """The reference's PrecisionPullSolver<D, double> (two-grid, periodic, unforced FP64)."""
+
+    def __init__(self, q, nx, ny, nz, tau):
+        self.L = ref_lib()
+        h = C.c_void_p()
+        _check(self.L.ref_prec_create(q, nx, ny, nz, tau, C.byref(h)), self.L, "ref")
+        self.h = h
+        self.cells = nx * ny * (nz if q != 9 else 1)
+        self.shape = (nz if q != 9 else 1, ny, nx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_prec_destroy(self.h)
+            self.h = None
+
+    def init_taylor_green(self, a):
+        _check(self.L.ref_prec_init_tg(self.h, a), self.L, "ref")
+
+    def step(self, n=1):
+        _check(self.L.ref_prec_step(self.h, n), self.L, "ref")
+
+    def macroscopic(self):
+        out = [np.empty(self.cells) for _ in range(4)]
+        _check(self.L.ref_prec_macroscopic(self.h, *out), self.L, "ref")
+        return [o.reshape(self.shape) for o in out]
+
+
+class RefD2Q9:
+    """The reference's two-grid D2Q9Solver (halfway bounce-back, moving walls)."""
+
+    def __init__(self, nx, ny, tau, ax=0.0, ay=0.0):
+        self.L = ref_lib()
+        h = C.c_void_p()
+        _check(self.L.ref_d2q9_create(nx, ny, tau, ax, ay, C.byref(h)), self.L, "ref")
+        self.h = h
+        self.nx, self.ny = nx, ny
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_d2q9_destroy(self.h)
+            self.h = None
+
+    def set_solid(self, x, y):
+        _check(self.L.ref_d2q9_set_solid(self.h, x, y), self.L, "ref")
+
+    def set_wall_velocity(self, x, y, ux, uy):
+        _check(self.L.ref_d2q9_set_wall_velocity(self.h, x, y, ux, uy), self.L, "ref")
+
+    def init_uniform(self, rho=1.0, ux=0.0, uy=0.0):
+        _check(self.L.ref_d2q9_init_uniform(self.h, rho, ux, uy), self.L, "ref")
+
+    def init_taylor_green(self, a):
+        _check(self.L.ref_d2q9_init_tg(self.h, a), self.L, "ref")
+
+    def step(self, n=1):
+        _check(self.L.ref_d2q9_step(self.h, n), self.L, "ref")
+
+    def fields(self):
+        n = self.nx * self.ny
+        out = [np.empty(n, dtype=np.float32) for _ in range(3)]
+        _check(self.L.ref_d2q9_fields(self.h, *out), self.L, "ref")
+        return [o.reshape(self.ny, self.nx) for o in out]
+
+    def mass(self):
+        return self.L.ref_d2q9_mass(self.h)

@@ -1,0 +1,840 @@
+// lbm_oracle.cpp -- CPU restatement of the reference's esoteric-pull LBM path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP kernels in
+// cfd_solvers_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it.  The product path never links or calls it.
+//
+// What it restates (reference paths relative to the xsession/cfd_solvers root):
+//   * velocity sets and weights ........ include/cfd/solvers/lbm/descriptors.hpp:31-146
+//   * periodic_shift / neighbor_index .. include/cfd/solvers/lbm/esoteric_pull.hpp:40-46,304-310
+//   * equilibrium, Guo forcing ......... esoteric_pull.hpp:48-70
+//   * initialize (even layout) ......... esoteric_pull.hpp:117-137,312-333
+//   * load_cell / store_cell ........... esoteric_pull.hpp:349-382
+//   * accumulate_macroscopic, collide .. esoteric_pull.hpp:384-432
+//   * periodic step_once ............... esoteric_pull.hpp:494-514
+//   * bounded step_once_with_solids .... esoteric_pull.hpp:434-492, with the odd-parity repack at
+//                                        :483 corrected (read solid_next_[(i+1), n], not [.., forward])
+//   * moments / mass / logical pops .... esoteric_pull.hpp:180-267
+//   * moving-wall correction ........... src/lbm/d2q9.cpp:217-223 (+6 w rho_w c.u_w, rho_w = max(rho,1e-8))
+//   * local body acceleration .......... esoteric_pull.hpp:144-159,414-420
+//
+// Extensions the reference does not have (spec restated here, pinned by closed forms in tests/):
+//   * seeded noise on the Taylor-Green initial velocity (SplitMix64 counter hash),
+//   * 3-D velocity inlet / pressure outlet with Zou-He-style densities and equilibrium populations,
+//   * partial bounce-back porosity gamma (Walsh, Burwinkle & Saar 2009 form),
+//   * FP16 shifted-population storage h = half_rne((f - w_d) * 4096).
+//
+// Arithmetic follows the reference operation by operation (same association order) and the
+// library is built with -ffp-contract=off so that it is reproducible bit for bit.
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------- velocity sets
+// Same direction order and float weights as descriptors.hpp (pairs (i, i+1) are opposite).
+template <int Q> struct Lat;
+template <> struct Lat<9> {
+    static constexpr int dims = 2;
+    static constexpr int cx[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
+    static constexpr int cy[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+    static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    static constexpr float w[9] = {4.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F, 1.0F / 9.0F,
+                                   1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+};
+template <> struct Lat<19> {
+    static constexpr int dims = 3;
+    static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+    static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+    static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+    static constexpr float w[19] = {1.0F / 3.0F,  1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 18.0F,
+                                    1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
+                                    1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
+                                    1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+};
+template <> struct Lat<27> {
+    static constexpr int dims = 3;
+    static constexpr int cx[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0,
+                                   1, -1, 1, -1, 1, -1, 1, -1};
+    static constexpr int cy[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1,
+                                   1, -1, 1, -1, -1, 1, -1, 1};
+    static constexpr int cz[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1,
+                                   1, -1, -1, 1, 1, -1, -1, 1};
+    static constexpr float w[27] = {
+        8.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,  2.0F / 27.0F,
+        1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,
+        1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 216.0F, 1.0F / 216.0F,
+        1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F};
+};
+
+inline int opposite(int i) { return i == 0 ? 0 : ((i & 1) != 0 ? i + 1 : i - 1); }
+
+// ---------------------------------------------------------------- flags
+// uint8 cell flags shared with the HIP path.
+constexpr uint8_t kFluid = 0x00;
+constexpr uint8_t kInlet = 0x01;
+constexpr uint8_t kOutlet = 0x02;
+constexpr uint8_t kWall = 0x80;  // bit 7: halfway bounce-back wall; bits 0-2: wall-velocity index
+inline bool is_wall(uint8_t f) { return (f & kWall) != 0; }
+
+// ---------------------------------------------------------------- FP16 codec (IEEE binary16, RNE)
+inline uint16_t f32_to_f16_rne(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000U;
+    const uint32_t ax = x & 0x7fffffffU;
+    if (ax > 0x7f800000U) return static_cast<uint16_t>(sign | 0x7e00U);
+    if (ax >= 0x477ff000U) return static_cast<uint16_t>(sign | 0x7c00U);  // >= 65520 rounds to inf
+    if (ax < 0x38800000U) {                                                  // below 2^-14: subnormal
+        if (ax <= 0x33000000U) return static_cast<uint16_t>(sign);           // <= 2^-25 rounds to 0
+        const int e = static_cast<int>(ax >> 23);
+        const uint32_t m = (ax & 0x7fffffU) | 0x800000U;
+        const int shift = 126 - e;
+        uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1U << shift) - 1U);
+        const uint32_t half = 1U << (shift - 1);
+        if (rem > half || (rem == half && (q & 1U))) ++q;
+        return static_cast<uint16_t>(sign | q);
+    }
+    const uint32_t e16 = (ax >> 23) - 127U + 15U;
+    const uint32_t m = ax & 0x7fffffU;
+    uint32_t q = (e16 << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffU;
+    if (rem > 0x1000U || (rem == 0x1000U && (q & 1U))) ++q;
+    return static_cast<uint16_t>(sign | q);
+}
+
+inline float f16_to_f32(uint16_t h) {
+    const uint32_t sign = (static_cast<uint32_t>(h) & 0x8000U) << 16;
+    const uint32_t e = (h >> 10) & 0x1fU;
+    const uint32_t m = h & 0x3ffU;
+    if (e == 0U) {
+        const float v = static_cast<float>(m) * 0x1p-24F;
+        return sign ? -v : v;
+    }
+    uint32_t bits;
+    if (e == 31U) bits = sign | 0x7f800000U | (m << 13);
+    else bits = sign | ((e - 15U + 127U) << 23) | (m << 13);
+    float out;
+    std::memcpy(&out, &bits, 4);
+    return out;
+}
+
+constexpr float kHalfScale = 4096.0F;          // fixed per-lattice scale (power of two: exact)
+constexpr float kHalfInvScale = 1.0F / 4096.0F;
+
+struct Half {
+    uint16_t bits;
+};
+
+template <class Store, class Real, int Q> struct Codec;
+template <class Real, int Q> struct Codec<Real, Real, Q> {
+    static Real load(Real v, int) { return v; }
+    static Real store(Real v, int) { return v; }
+};
+template <int Q> struct Codec<Half, float, Q> {
+    static float load(Half h, int d) { return f16_to_f32(h.bits) * kHalfInvScale + Lat<Q>::w[d]; }
+    static Half store(float v, int d) { return Half{f32_to_f16_rne((v - Lat<Q>::w[d]) * kHalfScale)}; }
+};
+
+// ---------------------------------------------------------------- seeded noise (SplitMix64)
+// Steele, Lea & Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014.
+inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+// Uniform in [0,1) on a 2^-24 grid: exact in float and double.
+template <class Real> inline Real uniform01(uint64_t seed, uint64_t cell, uint64_t component) {
+    const uint64_t h = splitmix64(splitmix64(seed) + 4ULL * cell + component);
+    return static_cast<Real>(h >> 40) * static_cast<Real>(0x1p-24);
+}
+
+// ---------------------------------------------------------------- oracle
+class OracleBase {
+public:
+    virtual ~OracleBase() = default;
+    virtual void set_flags(const uint8_t* flags) = 0;
+    virtual void set_wall_velocity(int index, double ux, double uy, double uz) = 0;
+    virtual void set_gamma(const double* gamma) = 0;
+    virtual void set_local_acceleration(const double* ax, const double* ay, const double* az) = 0;
+    virtual void set_inlet_outlet(double u_peak, double rho_out) = 0;
+    virtual void init_fields(const double* rho, const double* ux, const double* uy, const double* uz) = 0;
+    virtual void init_uniform(double rho, double ux, double uy, double uz) = 0;
+    virtual void init_taylor_green(double amplitude, double noise, uint64_t seed) = 0;
+    virtual void step(uint64_t count) = 0;
+    virtual void macroscopic(double* rho, double* ux, double* uy, double* uz) const = 0;
+    virtual double mass() const = 0;
+    virtual double kinetic_energy() const = 0;
+    virtual double max_speed() const = 0;
+    virtual void logical(double* out) const = 0;
+    virtual void set_logical(const double* in) = 0;
+    virtual void raw(double* out) const = 0;
+    virtual void load_map(int parity, int64_t* out) const = 0;
+    virtual void store_map(int parity, int64_t* out) const = 0;
+    virtual void bb_load_map(int parity, int64_t* out) const = 0;
+    virtual uint64_t time_step() const = 0;
+    virtual void set_time_step(uint64_t t) = 0;
+    virtual size_t cells() const = 0;
+};
+
+template <int Q, class Real, class Store> class Oracle final : public OracleBase {
+    using L = Lat<Q>;
+    using C = Codec<Store, Real, Q>;
+
+public:
+    Oracle(size_t nx, size_t ny, size_t nz, double tau, double ax, double ay, double az)
+        : nx_(nx), ny_(ny), nz_(L::dims == 2 ? 1 : nz), tau_(static_cast<Real>(tau)),
+          ax_(static_cast<Real>(ax)), ay_(static_cast<Real>(ay)), az_(static_cast<Real>(az)) {
+        if (nx_ < 2 || ny_ < 2) throw std::invalid_argument("LBM grid requires nx, ny >= 2");
+        if (L::dims == 3 && nz_ < 2) throw std::invalid_argument("3-D LBM grid requires nz >= 2");
+        if (!(tau_ > static_cast<Real>(0.5))) throw std::invalid_argument("LBM tau must be > 0.5");
+        N_ = nx_ * ny_ * nz_;
+        F_.resize(static_cast<size_t>(Q) * N_);
+        for (auto& v : wall_u_) v[0] = v[1] = v[2] = Real(0);
+        init_uniform(1.0, 0.0, 0.0, 0.0);
+    }
+
+    size_t cells() const override { return N_; }
+    uint64_t time_step() const override { return t_; }
+    void set_time_step(uint64_t t) override { t_ = t; }
+
+    void set_flags(const uint8_t* flags) override {
+        if (!flags) {
+            flags_.clear();
+            return;
+        }
+        flags_.assign(flags, flags + N_);
+    }
+    void set_wall_velocity(int index, double ux, double uy, double uz) override {
+        if (index < 1 || index > 7) throw std::invalid_argument("wall velocity index must be 1..7");
+        wall_u_[index][0] = static_cast<Real>(ux);
+        wall_u_[index][1] = static_cast<Real>(uy);
+        wall_u_[index][2] = static_cast<Real>(uz);
+    }
+    void set_gamma(const double* gamma) override {
+        if (!gamma) {
+            gamma_.clear();
+            return;
+        }
+        gamma_.resize(N_);
+        for (size_t n = 0; n < N_; ++n) gamma_[n] = static_cast<Real>(gamma[n]);
+    }
+    void set_local_acceleration(const double* ax, const double* ay, const double* az) override {
+        if (!ax) {
+            lax_.clear();
+            lay_.clear();
+            laz_.clear();
+            return;
+        }
+        lax_.resize(N_);
+        lay_.resize(N_);
+        laz_.resize(N_);
+        for (size_t n = 0; n < N_; ++n) {
+            lax_[n] = static_cast<Real>(ax[n]);
+            lay_[n] = static_cast<Real>(ay[n]);
+            laz_[n] = static_cast<Real>(az[n]);
+        }
+    }
+    void set_inlet_outlet(double u_peak, double rho_out) override {
+        u_peak_ = static_cast<Real>(u_peak);
+        rho_out_ = static_cast<Real>(rho_out);
+    }
+
+    // esoteric_pull.hpp:312-333 -- equilibrium written in the even layout; solids at rest.
+    template <class Init> void initialize(Init&& init) {
+        t_ = 0;
+#pragma omp parallel for schedule(static)
+        for (int64_t nn = 0; nn < static_cast<int64_t>(N_); ++nn) {
+            const size_t n = static_cast<size_t>(nn);
+            size_t x, y, z;
+            coords(n, x, y, z);
+            Real s[4];
+            init(n, x, y, z, s);
+            const bool solid = !flags_.empty() && is_wall(flags_[n]);
+            const Real ux = solid ? Real(0) : s[1];
+            const Real uy = solid ? Real(0) : s[2];
+            const Real uz = solid ? Real(0) : s[3];
+            at(0, n) = C::store(feq(0, s[0], ux, uy, uz), 0);
+            for (int i = 1; i < Q; i += 2) {
+                const size_t nb = neighbor(x, y, z, i + 1);
+                at(i, nb) = C::store(feq(i, s[0], ux, uy, uz), i);
+                at(i + 1, n) = C::store(feq(i + 1, s[0], ux, uy, uz), i + 1);
+            }
+        }
+    }
+
+    void init_fields(const double* rho, const double* ux, const double* uy, const double* uz) override {
+        initialize([&](size_t n, size_t, size_t, size_t, Real* s) {
+            s[0] = static_cast<Real>(rho[n]);
+            s[1] = static_cast<Real>(ux[n]);
+            s[2] = static_cast<Real>(uy[n]);
+            s[3] = static_cast<Real>(uz[n]);
+        });
+    }
+    void init_uniform(double rho, double ux, double uy, double uz) override {
+        const Real r = static_cast<Real>(rho), a = static_cast<Real>(ux), b = static_cast<Real>(uy),
+                   c = static_cast<Real>(uz);
+        initialize([=](size_t, size_t, size_t, size_t, Real* s) {
+            s[0] = r;
+            s[1] = a;
+            s[2] = b;
+            s[3] = c;
+        });
+    }
+    // esoteric_pull.hpp:121-137, plus optional seeded uniform noise of amplitude `noise` per component.
+    void init_taylor_green(double amplitude_d, double noise_d, uint64_t seed) override {
+        const Real amplitude = static_cast<Real>(amplitude_d);
+        const Real noise = static_cast<Real>(noise_d);
+        const Real two_pi = Real(2) * static_cast<Real>(3.14159265358979323846);
+        initialize([=](size_t n, size_t x, size_t y, size_t z, Real* s) {
+            const Real xf = (static_cast<Real>(x) + Real(0.5)) / static_cast<Real>(nx_);
+            const Real yf = (static_cast<Real>(y) + Real(0.5)) / static_cast<Real>(ny_);
+            const Real sx = std::sin(two_pi * xf);
+            const Real cx = std::cos(two_pi * xf);
+            const Real sy = std::sin(two_pi * yf);
+            const Real cy = std::cos(two_pi * yf);
+            Real z_factor = Real(1);
+            if (L::dims == 3) {
+                const Real zf = (static_cast<Real>(z) + Real(0.5)) / static_cast<Real>(nz_);
+                z_factor = std::cos(two_pi * zf);
+            }
+            s[0] = Real(1);
+            s[1] = amplitude * sx * cy * z_factor;
+            s[2] = -amplitude * cx * sy * z_factor;
+            s[3] = Real(0);
+            if (noise != Real(0)) {
+                s[1] = s[1] + noise * (Real(2) * uniform01<Real>(seed, n, 0) - Real(1));
+                s[2] = s[2] + noise * (Real(2) * uniform01<Real>(seed, n, 1) - Real(1));
+                if (L::dims == 3) s[3] = s[3] + noise * (Real(2) * uniform01<Real>(seed, n, 2) - Real(1));
+            }
+        });
+    }
+
+    void step(uint64_t count) override {
+        for (uint64_t k = 0; k < count; ++k) {
+            if (flags_.empty()) step_periodic();
+            else step_bounded();
+        }
+    }
+
+    void macroscopic(double* rho_out, double* ux_out, double* uy_out, double* uz_out) const override {
+#pragma omp parallel for schedule(static)
+        for (int64_t nn = 0; nn < static_cast<int64_t>(N_); ++nn) {
+            const size_t n = static_cast<size_t>(nn);
+            Real m[4];
+            cell_moments(n, m);
+            rho_out[n] = static_cast<double>(m[0]);
+            ux_out[n] = static_cast<double>(m[1]);
+            uy_out[n] = static_cast<double>(m[2]);
+            uz_out[n] = static_cast<double>(m[3]);
+        }
+    }
+
+    // esoteric_pull.hpp:211-220: per-cell sum of loaded populations, accumulated in double.
+    double mass() const override {
+        double total = 0.0;
+        for (size_t n = 0; n < N_; ++n) {
+            Real fin[Q];
+            load(n, fin);
+            Real rho = Real(0);
+            for (int d = 0; d < Q; ++d) rho += fin[d];
+            total += static_cast<double>(rho);
+        }
+        return total;
+    }
+
+    // 0.5 * sum rho |u|^2 over non-wall cells (wall cells excluded as in src/lbm/d2q9.cpp mass/energy).
+    double kinetic_energy() const override {
+        double total = 0.0;
+        for (size_t n = 0; n < N_; ++n) {
+            if (!flags_.empty() && is_wall(flags_[n])) continue;
+            Real m[4];
+            cell_moments(n, m);
+            total += static_cast<double>(m[0]) * (static_cast<double>(m[1]) * m[1] +
+                                                  static_cast<double>(m[2]) * m[2] + static_cast<double>(m[3]) * m[3]);
+        }
+        return 0.5 * total;
+    }
+
+    double max_speed() const override {
+        double best = 0.0;
+        for (size_t n = 0; n < N_; ++n) {
+            if (!flags_.empty() && is_wall(flags_[n])) continue;
+            Real m[4];
+            cell_moments(n, m);
+            const Real s = std::sqrt(m[1] * m[1] + m[2] * m[2] + m[3] * m[3]);
+            best = std::max(best, static_cast<double>(s));
+        }
+        return best;
+    }
+
+    void logical(double* out) const override {
+        for (size_t n = 0; n < N_; ++n) {
+            Real fin[Q];
+            load(n, fin);
+            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = static_cast<double>(fin[d]);
+        }
+    }
+    // Writes logical populations (the inverse of load) at the current parity.
+    void set_logical(const double* in) override {
+        const bool odd = (t_ & 1U) != 0U;
+        for (size_t n = 0; n < N_; ++n) {
+            size_t x, y, z;
+            coords(n, x, y, z);
+            at(0, n) = C::store(static_cast<Real>(in[n]), 0);
+            for (int i = 1; i < Q; i += 2) {
+                const size_t nb = neighbor(x, y, z, i + 1);
+                const Real a = static_cast<Real>(in[static_cast<size_t>(i) * N_ + n]);
+                const Real b = static_cast<Real>(in[static_cast<size_t>(i + 1) * N_ + n]);
+                if (odd) {
+                    at(i + 1, nb) = C::store(a, i + 1);
+                    at(i, n) = C::store(b, i);
+                } else {
+                    at(i, nb) = C::store(a, i);
+                    at(i + 1, n) = C::store(b, i + 1);
+                }
+            }
+        }
+    }
+    void raw(double* out) const override {
+        for (size_t k = 0; k < F_.size(); ++k) out[k] = static_cast<double>(C::load(F_[k], static_cast<int>(k / N_)));
+    }
+
+    // Slot (d' * N + n') from which fin[d] of cell n is read at the given parity (esoteric_pull.hpp:349-365).
+    void load_map(int parity, int64_t* out) const override {
+        for (size_t n = 0; n < N_; ++n) {
+            size_t x, y, z;
+            coords(n, x, y, z);
+            out[n] = slot(0, n);
+            for (int i = 1; i < Q; i += 2) {
+                const size_t nb = neighbor(x, y, z, i + 1);
+                if (parity & 1) {
+                    out[static_cast<size_t>(i) * N_ + n] = slot(i + 1, nb);
+                    out[static_cast<size_t>(i + 1) * N_ + n] = slot(i, n);
+                } else {
+                    out[static_cast<size_t>(i) * N_ + n] = slot(i, nb);
+                    out[static_cast<size_t>(i + 1) * N_ + n] = slot(i + 1, n);
+                }
+            }
+        }
+    }
+    // Slot into which fout[d] of cell n is written at the given parity (esoteric_pull.hpp:367-382).
+    void store_map(int parity, int64_t* out) const override {
+        for (size_t n = 0; n < N_; ++n) {
+            size_t x, y, z;
+            coords(n, x, y, z);
+            out[n] = slot(0, n);
+            for (int i = 1; i < Q; i += 2) {
+                const size_t nb = neighbor(x, y, z, i + 1);
+                if (parity & 1) {
+                    out[static_cast<size_t>(i) * N_ + n] = slot(i, n);
+                    out[static_cast<size_t>(i + 1) * N_ + n] = slot(i + 1, nb);
+                } else {
+                    out[static_cast<size_t>(i) * N_ + n] = slot(i + 1, n);
+                    out[static_cast<size_t>(i + 1) * N_ + n] = slot(i, nb);
+                }
+            }
+        }
+    }
+    // Physical slot read by the single-pass in-place bounded kernel: a population whose upstream
+    // cell is a wall is read from the slot of the opposite parity, which holds the value this cell
+    // stored for the opposite direction one step earlier (halfway bounce-back).  -1 for wall cells.
+    void bb_load_map(int parity, int64_t* out) const override {
+        std::vector<int64_t> normal(static_cast<size_t>(Q) * N_), other(static_cast<size_t>(Q) * N_);
+        load_map(parity, normal.data());
+        load_map(parity ^ 1, other.data());
+        for (size_t n = 0; n < N_; ++n) {
+            size_t x, y, z;
+            coords(n, x, y, z);
+            const bool wall = !flags_.empty() && is_wall(flags_[n]);
+            for (int d = 0; d < Q; ++d) {
+                const size_t k = static_cast<size_t>(d) * N_ + n;
+                if (wall) {
+                    out[k] = -1;
+                    continue;
+                }
+                const size_t up = neighbor(x, y, z, opposite(d));
+                const bool bb = d != 0 && !flags_.empty() && is_wall(flags_[up]);
+                out[k] = bb ? other[k] : normal[k];
+            }
+        }
+    }
+
+private:
+    size_t nx_, ny_, nz_, N_{};
+    Real tau_, ax_, ay_, az_;
+    std::vector<Store> F_;
+    std::vector<uint8_t> flags_;
+    std::vector<Real> gamma_, lax_, lay_, laz_;
+    Real wall_u_[8][3];
+    Real u_peak_{0}, rho_out_{1};
+    uint64_t t_{0};
+
+    Store& at(int d, size_t n) { return F_[static_cast<size_t>(d) * N_ + n]; }
+    const Store& at(int d, size_t n) const { return F_[static_cast<size_t>(d) * N_ + n]; }
+    int64_t slot(int d, size_t n) const { return static_cast<int64_t>(static_cast<size_t>(d) * N_ + n); }
+
+    void coords(size_t n, size_t& x, size_t& y, size_t& z) const {
+        x = n % nx_;
+        const size_t yz = n / nx_;
+        y = yz % ny_;
+        z = yz / ny_;
+    }
+    static size_t shift(size_t c, int delta, size_t extent) {
+        if (delta > 0) return c + 1U == extent ? 0U : c + 1U;
+        if (delta < 0) return c == 0U ? extent - 1U : c - 1U;
+        return c;
+    }
+    size_t neighbor(size_t x, size_t y, size_t z, int d) const {
+        return (shift(z, L::cz[d], nz_) * ny_ + shift(y, L::cy[d], ny_)) * nx_ + shift(x, L::cx[d], nx_);
+    }
+
+    static Real w(int d) { return static_cast<Real>(L::w[d]); }
+    static Real feq(int d, Real rho, Real ux, Real uy, Real uz) {
+        const Real cu = Real(3) * (static_cast<Real>(L::cx[d]) * ux + static_cast<Real>(L::cy[d]) * uy +
+                                   static_cast<Real>(L::cz[d]) * uz);
+        const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
+        return w(d) * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
+    }
+    static Real guo(int d, Real rho, Real ux, Real uy, Real uz, Real ax, Real ay, Real az, Real omega) {
+        if (ax == Real(0) && ay == Real(0) && az == Real(0)) return Real(0);
+        const Real cx = static_cast<Real>(L::cx[d]), cy = static_cast<Real>(L::cy[d]), cz = static_cast<Real>(L::cz[d]);
+        const Real fx = rho * ax, fy = rho * ay, fz = rho * az;
+        const Real cu = cx * ux + cy * uy + cz * uz;
+        const Real cf = cx * fx + cy * fy + cz * fz;
+        const Real uf = ux * fx + uy * fy + uz * fz;
+        return w(d) * (Real(1) - Real(0.5) * omega) * (Real(3) * (cf - uf) + Real(9) * cu * cf);
+    }
+
+    void load(size_t n, Real* fin) const {
+        size_t x, y, z;
+        coords(n, x, y, z);
+        const bool odd = (t_ & 1U) != 0U;
+        fin[0] = C::load(at(0, n), 0);
+        for (int i = 1; i < Q; i += 2) {
+            const size_t nb = neighbor(x, y, z, i + 1);
+            if (odd) {
+                fin[i] = C::load(at(i + 1, nb), i + 1);
+                fin[i + 1] = C::load(at(i, n), i);
+            } else {
+                fin[i] = C::load(at(i, nb), i);
+                fin[i + 1] = C::load(at(i + 1, n), i + 1);
+            }
+        }
+    }
+    void store(size_t n, size_t x, size_t y, size_t z, bool odd, const Real* fout) {
+        at(0, n) = C::store(fout[0], 0);
+        for (int i = 1; i < Q; i += 2) {
+            const size_t nb = neighbor(x, y, z, i + 1);
+            if (odd) {
+                at(i, n) = C::store(fout[i], i);
+                at(i + 1, nb) = C::store(fout[i + 1], i + 1);
+            } else {
+                at(i + 1, n) = C::store(fout[i], i + 1);
+                at(i, nb) = C::store(fout[i + 1], i);
+            }
+        }
+    }
+
+    static void accumulate(const Real* fin, Real& rho, Real& ux, Real& uy, Real& uz) {
+        rho = ux = uy = uz = Real(0);
+        for (int d = 0; d < Q; ++d) {
+            const Real v = fin[d];
+            rho += v;
+            ux += v * static_cast<Real>(L::cx[d]);
+            uy += v * static_cast<Real>(L::cy[d]);
+            uz += v * static_cast<Real>(L::cz[d]);
+        }
+        if (rho > Real(0)) {
+            ux /= rho;
+            uy /= rho;
+            uz /= rho;
+        } else {
+            ux = uy = uz = Real(0);
+        }
+    }
+
+    void acceleration(size_t n, Real& ax, Real& ay, Real& az) const {
+        const bool local = !lax_.empty();
+        ax = ax_ + (local ? lax_[n] : Real(0));
+        ay = ay_ + (local ? lay_[n] : Real(0));
+        az = az_ + (local ? laz_[n] : Real(0));
+    }
+
+    Real inlet_velocity(size_t y, size_t z) const {
+        const Real eta = (static_cast<Real>(y) - Real(0.5)) / static_cast<Real>(ny_ - 2);
+        Real u = u_peak_ * (Real(4) * eta * (Real(1) - eta));
+        if (L::dims == 3) {
+            const Real zeta = (static_cast<Real>(z) - Real(0.5)) / static_cast<Real>(nz_ - 2);
+            u = u * (Real(4) * zeta * (Real(1) - zeta));
+        }
+        return u;
+    }
+    // Zou-He density of a velocity-inlet cell at x = 0 from the populations that come from the interior.
+    Real inlet_density(const Real* fin, Real uin) const {
+        Real s0 = Real(0), sn = Real(0);
+        for (int d = 0; d < Q; ++d) {
+            if (L::cx[d] == 0) s0 += fin[d];
+            else if (L::cx[d] < 0) sn += fin[d];
+        }
+        return (s0 + Real(2) * sn) / (Real(1) - uin);
+    }
+    // Zou-He normal velocity of a pressure-outlet cell at x = nx-1.
+    Real outlet_velocity(const Real* fin) const {
+        Real s0 = Real(0), sp = Real(0);
+        for (int d = 0; d < Q; ++d) {
+            if (L::cx[d] == 0) s0 += fin[d];
+            else if (L::cx[d] > 0) sp += fin[d];
+        }
+        return Real(-1) + (s0 + Real(2) * sp) / rho_out_;
+    }
+
+    // Collision of one non-wall cell; returns the density used for moving-wall corrections.
+    Real collide(size_t n, size_t y, size_t z, const Real* fin, Real* fout) const {
+        const uint8_t flag = flags_.empty() ? kFluid : flags_[n];
+        if (flag == kInlet) {
+            const Real uin = inlet_velocity(y, z);
+            const Real rho = inlet_density(fin, uin);
+            for (int d = 0; d < Q; ++d) fout[d] = feq(d, rho, uin, Real(0), Real(0));
+            return rho;
+        }
+        if (flag == kOutlet) {
+            const Real ux = outlet_velocity(fin);
+            for (int d = 0; d < Q; ++d) fout[d] = feq(d, rho_out_, ux, Real(0), Real(0));
+            return rho_out_;
+        }
+        Real rho, ux, uy, uz, ax, ay, az;
+        accumulate(fin, rho, ux, uy, uz);
+        acceleration(n, ax, ay, az);
+        ux += Real(0.5) * ax;
+        uy += Real(0.5) * ay;
+        uz += Real(0.5) * az;
+        const Real omega = Real(1) / tau_;
+        for (int d = 0; d < Q; ++d) {
+            Real p = fin[d];
+            p -= omega * (p - feq(d, rho, ux, uy, uz));
+            p += guo(d, rho, ux, uy, uz, ax, ay, az, omega);
+            fout[d] = p;
+        }
+        if (!gamma_.empty()) {
+            const Real g = gamma_[n];
+            Real mixed[Q];
+            for (int d = 0; d < Q; ++d) mixed[d] = (Real(1) - g) * fout[d] + g * fin[opposite(d)];
+            for (int d = 0; d < Q; ++d) fout[d] = mixed[d];
+        }
+        return rho;
+    }
+
+    // esoteric_pull.hpp:494-514
+    void step_periodic() {
+        const bool odd = (t_ & 1U) != 0U;
+#pragma omp parallel for schedule(static)
+        for (int64_t nn = 0; nn < static_cast<int64_t>(N_); ++nn) {
+            const size_t n = static_cast<size_t>(nn);
+            size_t x, y, z;
+            coords(n, x, y, z);
+            Real fin[Q], fout[Q];
+            load(n, fin);
+            collide(n, y, z, fin, fout);
+            store(n, x, y, z, odd, fout);
+        }
+        ++t_;
+    }
+
+    // esoteric_pull.hpp:434-492 with the :483 repack corrected; moving walls per src/lbm/d2q9.cpp:217-223.
+    void step_bounded() {
+        const size_t total = static_cast<size_t>(Q) * N_;
+        std::vector<Real> cur(total), nxt(total);
+#pragma omp parallel for schedule(static)
+        for (int64_t nn = 0; nn < static_cast<int64_t>(N_); ++nn) {
+            const size_t n = static_cast<size_t>(nn);
+            Real fin[Q];
+            load(n, fin);
+            for (int d = 0; d < Q; ++d) cur[static_cast<size_t>(d) * N_ + n] = fin[d];
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t nn = 0; nn < static_cast<int64_t>(N_); ++nn) {
+            const size_t n = static_cast<size_t>(nn);
+            if (is_wall(flags_[n])) {
+                for (int d = 0; d < Q; ++d) nxt[static_cast<size_t>(d) * N_ + n] = cur[static_cast<size_t>(d) * N_ + n];
+                continue;
+            }
+            size_t x, y, z;
+            coords(n, x, y, z);
+            Real fin[Q], fout[Q];
+            for (int d = 0; d < Q; ++d) fin[d] = cur[static_cast<size_t>(d) * N_ + n];
+            const Real rho = collide(n, y, z, fin, fout);
+            const Real rho_wall = std::max(rho, static_cast<Real>(1.0e-8));
+            for (int d = 0; d < Q; ++d) {
+                const size_t target = neighbor(x, y, z, d);
+                const uint8_t tf = flags_[target];
+                if (is_wall(tf)) {
+                    Real value = fout[d];
+                    const int k = tf & 7;
+                    if (k != 0) {
+                        const Real cu = static_cast<Real>(L::cx[d]) * wall_u_[k][0] +
+                                        static_cast<Real>(L::cy[d]) * wall_u_[k][1] +
+                                        static_cast<Real>(L::cz[d]) * wall_u_[k][2];
+                        value -= Real(6) * w(d) * rho_wall * cu;
+                    }
+                    nxt[static_cast<size_t>(opposite(d)) * N_ + n] = value;
+                } else {
+                    nxt[static_cast<size_t>(d) * N_ + target] = fout[d];
+                }
+            }
+        }
+        const bool new_odd = ((t_ + 1U) & 1U) != 0U;
+#pragma omp parallel for schedule(static)
+        for (int64_t nn = 0; nn < static_cast<int64_t>(N_); ++nn) {
+            const size_t n = static_cast<size_t>(nn);
+            size_t x, y, z;
+            coords(n, x, y, z);
+            at(0, n) = C::store(nxt[n], 0);
+            for (int i = 1; i < Q; i += 2) {
+                const size_t backward = neighbor(x, y, z, i + 1);
+                if (new_odd) {
+                    at(i, n) = C::store(nxt[static_cast<size_t>(i + 1) * N_ + n], i);  // corrected :483
+                    at(i + 1, backward) = C::store(nxt[static_cast<size_t>(i) * N_ + n], i + 1);
+                } else {
+                    at(i, backward) = C::store(nxt[static_cast<size_t>(i) * N_ + n], i);
+                    at(i + 1, n) = C::store(nxt[static_cast<size_t>(i + 1) * N_ + n], i + 1);
+                }
+            }
+        }
+        ++t_;
+    }
+
+    // Macroscopic state reported per cell (esoteric_pull.hpp:180-209): fluid u = m/rho + a/2,
+    // wall cells rho = sum of frozen populations and u = 0, inlet/outlet cells report the state
+    // their boundary rule imposes.
+    void cell_moments(size_t n, Real* m) const {
+        Real fin[Q];
+        load(n, fin);
+        const uint8_t flag = flags_.empty() ? kFluid : flags_[n];
+        if (is_wall(flag)) {
+            Real rho = Real(0);
+            for (int d = 0; d < Q; ++d) rho += fin[d];
+            m[0] = rho;
+            m[1] = m[2] = m[3] = Real(0);
+            return;
+        }
+        size_t x, y, z;
+        coords(n, x, y, z);
+        if (flag == kInlet) {
+            const Real uin = inlet_velocity(y, z);
+            m[0] = inlet_density(fin, uin);
+            m[1] = uin;
+            m[2] = m[3] = Real(0);
+            return;
+        }
+        if (flag == kOutlet) {
+            m[0] = rho_out_;
+            m[1] = outlet_velocity(fin);
+            m[2] = m[3] = Real(0);
+            return;
+        }
+        Real rho, ux, uy, uz, ax, ay, az;
+        accumulate(fin, rho, ux, uy, uz);
+        acceleration(n, ax, ay, az);
+        m[0] = rho;
+        m[1] = ux + Real(0.5) * ax;
+        m[2] = uy + Real(0.5) * ay;
+        m[3] = uz + Real(0.5) * az;
+    }
+};
+
+template <int Q> OracleBase* make_q(int storage, size_t nx, size_t ny, size_t nz, double tau, double ax, double ay,
+                                    double az) {
+    switch (storage) {
+    case 0: return new Oracle<Q, float, float>(nx, ny, nz, tau, ax, ay, az);
+    case 1: return new Oracle<Q, float, Half>(nx, ny, nz, tau, ax, ay, az);
+    case 2: return new Oracle<Q, double, double>(nx, ny, nz, tau, ax, ay, az);
+    default: throw std::invalid_argument("storage must be 0 (fp32), 1 (fp16 shifted) or 2 (fp64)");
+    }
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------- C ABI for the tests (ctypes)
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+thread_local char g_error[512];
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        std::snprintf(g_error, sizeof(g_error), "%s", e.what());
+        return 1;
+    } catch (...) {
+        std::snprintf(g_error, sizeof(g_error), "unknown exception");
+        return 2;
+    }
+}
+orc::OracleBase* H(void* h) { return static_cast<orc::OracleBase*>(h); }
+}  // namespace
+
+ORC_API const char* orc_last_error(void) { return g_error; }
+ORC_API int orc_create(int q, int storage, size_t nx, size_t ny, size_t nz, double tau, double ax, double ay,
+                       double az, void** out) {
+    return guarded([&] {
+        switch (q) {
+        case 9: *out = orc::make_q<9>(storage, nx, ny, nz, tau, ax, ay, az); break;
+        case 19: *out = orc::make_q<19>(storage, nx, ny, nz, tau, ax, ay, az); break;
+        case 27: *out = orc::make_q<27>(storage, nx, ny, nz, tau, ax, ay, az); break;
+        default: throw std::invalid_argument("lattice must have q = 9, 19 or 27");
+        }
+    });
+}
+ORC_API void orc_destroy(void* h) { delete H(h); }
+ORC_API int orc_set_flags(void* h, const uint8_t* f) { return guarded([&] { H(h)->set_flags(f); }); }
+ORC_API int orc_set_wall_velocity(void* h, int k, double ux, double uy, double uz) {
+    return guarded([&] { H(h)->set_wall_velocity(k, ux, uy, uz); });
+}
+ORC_API int orc_set_gamma(void* h, const double* g) { return guarded([&] { H(h)->set_gamma(g); }); }
+ORC_API int orc_set_local_acceleration(void* h, const double* ax, const double* ay, const double* az) {
+    return guarded([&] { H(h)->set_local_acceleration(ax, ay, az); });
+}
+ORC_API int orc_set_inlet_outlet(void* h, double u_peak, double rho_out) {
+    return guarded([&] { H(h)->set_inlet_outlet(u_peak, rho_out); });
+}
+ORC_API int orc_init_fields(void* h, const double* r, const double* ux, const double* uy, const double* uz) {
+    return guarded([&] { H(h)->init_fields(r, ux, uy, uz); });
+}
+ORC_API int orc_init_uniform(void* h, double r, double ux, double uy, double uz) {
+    return guarded([&] { H(h)->init_uniform(r, ux, uy, uz); });
+}
+ORC_API int orc_init_taylor_green(void* h, double a, double noise, uint64_t seed) {
+    return guarded([&] { H(h)->init_taylor_green(a, noise, seed); });
+}
+ORC_API int orc_step(void* h, uint64_t count) { return guarded([&] { H(h)->step(count); }); }
+ORC_API int orc_macroscopic(void* h, double* r, double* ux, double* uy, double* uz) {
+    return guarded([&] { H(h)->macroscopic(r, ux, uy, uz); });
+}
+ORC_API double orc_mass(void* h) { return H(h)->mass(); }
+ORC_API double orc_kinetic_energy(void* h) { return H(h)->kinetic_energy(); }
+ORC_API double orc_max_speed(void* h) { return H(h)->max_speed(); }
+ORC_API int orc_logical(void* h, double* out) { return guarded([&] { H(h)->logical(out); }); }
+ORC_API int orc_set_logical(void* h, const double* in) { return guarded([&] { H(h)->set_logical(in); }); }
+ORC_API int orc_raw(void* h, double* out) { return guarded([&] { H(h)->raw(out); }); }
+ORC_API int orc_load_map(void* h, int p, int64_t* out) { return guarded([&] { H(h)->load_map(p, out); }); }
+ORC_API int orc_store_map(void* h, int p, int64_t* out) { return guarded([&] { H(h)->store_map(p, out); }); }
+ORC_API int orc_bb_load_map(void* h, int p, int64_t* out) { return guarded([&] { H(h)->bb_load_map(p, out); }); }
+ORC_API uint64_t orc_time_step(void* h) { return H(h)->time_step(); }
+ORC_API void orc_set_time_step(void* h, uint64_t t) { H(h)->set_time_step(t); }
+ORC_API size_t orc_cells(void* h) { return H(h)->cells(); }
+ORC_API uint16_t orc_f32_to_f16(float f) { return orc::f32_to_f16_rne(f); }
+ORC_API float orc_f16_to_f32(uint16_t h) { return orc::f16_to_f32(h); }
+ORC_API double orc_uniform01(uint64_t seed, uint64_t cell, uint64_t comp) {
+    return static_cast<double>(orc::uniform01<float>(seed, cell, comp));
+}

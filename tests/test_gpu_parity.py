@@ -1,0 +1,372 @@
+"""Parity of the HIP esoteric-pull path (through the C ABI) with the CPU oracle and with the
+reference's own outputs (tests/golden).  The FP32 kernels are compiled with -ffp-contract=off
+and keep the reference's operation order, so FP32 fields are compared bit for bit; the stated
+north-star bar (rho, u within 1e-6 relative) is asserted as well.
+"""
+import numpy as np
+import pytest
+
+from conftest import report
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1.0e-6  # rho and u, relative to max |value|
+
+
+def make_pair(hip, oracle_mod, q, n, tau, accel=(0.0, 0.0, 0.0), storage="fp32"):
+    nx, ny, nz = n
+    g = hip.EsotericPullHipSolver(q, nx, ny, nz, tau, accel, storage=storage)
+    # The oracle gets the float32 values the C ABI receives.
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    o = oracle_mod.Oracle(q, nx, ny, nz, f32(tau), tuple(f32(a) for a in accel), storage=storage)
+    return g, o
+
+
+def gpu_fields(g, f64=False):
+    m = g.download_macroscopic(f64=f64)
+    return np.stack([m.rho, m.ux, m.uy, m.uz]).reshape(4, -1).astype(np.float64)
+
+
+def assert_fields(gf, of, bit_exact=True, mask=None, tol=REL_TOL):
+    of = np.asarray(of).reshape(4, -1)
+    if mask is not None:
+        gf, of = gf[:, mask], of[:, mask]
+    for k, name in enumerate(("rho", "ux", "uy", "uz")):
+        scale = max(np.abs(of[k]).max(), 1e-30)
+        err = np.abs(gf[k] - of[k]).max() / scale
+        assert err <= tol, f"{name}: max relative error {err:.3e} > {tol}"
+    if bit_exact:
+        assert np.array_equal(gf, of), "fields differ from the oracle in the last bit"
+
+
+# ---------------------------------------------------------------------- index map
+@pytest.mark.parametrize("q", [9, 19, 27])
+@pytest.mark.parametrize("parity", [0, 1])
+def test_index_map_matches_reference(hip, oracle_mod, golden, q, parity):
+    report("index_map", q=q, parity=parity)
+    nx, ny, nz = golden("index_maps")[f"q{q}_shape"]
+    g, o = make_pair(hip, oracle_mod, q, (int(nx), int(ny), int(nz)), 0.7)
+    load, store = g.index_map(parity)
+    ref = golden("index_maps")[f"q{q}_parity{parity}"]
+    assert np.array_equal(load, ref), "load slots differ from the reference's load_cell"
+    assert np.array_equal(load, o.load_map(parity))
+    assert np.array_equal(store, o.store_map(parity))
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_index_map_bounded(hip, oracle_mod, q):
+    report("index_map_bounded", q=q)
+    rng = np.random.default_rng(7)
+    n = (12, 10, 8)
+    flags = np.where(rng.random(np.prod(n)) < 0.25, 0x80, 0).astype(np.uint8)
+    g, o = make_pair(hip, oracle_mod, q, n, 0.7)
+    g.set_flags(flags)
+    o.set_flags(flags)
+    for parity in (0, 1):
+        load, store = g.index_map(parity)
+        fluid = flags == 0
+        assert np.array_equal(load[:, fluid], o.bb_load_map(parity)[:, fluid])
+        assert np.array_equal(store[:, fluid], o.store_map(parity)[:, fluid])
+        assert np.all(load[:, ~fluid] == -1)
+        # Race freedom: every slot is written at most once per step.
+        written = store[:, fluid].reshape(-1)
+        assert np.unique(written).size == written.size
+
+
+# ---------------------------------------------------------------------- periodic parity
+@pytest.mark.parametrize("q", [9, 19, 27])
+@pytest.mark.parametrize("steps", [7, 8])
+def test_taylor_green_parity_with_reference(hip, oracle_mod, golden, q, steps):
+    """tests/test_main.cpp:341-372 configuration: HIP vs reference output vs oracle."""
+    report("tg_parity", q=q, steps=steps)
+    nx, ny, nz = (int(v) for v in golden("parity_runs")[f"q{q}_shape"])
+    g, o = make_pair(hip, oracle_mod, q, (nx, ny, nz), 0.73)
+    g.initialize_taylor_green(0.017)
+    o.init_taylor_green(0.017)
+    assert np.array_equal(g.logical_populations().astype(np.float64), golden("parity_runs")[f"q{q}_init_logical"])
+    g.step(steps)
+    o.step(steps)
+    gf = gpu_fields(g)
+    ref = golden("parity_runs")[f"q{q}_tg{steps}"].reshape(4, -1).astype(np.float64)
+    assert_fields(gf, ref)
+    assert_fields(gf, o.macroscopic())
+    mass, ke, vmax = g.diagnostics()
+    rm, rke, rv = golden("parity_runs")[f"q{q}_tg{steps}_scalars"]
+    assert abs(mass - rm) <= 1e-12 * abs(rm)
+    assert abs(ke - rke) <= 1e-12 * abs(rke)
+    assert abs(vmax - rv) <= 1e-7 * rv
+
+
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_guo_forcing_parity_with_reference(hip, oracle_mod, golden, q):
+    """tests/test_main.cpp:404-434 configuration (60 forced steps from rest)."""
+    report("forced_parity", q=q)
+    d = golden("parity_runs")
+    nx, ny, nz = (int(v) for v in d[f"q{q}_shape"])
+    g, o = make_pair(hip, oracle_mod, q, (nx, ny, nz), 0.73, tuple(d[f"q{q}_forced_accel"]))
+    g.initialize_uniform()
+    o.init_uniform()
+    g.step(60)
+    o.step(60)
+    gf = gpu_fields(g)
+    assert_fields(gf, d[f"q{q}_forced60"].reshape(4, -1).astype(np.float64))
+    assert_fields(gf, o.macroscopic())
+    assert gf[1].reshape(-1)[gf.shape[1] // 2] > 0.0
+
+
+def test_config1_taylor_green_64(hip, oracle_mod):
+    """D3Q19 64^3 periodic, tau 0.6, TG amplitude 0.05 + seeded noise 1e-4, 10 + 100 steps."""
+    report("config1", n=64)
+    g, o = make_pair(hip, oracle_mod, 19, (64, 64, 64), 0.6)
+    g.initialize_taylor_green(0.05, 1.0e-4, 1)
+    o.init_taylor_green(0.05, 1.0e-4, 1)
+    m0 = g.total_mass()
+    assert abs(m0 - o.mass()) <= 1e-12 * abs(m0)
+    g.step(10)
+    g.step(100)
+    o.step(110)
+    assert_fields(gpu_fields(g), o.macroscopic())
+    m1 = g.total_mass()
+    assert abs(m1 - o.mass()) <= 1e-12 * abs(m1)
+    assert abs(m1 - m0) / m0 < 1e-5
+
+
+def test_taylor_green_32_matches_reference_fixture(hip, golden):
+    report("tg32")
+    g = hip.EsotericPullHipSolver(19, 32, 32, 32, 0.6)
+    g.initialize_taylor_green(0.05)
+    g.step(110)
+    assert_fields(gpu_fields(g), golden("tg_d3q19_32")["fields"].reshape(4, -1).astype(np.float64))
+
+
+def test_full_size_256_parity(hip, oracle_mod):
+    """D3Q19 256^3 FP32 periodic at full size, 3 steps (both parities) vs the oracle."""
+    report("full256")
+    g, o = make_pair(hip, oracle_mod, 19, (256, 256, 256), 0.6)
+    g.initialize_taylor_green(0.05, 1.0e-4, 1)
+    o.init_taylor_green(0.05, 1.0e-4, 1)
+    g.step(3)
+    o.step(3)
+    assert_fields(gpu_fields(g), o.macroscopic())
+
+
+# ---------------------------------------------------------------------- bounded parity
+def channel_flags(nx, ny, nz, rng=None, sphere_fraction=0.0):
+    f = np.zeros((nz, ny, nx), dtype=np.uint8)
+    f[:, 0, :] = f[:, -1, :] = 0x80
+    f[0, :, :] = f[-1, :, :] = 0x80
+    if rng is not None and sphere_fraction > 0:
+        inner = rng.random((nz, ny, nx)) < sphere_fraction
+        f[inner] = 0x80
+    return f.reshape(-1)
+
+
+def test_zero_flags_equal_periodic(hip):
+    report("zero_flags")
+    a = hip.EsotericPullHipSolver(19, 16, 12, 10, 0.7)
+    b = hip.EsotericPullHipSolver(19, 16, 12, 10, 0.7)
+    for s in (a, b):
+        s.initialize_taylor_green(0.02)
+    b.set_flags(np.zeros(a.cells, dtype=np.uint8))
+    a.step(5)
+    b.step(5)
+    assert np.array_equal(gpu_fields(a), gpu_fields(b))
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_walled_channel_parity(hip, oracle_mod, q):
+    report("walled_channel", q=q)
+    rng = np.random.default_rng(11)
+    n = (20, 12, 10)
+    flags = channel_flags(*n, rng, 0.05)
+    g, o = make_pair(hip, oracle_mod, q, n, 0.65, (2e-5, 0.0, 0.0))
+    for s in (g, o):
+        s.set_flags(flags)
+    g.initialize_taylor_green(0.02, 1e-3, 5)
+    o.init_taylor_green(0.02, 1e-3, 5)
+    g.step(41)
+    o.step(41)
+    gf = gpu_fields(g)
+    assert_fields(gf, o.macroscopic())
+    assert abs(g.total_mass() - o.mass()) <= 1e-12 * o.mass()
+    fluid = flags == 0
+    assert np.array_equal(g.logical_populations()[:, fluid].astype(np.float64), o.logical()[:, fluid])
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_moving_lid_cavity_parity(hip, oracle_mod, q):
+    """Config-2 geometry at 24^3: walls x=0, x=nx-1, y=0, z=0, z=nz-1, lid y=ny-1 moving at 0.1."""
+    report("cavity", q=q)
+    n = 24
+    f = np.zeros((n, n, n), dtype=np.uint8)
+    f[:, :, 0] = f[:, :, -1] = 0x80
+    f[:, 0, :] = 0x80
+    f[0, :, :] = f[-1, :, :] = 0x80
+    f[:, -1, :] = 0x81
+    flags = f.reshape(-1)
+    g, o = make_pair(hip, oracle_mod, q, (n, n, n), 0.55)
+    for s in (g, o):
+        s.set_flags(flags)
+        s.set_wall_velocity(1, (0.1, 0.0, 0.0))
+    g.initialize_uniform()
+    o.init_uniform()
+    g.step(100)
+    o.step(100)
+    gf = gpu_fields(g)
+    assert_fields(gf, o.macroscopic())
+    ux = gf[1].reshape(n, n, n)
+    assert ux[n // 2, n - 2, n // 2] > 0.01  # lid drags the fluid below it
+
+
+def test_inlet_outlet_porosity_parity(hip, oracle_mod):
+    report("inlet_outlet_pbb")
+    rng = np.random.default_rng(4)
+    nx, ny, nz = 24, 12, 10
+    f = channel_flags(nx, ny, nz).reshape(nz, ny, nx)
+    inner = f[:, :, 0] == 0
+    f[:, :, 0][inner] = 0x01
+    f[:, :, -1][f[:, :, -1] == 0] = 0x02
+    flags = f.reshape(-1)
+    gamma = np.where(flags == 0, rng.uniform(0.3, 0.7, flags.size), 0.0).astype(np.float32)
+    g, o = make_pair(hip, oracle_mod, 19, (nx, ny, nz), 0.6)
+    for s in (g, o):
+        s.set_flags(flags)
+        s.set_inlet_outlet(0.05, 1.0)
+    g.set_porosity(gamma)
+    o.set_gamma(gamma.astype(np.float64))
+    g.initialize_uniform()
+    o.init_uniform()
+    g.step(60)
+    o.step(60)
+    assert_fields(gpu_fields(g), o.macroscopic())
+
+
+def test_flags_added_mid_run(hip, oracle_mod):
+    """Walls added between steps (periodic -> f1 -> f1 plus more walls), both parities."""
+    report("flags_added")
+    rng = np.random.default_rng(3)
+    n = (16, 12, 10)
+    f1 = channel_flags(*n, rng, 0.05)
+    f2 = np.maximum(f1, channel_flags(*n, rng, 0.05))
+    g, o = make_pair(hip, oracle_mod, 19, n, 0.7, (1e-5, 0, 0))
+    g.initialize_taylor_green(0.02)
+    o.init_taylor_green(0.02)
+    g.step(3)
+    o.step(3)
+    for flags, steps in ((f1, 5), (f2, 4)):
+        g.set_flags(flags)
+        o.set_flags(flags)
+        g.step(steps)
+        o.step(steps)
+        assert_fields(gpu_fields(g), o.macroscopic(), mask=flags == 0)
+
+
+def test_walls_removed_restart_at_rest(hip, oracle_mod):
+    """Walls set before initialisation hold rest equilibrium; clearing them resumes from it."""
+    report("flags_removed")
+    rng = np.random.default_rng(5)
+    n = (16, 12, 10)
+    flags = channel_flags(*n, rng, 0.05)
+    g, o = make_pair(hip, oracle_mod, 19, n, 0.7, (1e-5, 0, 0))
+    for s in (g, o):
+        s.set_flags(flags)
+    g.initialize_uniform()
+    o.init_uniform()
+    g.step(10)
+    o.step(10)
+    g.set_flags(None)
+    o.set_flags(None)
+    g.step(5)
+    o.step(5)
+    # The frozen density is re-expanded from its float sum: equal to ~1 ulp, not bit for bit.
+    assert_fields(gpu_fields(g), o.macroscopic(), bit_exact=False, tol=1e-5)
+
+
+def test_local_acceleration_parity(hip, oracle_mod):
+    report("local_accel")
+    rng = np.random.default_rng(9)
+    n = (12, 10, 8)
+    cells = int(np.prod(n))
+    a = [rng.uniform(-1e-5, 1e-5, cells).astype(np.float32) for _ in range(3)]
+    g, o = make_pair(hip, oracle_mod, 19, n, 0.7, (1e-6, 0, 0))
+    g.set_local_body_acceleration(*a)
+    o.set_local_acceleration(*(x.astype(np.float64) for x in a))
+    g.initialize_taylor_green(0.02)
+    o.init_taylor_green(0.02)
+    g.step(9)
+    o.step(9)
+    assert_fields(gpu_fields(g), o.macroscopic())
+
+
+# ---------------------------------------------------------------------- storage precisions
+@pytest.mark.parametrize("bounded", [False, True])
+def test_fp16_storage_parity(hip, oracle_mod, bounded):
+    report("fp16", bounded=bounded)
+    n = (20, 12, 10)
+    g, o = make_pair(hip, oracle_mod, 19, n, 0.6, (1e-5, 0, 0), storage="fp16s")
+    if bounded:
+        flags = channel_flags(*n, np.random.default_rng(2), 0.05)
+        g.set_flags(flags)
+        o.set_flags(flags)
+    g.initialize_taylor_green(0.03, 1e-4, 2)
+    o.init_taylor_green(0.03, 1e-4, 2)
+    g.step(25)
+    o.step(25)
+    assert_fields(gpu_fields(g), o.macroscopic())
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_fp64_storage_parity(hip, oracle_mod, q):
+    report("fp64", q=q)
+    n = (14, 12, 10)
+    g, o = make_pair(hip, oracle_mod, q, n, 0.7, (1e-6, 0, 0), storage="fp64")
+    g.initialize_taylor_green(0.03, 1e-4, 3)
+    o.init_taylor_green(0.03, 1e-4, 3)
+    g.step(11)
+    o.step(11)
+    assert_fields(gpu_fields(g, f64=True), o.macroscopic())
+
+
+# ---------------------------------------------------------------------- API contracts
+def test_residency_ledger(hip):
+    """tests/test_v0155_gpu_residency.cpp:89-113: stepping moves no host bytes; a macroscopic
+    download moves exactly 4 * cells * 4 bytes."""
+    report("ledger")
+    g = hip.EsotericPullHipSolver(19, 32, 16, 8, 0.6)
+    g.initialize_taylor_green(0.02)
+    g.transfer_stats(reset=True)
+    g.step(10)
+    g.wait()
+    st = g.transfer_stats(reset=True)
+    assert st["host_to_device_bytes"] == 0 and st["device_to_host_bytes"] == 0
+    g.download_macroscopic()
+    st = g.transfer_stats()
+    assert st["device_to_host_bytes"] == 4 * g.cells * 4
+    assert st["host_to_device_bytes"] == 0
+
+
+def test_population_roundtrip_and_accounting(hip):
+    report("roundtrip")
+    g = hip.EsotericPullHipSolver(27, 10, 8, 6, 0.6)
+    rng = np.random.default_rng(1)
+    pops = rng.uniform(0.01, 0.1, (27, g.cells)).astype(np.float32)
+    g.step(1)  # odd parity
+    g.upload_populations(pops)
+    assert np.array_equal(g.logical_populations(), pops)
+    assert g.population_bytes() == 27 * g.cells * 4
+    assert g.two_lattice_population_bytes() == 2 * g.population_bytes()
+    assert g.time_step == 1
+    assert "gfx950" in g.device_name()
+
+
+def test_invalid_arguments_raise(hip):
+    report("errors")
+    with pytest.raises(RuntimeError, match="tau"):
+        hip.EsotericPullHipSolver(19, 8, 8, 8, 0.5)
+    with pytest.raises(RuntimeError, match="nz"):
+        hip.EsotericPullHipSolver(19, 8, 8, 1, 0.6)
+    g = hip.EsotericPullHipSolver(19, 8, 8, 8, 0.6)
+    with pytest.raises(RuntimeError, match="flags"):
+        g.set_flags(np.full(g.cells, 0x05, dtype=np.uint8))
+    with pytest.raises(RuntimeError, match="gamma"):
+        g.set_porosity(np.full(g.cells, 1.5, dtype=np.float32))

@@ -1,0 +1,392 @@
+"""Pins the CPU oracle (oracle/lbm_oracle.cpp) before it is trusted as the checker.
+
+* Against the reference's own outputs (tests/golden, written by tests/golden/make_golden.py from
+  the reference sources): bit-exact for the periodic esoteric path.
+* Against the live reference build (oracle/_ref) when it is present.
+* Against closed forms and invariants for what the reference does not implement or gets wrong:
+  the bounded path (the reference's has a defect at esoteric_pull.hpp:483), moving walls,
+  partial bounce-back, inlet/outlet, FP16 shifted storage, FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+F32 = lambda v: float(np.float32(v))  # noqa: E731
+
+
+def fields(o):
+    return np.stack(o.macroscopic()).reshape(4, -1)
+
+
+# ---------------------------------------------------------------- reference fixtures (bit-exact)
+@pytest.mark.parametrize("q", [9, 19, 27])
+@pytest.mark.parametrize("parity", [0, 1])
+def test_load_map_matches_reference(golden, q, parity):
+    d = golden("index_maps")
+    nx, ny, nz = (int(v) for v in d[f"q{q}_shape"])
+    o = O.Oracle(q, nx, ny, nz, 0.7)
+    assert np.array_equal(o.load_map(parity), d[f"q{q}_parity{parity}"])
+
+
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_store_map_is_streaming(q):
+    """fout[d] of cell n is stored where cell n + c_d loads fin[d] at the next parity; every slot
+    is written exactly once (the race-freedom invariant of the in-place scheme)."""
+    nx, ny, nz = (5, 4, 1) if q == 9 else (5, 4, 3)
+    o = O.Oracle(q, nx, ny, nz, 0.7)
+    lat = {9: (O.lib(),)}
+    cx, cy, cz = lattice_vectors(q)
+    n = np.arange(nx * ny * nz)
+    x, y, z = n % nx, (n // nx) % ny, n // (nx * ny)
+    for p in (0, 1):
+        store, nxt = o.store_map(p), o.load_map(1 - p)
+        for d in range(q):
+            tgt = ((z + cz[d]) % nz * ny + (y + cy[d]) % ny) * nx + (x + cx[d]) % nx
+            assert np.array_equal(store[d], nxt[d, tgt])
+        assert np.unique(store).size == store.size
+        assert np.unique(o.load_map(p)).size == store.size
+
+
+def lattice_vectors(q):
+    if q == 9:
+        return ([0, 1, -1, 0, 0, 1, -1, -1, 1], [0, 0, 0, 1, -1, 1, -1, 1, -1], [0] * 9)
+    cx = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0]
+    cy = [0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1]
+    cz = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1]
+    if q == 27:
+        cx += [1, -1, 1, -1, 1, -1, 1, -1]
+        cy += [1, -1, 1, -1, -1, 1, -1, 1]
+        cz += [1, -1, -1, 1, 1, -1, -1, 1]
+    return cx, cy, cz
+
+
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_periodic_runs_match_reference_bitwise(golden, q):
+    """tests/test_main.cpp:341-372 and :404-434 configurations."""
+    d = golden("parity_runs")
+    nx, ny, nz = (int(v) for v in d[f"q{q}_shape"])
+    o = O.Oracle(q, nx, ny, nz, F32(0.73))
+    o.init_taylor_green(F32(0.017))
+    assert np.array_equal(o.logical(), d[f"q{q}_init_logical"].astype(np.float64))
+    for steps in (7, 8):
+        o.init_taylor_green(F32(0.017))
+        o.step(steps)
+        assert np.array_equal(fields(o), d[f"q{q}_tg{steps}"].reshape(4, -1).astype(np.float64))
+        m, ke, vmax = d[f"q{q}_tg{steps}_scalars"]
+        assert o.mass() == pytest.approx(m, rel=1e-13)
+        assert o.kinetic_energy() == pytest.approx(ke, rel=1e-13)
+        assert o.max_speed() == pytest.approx(vmax, rel=1e-7)
+        # The reference's own in-place/two-grid gate (5e-6).
+        assert np.abs(fields(o) - d[f"q{q}_tg{steps}_twogrid"].reshape(4, -1)).max() < 5e-6
+    accel = tuple(F32(a) for a in d[f"q{q}_forced_accel"])
+    o = O.Oracle(q, nx, ny, nz, F32(0.73), accel)
+    o.init_uniform()
+    o.step(60)
+    assert np.array_equal(fields(o), d[f"q{q}_forced60"].reshape(4, -1).astype(np.float64))
+
+
+def test_taylor_green_32_matches_reference(golden):
+    d = golden("tg_d3q19_32")
+    o = O.Oracle(19, 32, 32, 32, F32(0.6))
+    o.init_taylor_green(F32(0.05))
+    assert o.mass() == pytest.approx(d["scalars"][0], rel=1e-13)
+    o.step(110)
+    assert np.array_equal(fields(o), d["fields"].reshape(4, -1).astype(np.float64))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("q", [19, 27])
+def test_noisy_taylor_green_matches_live_reference(q):
+    """Seeded-noise initial state planted into the reference solver, stepped by both."""
+    n = (12, 10, 8)
+    o = O.Oracle(q, *n, F32(0.6), (F32(1e-6), 0.0, F32(-2e-6)))
+    o.init_taylor_green(F32(0.05), F32(1e-4), 1)
+    r = O.RefEsoteric(q, *n, 0.6, (1e-6, 0.0, -2e-6))
+    r.set_raw(o.raw().astype(np.float32))
+    assert np.array_equal(r.logical().astype(np.float64), o.logical())
+    o.step(13)
+    r.step(13)
+    assert np.array_equal(fields(o), np.stack(r.macroscopic()).reshape(4, -1).astype(np.float64))
+    assert np.array_equal(o.raw(), r.raw().astype(np.float64))
+
+
+# ---------------------------------------------------------------- bounded path
+def test_reference_bounded_defect_is_recorded(golden):
+    """The reference's all-fluid-mask step differs from its periodic step by ~19% of |u|."""
+    d = golden("bounded_defect")
+    assert float(d["max_du"]) > 0.1 * float(d["max_u"])
+
+
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_zero_mask_equals_periodic_bitwise(q):
+    n = (16, 12, 1 if q == 9 else 10)
+    a = O.Oracle(q, *n, F32(0.7), (F32(1e-6), 0.0, 0.0))
+    b = O.Oracle(q, *n, F32(0.7), (F32(1e-6), 0.0, 0.0))
+    for s in (a, b):
+        s.init_taylor_green(F32(0.02))
+    b.set_flags(np.zeros(a.cells, dtype=np.uint8))
+    a.step(5)
+    b.step(5)
+    assert np.array_equal(a.raw(), b.raw())
+
+
+def test_d2q9_poiseuille_matches_reference_d2q9solver(golden):
+    """tests/test_main.cpp:449-477 geometry.  D2Q9Solver reports u from post-collision
+    populations (src/lbm/d2q9.cpp:280-307), i.e. u + a; the oracle reports the pre-collision u."""
+    nx, ny, a = 24, 14, F32(1e-5)
+    o = O.Oracle(9, nx, ny, 1, F32(0.8), (a, 0.0, 0.0))
+    f = np.zeros((ny, nx), np.uint8)
+    f[0, :] = f[-1, :] = O.WALL
+    o.set_flags(f)
+    o.init_uniform()
+    o.step(5000)
+    rho, ux, _, _ = o.macroscopic()
+    ref = golden("d2q9_walls")["poiseuille"]
+    fl = f == 0
+    assert np.abs(ux[0][fl] + a - ref[1][fl]).max() / np.abs(ref[1][fl]).max() < 2e-4
+    # Analytic parabola with halfway walls (the reference's own gate is 1 %).
+    nu = (F32(0.8) - 0.5) / 3.0
+    yw = np.arange(1, ny - 1) - 0.5
+    exact = a * yw * ((ny - 2) - yw) / (2 * nu)
+    num = ux[0][1:-1].mean(axis=1)
+    assert np.linalg.norm(num - exact) / np.linalg.norm(exact) < 0.01
+
+
+def test_d2q9_moving_lid_matches_reference_d2q9solver(golden):
+    """tests/test_main.cpp:479-499 geometry (side walls override the lid corners)."""
+    n = 48
+    o = O.Oracle(9, n, n, 1, F32(0.8))
+    f = np.zeros((n, n), np.uint8)
+    f[-1, :] = O.WALL | 1
+    f[0, :] = f[:, 0] = f[:, -1] = O.WALL
+    o.set_flags(f)
+    o.set_wall_velocity(1, (F32(0.05), 0.0, 0.0))
+    o.init_uniform()
+    o.step(6000)
+    _, ux, uy, _ = o.macroscopic()
+    ref = golden("d2q9_walls")["cavity"]
+    fl = f == 0
+    assert np.abs(ux[0][fl] - ref[1][fl]).max() / np.abs(ref[1][fl]).max() < 2e-5
+    assert np.abs(uy[0][fl] - ref[2][fl]).max() / np.abs(ref[2][fl]).max() < 4e-5
+    assert ux[0][n - 2, n // 2] > 0.03 and ux[0][n // 2, n // 2] < -0.003  # the reference's gates
+
+
+def test_d3q19_poiseuille_analytic():
+    nx, ny, nz, a, tau = 4, 14, 4, F32(1e-5), F32(0.8)
+    o = O.Oracle(19, nx, ny, nz, tau, (a, 0.0, 0.0))
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, 0, :] = f[:, -1, :] = O.WALL
+    o.set_flags(f)
+    o.init_uniform()
+    o.step(4000)
+    ux = o.macroscopic()[1]
+    nu = (tau - 0.5) / 3.0
+    yw = np.arange(1, ny - 1) - 0.5
+    exact = a * yw * ((ny - 2) - yw) / (2 * nu)
+    num = ux[:, 1:-1, :].mean(axis=(0, 2))
+    assert np.linalg.norm(num - exact) / np.linalg.norm(exact) < 0.01
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_couette_linear_profile(q):
+    """Plane Couette flow: wall at y=0 at rest, wall at y=ny-1 moving at U; steady u_x is linear
+    between the halfway walls, u(y) = U (y - 1/2) / (ny - 2), which BGK reproduces to round-off."""
+    nx, ny, nz, U = 4, 10, 4, F32(0.02)
+    o = O.Oracle(q, nx, ny, nz, F32(0.9))
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, 0, :] = O.WALL
+    f[:, -1, :] = O.WALL | 1
+    o.set_flags(f)
+    o.set_wall_velocity(1, (U, 0.0, 0.0))
+    o.init_uniform()
+    o.step(6000)
+    ux = o.macroscopic()[1][:, 1:-1, :].mean(axis=(0, 2))
+    exact = U * (np.arange(1, ny - 1) - 0.5) / (ny - 2)
+    assert np.abs(ux - exact).max() / U < 2e-4
+
+
+# ---------------------------------------------------------------- closed forms on the periodic path
+def test_uniform_flow_invariance():
+    """tests/test_main.cpp:374-388."""
+    o = O.Oracle(19, 24, 16, 12, F32(0.72))
+    o.init_uniform(1.0, F32(0.015), F32(-0.01), F32(0.005))
+    m0 = o.mass()
+    o.step(20)
+    r, ux, uy, uz = o.macroscopic()
+    assert abs(o.mass() - m0) < 1e-2
+    assert np.abs(r - 1).max() < 2e-5 and np.abs(ux - F32(0.015)).max() < 2e-5
+    assert np.abs(uy + F32(0.01)).max() < 2e-5 and np.abs(uz - F32(0.005)).max() < 2e-5
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_tau_one_is_equilibrium_then_streaming(q):
+    """tau = 1: BGK returns the second-order equilibrium of the cell's moments, and streaming is the
+    permutation f(n + c_d, d) <- f(n, d).  Checked from a random state against a direct numpy
+    evaluation of that textbook step."""
+    n = (6, 5, 4)
+    nx, ny, nz = n
+    o = O.Oracle(q, *n, 1.0, storage="fp64")
+    rng = np.random.default_rng(1)
+    cx, cy, cz = (np.array(v, dtype=np.float64) for v in lattice_vectors(q))
+    w = {19: [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12,
+         27: [8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8}[q]
+    w = np.array(w, dtype=np.float32).astype(np.float64)  # the reference's float weights
+    fin = w[:, None] * (1 + 0.05 * rng.standard_normal((q, o.cells)))
+    o.set_logical(fin)
+    o.step(1)
+    rho = fin.sum(0)
+    u = [(fin * c[:, None]).sum(0) / rho for c in (cx, cy, cz)]
+    cu = 3 * (cx[:, None] * u[0] + cy[:, None] * u[1] + cz[:, None] * u[2])
+    feq = w[:, None] * rho * (1 + cu + 0.5 * cu * cu - 1.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2))
+    cell = np.arange(o.cells)
+    x, y, z = cell % nx, (cell // nx) % ny, cell // (nx * ny)
+    expected = np.empty_like(feq)
+    for d in range(q):
+        tgt = ((z + int(cz[d])) % nz * ny + (y + int(cy[d])) % ny) * nx + (x + int(cx[d])) % nx
+        expected[d, tgt] = feq[d]
+    assert np.abs(o.logical() - expected).max() < 1e-15
+
+
+def test_taylor_green_fp64_convergence():
+    """tests/test_main.cpp:526-565 on the FP64 oracle (D2Q9, A = 0.001, tau 0.8, 20 steps)."""
+    errs = []
+    for n in (16, 32, 64):
+        o = O.Oracle(9, n, n, 1, 0.8, storage="fp64")
+        o.init_taylor_green(0.001)
+        o.step(20)
+        _, ux, uy, _ = o.macroscopic()
+        nu = (0.8 - 0.5) / 3
+        k = 2 * np.pi / n
+        decay = np.exp(-2 * nu * k * k * 20)
+        xf = (np.arange(n) + 0.5) / n
+        X, Y = np.meshgrid(xf, xf)
+        ex = 0.001 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) * decay
+        ey = -0.001 * np.cos(2 * np.pi * X) * np.sin(2 * np.pi * Y) * decay
+        errs.append(np.sqrt((((ux[0] - ex) ** 2 + (uy[0] - ey) ** 2).sum()) / ((ex ** 2 + ey ** 2).sum())))
+    assert errs[1] < 0.35 * errs[0] and errs[2] < 0.35 * errs[1] and errs[2] < 2e-3
+
+
+def test_fp64_matches_reference_precision_solver(golden):
+    """PrecisionPullSolver<D, double> is a two-grid scheme: same physics, fields within 1e-9."""
+    d = golden("precision_fp64")
+    o = O.Oracle(19, 16, 16, 16, 0.7, storage="fp64")
+    o.init_taylor_green(0.02)
+    o.step(20)
+    assert np.abs(fields(o) - d["d3q19_16"].reshape(4, -1)).max() < 1e-9
+    o = O.Oracle(9, 32, 32, 1, 0.8, storage="fp64")
+    o.init_taylor_green(0.001)
+    o.step(20)
+    assert np.abs(fields(o) - d["d2q9_32"].reshape(4, -1)).max() < 1e-12
+
+
+def test_fp32_vs_fp64():
+    """tests/test_main.cpp:567-584 gate (1e-6) on D3Q19."""
+    a = O.Oracle(19, 20, 16, 12, F32(0.73))
+    b = O.Oracle(19, 20, 16, 12, F32(0.73), storage="fp64")
+    a.init_taylor_green(F32(0.02))
+    b.init_taylor_green(F32(0.02))
+    a.step(120)
+    b.step(120)
+    assert np.abs(fields(a) - fields(b)).max() < 1e-6
+
+
+# ---------------------------------------------------------------- FP16 shifted storage
+def test_f16_codec_matches_ieee_rne():
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([
+        rng.standard_normal(20000).astype(np.float32) * 10.0 ** rng.integers(-9, 5, 20000),
+        np.array([0.0, -0.0, 65504, 65519.99, 65520, 1e-8, 2 ** -25, 2 ** -24, 3 * 2 ** -25, 6.1e-5, np.inf,
+                  -np.inf], dtype=np.float32),
+    ]).astype(np.float32)
+    ours = O.f32_to_f16_bits(vals)
+    assert np.array_equal(ours, vals.astype(np.float16).view(np.uint16))
+    bits = np.arange(0, 65536, dtype=np.uint32).astype(np.uint16)
+    finite = (bits & 0x7C00) != 0x7C00
+    dec = O.f16_bits_to_f32(bits[finite])
+    assert np.array_equal(dec, bits[finite].view(np.float16).astype(np.float32))
+
+
+def test_fp16_storage_roundtrip_and_error():
+    """Equilibrium at rest stores exactly zero deviations; FP16 storage tracks FP32 closely."""
+    o = O.Oracle(19, 8, 8, 8, 0.6, storage="fp16s")
+    o.init_uniform()
+    assert np.array_equal(o.logical(), O.Oracle(19, 8, 8, 8, 0.6).logical())
+    a = O.Oracle(19, 24, 20, 16, F32(0.6))
+    b = O.Oracle(19, 24, 20, 16, F32(0.6), storage="fp16s")
+    for s in (a, b):
+        s.init_taylor_green(F32(0.05))
+    m0 = b.mass()
+    a.step(200)
+    b.step(200)
+    fa, fb = fields(a), fields(b)
+    u_err = np.linalg.norm(fa[1:] - fb[1:]) / np.linalg.norm(fa[1:])
+    assert u_err < 1e-3
+    assert abs(b.mass() - m0) / m0 < 2e-4
+
+
+# ---------------------------------------------------------------- partial bounce-back
+def test_pbb_zero_gamma_is_bgk_bitwise():
+    a = O.Oracle(19, 10, 8, 6, F32(0.7), (F32(1e-5), 0, 0))
+    b = O.Oracle(19, 10, 8, 6, F32(0.7), (F32(1e-5), 0, 0))
+    b.set_gamma(np.zeros(a.cells))
+    for s in (a, b):
+        s.init_taylor_green(F32(0.02))
+        s.step(6)
+    assert np.array_equal(a.raw(), b.raw())
+
+
+def test_pbb_unit_gamma_reflects_on_site():
+    q, n = 19, (6, 5, 4)
+    o = O.Oracle(q, *n, 0.7, storage="fp64")
+    o.set_gamma(np.ones(int(np.prod(n))))
+    o.init_taylor_green(0.02)
+    fin = o.logical()
+    o.step(1)
+    fout_expected = fin[[0 if d == 0 else (d + 1 if d % 2 else d - 1) for d in range(q)]]
+    cx, cy, cz = lattice_vectors(q)
+    nx, ny, nz = n
+    cell = np.arange(nx * ny * nz)
+    x, y, z = cell % nx, (cell // nx) % ny, cell // (nx * ny)
+    got = o.logical()
+    for d in range(q):
+        src = ((z - cz[d]) % nz * ny + (y - cy[d]) % ny) * nx + (x - cx[d]) % nx
+        assert np.array_equal(got[d], fout_expected[d][src])
+
+
+@pytest.mark.parametrize("gamma", [0.3, 0.6])
+def test_pbb_darcy_steady_state(gamma):
+    """Uniform gamma, body force a, periodic: the momentum balance of the blended collision gives
+    the steady reported velocity u = a / (2 gamma) (rho = 1)."""
+    a = 1e-5
+    o = O.Oracle(19, 6, 5, 4, 0.8, (a, 0.0, 0.0), storage="fp64")
+    o.set_gamma(np.full(6 * 5 * 4, gamma))
+    o.init_uniform()
+    o.step(200)
+    ux = o.macroscopic()[1]
+    assert np.allclose(ux, a / (2 * gamma), rtol=1e-9)
+
+
+# ---------------------------------------------------------------- inlet / outlet
+def test_inlet_outlet_channel():
+    """Velocity inlet (parabolic, peak U) and pressure outlet (rho_out) in a walled duct: the inlet
+    plane reports the prescribed profile, the outlet its density, and the steady mass flux is the
+    same through every cross-section."""
+    nx, ny, nz, U = 40, 10, 10, 0.03
+    o = O.Oracle(19, nx, ny, nz, 0.8, storage="fp64")
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, 0, :] = f[:, -1, :] = f[0, :, :] = f[-1, :, :] = O.WALL
+    f[1:-1, 1:-1, 0] = O.INLET
+    f[1:-1, 1:-1, -1] = O.OUTLET
+    o.set_flags(f)
+    o.set_inlet_outlet(U, 1.0)
+    o.init_uniform()
+    o.step(4000)
+    rho, ux, _, _ = o.macroscopic()
+    eta = (np.arange(1, ny - 1) - 0.5) / (ny - 2)
+    prof = U * np.outer(4 * eta * (1 - eta), 4 * eta * (1 - eta))
+    assert np.allclose(ux[1:-1, 1:-1, 0], prof, rtol=1e-12)
+    assert np.all(rho[1:-1, 1:-1, -1] == 1.0)
+    flux = (rho * ux)[1:-1, 1:-1, :].sum(axis=(0, 1))
+    assert np.abs(flux[1:-1] - flux[nx // 2]).max() / flux[nx // 2] < 5e-3
+    assert flux[0] == pytest.approx(flux[nx // 2], rel=2e-2)
