@@ -60,7 +60,7 @@ def build(verbose: bool = False) -> Path:
     os.replace(probe_tmp, PROBE)
     for name, src in APPS.items():
         if src.exists():
-            _run(["g++", "-O2", "-std=c++17", "-I", ROOT / "include", src, "-o", PKG / name,
+            _run(["g++", "-O2", "-std=c++20", "-I", ROOT / "include", src, "-o", PKG / name,
                   f"-L{PKG}", "-lcfd_lbm_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"], verbose)
     return LIB
 

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
@@ -111,7 +112,16 @@ struct cfd_lbm_solver {
         if (tiles >= (uint64_t{1} << 31)) throw std::invalid_argument("lattice too large for one launch");
         geo.tiles = static_cast<uint32_t>(tiles);
         geo.xcd_remap = (geo.tiles % 8U == 0U && geo.tiles >= 64U) ? 1U : 0U;
-        geo.stride = (static_cast<uint64_t>(cells) + 255U) / 256U * 256U;  // 1 KiB-aligned fp32 planes
+        if (const char* v = std::getenv("CFD_LBM_XCD_REMAP")) geo.xcd_remap = geo.xcd_remap && std::atoi(v) != 0;
+        // Planes start on 256-element boundaries; CFD_LBM_PLANE_PAD (elements, rounded up to 256)
+        // staggers the planes so that they do not sit a power of two apart.
+        uint64_t pad = 0;
+        if (const char* v = std::getenv("CFD_LBM_PLANE_PAD")) pad = std::strtoull(v, nullptr, 10);
+        geo.stride = (static_cast<uint64_t>(cells) + pad + 255U) / 256U * 256U;
+        // Kernels address a population as plane base + 32-bit byte offset, reaching one plane further
+        // for bounce-back: two planes must fit in 4 GiB.
+        if (2U * geo.stride * elem > (uint64_t{1} << 32))
+            throw std::invalid_argument("lattice too large: two population planes must fit in 4 GiB");
 
         ops = cfd_hip::kernel_ops(q, storage);
         int count = 0;
@@ -441,15 +451,34 @@ struct cfd_lbm_solver {
 
     void index_map(int parity, int64_t* load, int64_t* store) {
         use_device();
-        DeviceBuffer<unsigned char> buf;
         const size_t n = static_cast<size_t>(q) * cells;
-        buf.allocate(2 * n * sizeof(int64_t));
+        DeviceBuffer<unsigned char> ids, loads;
+        ids.allocate(static_cast<size_t>(q) * geo.stride * sizeof(uint32_t));
+        loads.allocate(n * sizeof(int64_t));
         const LaunchState st = state();
-        ops->index_map(st, (parity & 1) != 0, bounded(), flags.ptr, reinterpret_cast<int64_t*>(buf.ptr),
-                       reinterpret_cast<int64_t*>(buf.ptr) + n);
-        d2h(load, buf.ptr, n * sizeof(int64_t));
-        d2h(store, buf.ptr + n * sizeof(int64_t), n * sizeof(int64_t));
+        // Wall-velocity indices do not move slots; the probe sees plain walls.
+        DeviceBuffer<unsigned char> probe_flags;
+        if (bounded()) {
+            std::vector<uint8_t> host(cells);
+            hip_check(hipMemcpy(host.data(), flags.ptr, cells, hipMemcpyDeviceToHost), "hipMemcpy flags");
+            for (auto& v : host)
+                if (v & 0x80U) v = 0x80U;
+            probe_flags.allocate(cells);
+            hip_check(hipMemcpy(probe_flags.ptr, host.data(), cells, hipMemcpyHostToDevice), "hipMemcpy flags");
+        }
+        ops->index_map(st, (parity & 1) != 0, bounded(), probe_flags.ptr, reinterpret_cast<uint32_t*>(ids.ptr),
+                       reinterpret_cast<int64_t*>(loads.ptr));
+        std::vector<uint32_t> slots(static_cast<size_t>(q) * geo.stride);
+        d2h(load, loads.ptr, n * sizeof(int64_t));
+        d2h(slots.data(), ids.ptr, slots.size() * sizeof(uint32_t));
         sync();
+        // Convert identifiers to slot numbers d*cells + n and invert the store map.
+        std::fill(store, store + n, int64_t{-1});
+        for (size_t d = 0; d < static_cast<size_t>(q); ++d)
+            for (size_t c = 0; c < cells; ++c) {
+                const uint32_t id = slots[d * geo.stride + c];
+                if (id != 0xFFFFFFFFU) store[id] = static_cast<int64_t>(d * cells + c);
+            }
     }
 
     size_t population_bytes() const { return static_cast<size_t>(q) * cells * elem; }

@@ -98,6 +98,12 @@ template <> struct Storage<_Float16> {
         return static_cast<_Float16>((v - Lattice<Q>::w[D]) * kHalfScale);
     }
 };
+// Slot identifiers, used only by the index-map probe.
+template <> struct Storage<uint32_t> {
+    using Real = uint32_t;
+    template <int Q, int D> __device__ __forceinline__ static uint32_t decode(uint32_t v) { return v; }
+    template <int Q, int D> __device__ __forceinline__ static uint32_t encode(uint32_t v) { return v; }
+};
 template <> struct Storage<double> {
     using Real = double;
     template <int Q, int D> __device__ __forceinline__ static double decode(double v) { return v; }
@@ -159,7 +165,7 @@ __device__ __forceinline__ Neighbourhood neighbourhood(const Geometry& g, uint32
 // of a pair at a wall cell are touched only by the one fluid cell facing it, so the update stays
 // race-free with a single pass over the lattice.
 template <int Q> struct PairSlots {
-    uint32_t up[Q];        // index by i (odd)
+    uint32_t up[Q];        // byte offset of cell up = n - c_i in a plane, index by i (odd)
     uint8_t wall_up[Q];    // upstream wall flag of direction i (cell up)
     uint8_t wall_dn[Q];    // upstream wall flag of direction i+1 (cell dn)
     uint8_t flag_up[Q];    // full flags of up / dn (wall velocity index)

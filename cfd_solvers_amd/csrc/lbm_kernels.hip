@@ -24,6 +24,26 @@ template <class T> __device__ __forceinline__ const T* plane(const void* base, c
     return static_cast<const T*>(base) + static_cast<uint64_t>(d) * g.stride;
 }
 
+// Byte-offset access: a uniform 64-bit plane base (SGPRs) plus a 32-bit per-lane byte offset
+// (one VGPR), which maps onto the global_load/store "saddr" form.  Two consecutive planes must
+// fit in 4 GiB (checked at solver creation).
+template <class T> __device__ __forceinline__ T load_at(const void* base, uint32_t byte_off) {
+    const T* p = reinterpret_cast<const T*>(static_cast<const unsigned char*>(base) + byte_off);
+#if defined(CFD_LBM_NT_LOAD)
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t byte_off, T v) {
+    T* p = reinterpret_cast<T*>(static_cast<unsigned char*>(base) + byte_off);
+#if defined(CFD_LBM_NT_STORE)
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // Loads the logical populations of a non-wall cell (bounce-back aware) and records the slots.
 template <int Q, class T, bool Odd, bool Bounded>
 __device__ __forceinline__ void load_cell(const void* f, const Geometry& g, const uint8_t* flags,
@@ -31,27 +51,33 @@ __device__ __forceinline__ void load_cell(const void* f, const Geometry& g, cons
                                           typename Storage<T>::Real (&fin)[Q], PairSlots<Q>& s) {
     using L = Lattice<Q>;
     using S = Storage<T>;
-    fin[0] = S::template decode<Q, 0>(plane<T>(f, g, 0)[n]);
+    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
+    const uint32_t plane_bytes = static_cast<uint32_t>(g.stride * sizeof(T));
+    fin[0] = S::template decode<Q, 0>(load_at<T>(plane<T>(f, g, 0), off_n));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
-        s.up[i] = up;
-        bool wup = false, wdn = false;
+        const uint32_t off_up = up * static_cast<uint32_t>(sizeof(T));
+        s.up[i] = off_up;
         if constexpr (Bounded) {
             const uint32_t dn = h.cell(L::cx[i], L::cy[i], L::cz[i]);
             s.flag_up[i] = flags[up];
             s.flag_dn[i] = flags[dn];
-            wup = is_wall(s.flag_up[i]);
-            wdn = is_wall(s.flag_dn[i]);
+            const bool wup = is_wall(s.flag_up[i]);
+            const bool wdn = is_wall(s.flag_dn[i]);
             s.wall_up[i] = wup;
             s.wall_dn[i] = wdn;
+            // Plane i holds the pair; a wall link selects the partner plane (one plane further).
+            const uint32_t sel_a = (Odd != wup) ? plane_bytes : 0U;
+            const uint32_t sel_b = (Odd != wdn) ? 0U : plane_bytes;
+            fin[i] = S::template decode<Q, i>(load_at<T>(plane<T>(f, g, i), off_up + sel_a));
+            fin[i + 1] = S::template decode<Q, i + 1>(load_at<T>(plane<T>(f, g, i), off_n + sel_b));
+        } else {
+            // The slot of either plane of a pair holds a population of the direction it is
+            // loaded as (opposite directions share one weight, hence one FP16 shift).
+            fin[i] = S::template decode<Q, i>(load_at<T>(plane<T>(f, g, Odd ? i + 1 : i), off_up));
+            fin[i + 1] = S::template decode<Q, i + 1>(load_at<T>(plane<T>(f, g, Odd ? i : i + 1), off_n));
         }
-        const T* pa = (Odd != wup) ? plane<T>(f, g, i + 1) : plane<T>(f, g, i);
-        const T* pb = (Odd != wdn) ? plane<T>(f, g, i) : plane<T>(f, g, i + 1);
-        // Decoding uses the logical direction's shift: the slot of either plane of the pair
-        // holds a population of the direction it is loaded as.
-        fin[i] = S::template decode<Q, i>(pa[up]);
-        fin[i + 1] = S::template decode<Q, i + 1>(pb[n]);
     });
 }
 
@@ -62,7 +88,8 @@ __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Phy
     using L = Lattice<Q>;
     using S = Storage<T>;
     using Real = typename Storage<T>::Real;
-    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(fout[0]);
+    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
+    store_at<T>(plane<T>(f, g, 0), off_n, S::template encode<Q, 0>(fout[0]));
     const Real rho_wall = rho > Real(1.0e-8) ? rho : Real(1.0e-8);
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
@@ -87,11 +114,11 @@ __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Phy
             }
         }
         if constexpr (Odd) {
-            plane<T>(f, g, i)[n] = S::template encode<Q, i>(a);
-            plane<T>(f, g, i + 1)[s.up[i]] = S::template encode<Q, i + 1>(b);
+            store_at<T>(plane<T>(f, g, i), off_n, S::template encode<Q, i>(a));
+            store_at<T>(plane<T>(f, g, i + 1), s.up[i], S::template encode<Q, i + 1>(b));
         } else {
-            plane<T>(f, g, i + 1)[n] = S::template encode<Q, i>(a);
-            plane<T>(f, g, i)[s.up[i]] = S::template encode<Q, i + 1>(b);
+            store_at<T>(plane<T>(f, g, i + 1), off_n, S::template encode<Q, i>(a));
+            store_at<T>(plane<T>(f, g, i), s.up[i], S::template encode<Q, i + 1>(b));
         }
     });
 }
@@ -434,37 +461,49 @@ __global__ __launch_bounds__(256) void diagnostics_final(const double* __restric
 }
 
 // ------------------------------------------------------------------ index-map probe
-template <int Q, class T, bool Odd, bool Bounded>
-__global__ __launch_bounds__(256) void index_map(const Geometry g, const uint8_t* __restrict__ flags, uint32_t cells,
-                                                 int64_t* __restrict__ load, int64_t* __restrict__ store) {
-    using L = Lattice<Q>;
+// The slots are read off the hot kernel's own load_cell / store_cell: every slot of a plane array
+// holds its identifier d*cells + n, load_cell returns the identifiers it read, and store_cell
+// writes the identifier of each (direction, cell) it stores into the slot it stores to.
+__global__ __launch_bounds__(256) void fill_ids(uint32_t* __restrict__ ids, const Geometry g, uint32_t cells,
+                                                int q, uint32_t empty) {
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const uint32_t n = (z * g.ny + y) * g.nx + x;
+    for (int d = 0; d < q; ++d) ids[static_cast<uint64_t>(d) * g.stride + n] = empty ? empty : d * cells + n;
+}
+
+template <int Q, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void probe_load(const uint32_t* __restrict__ ids, const Geometry g,
+                                                  const uint8_t* __restrict__ flags, uint32_t cells,
+                                                  int64_t* __restrict__ load) {
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
     const uint32_t n = h.cell(0, 0, 0);
-    auto put = [&](int64_t* out, int d, int plane_dir, uint32_t cell) {
-        out[static_cast<uint64_t>(d) * cells + n] = static_cast<int64_t>(plane_dir) * cells + cell;
-    };
     if (Bounded && is_wall(flags[n])) {
-        for (int d = 0; d < Q; ++d) {
-            load[static_cast<uint64_t>(d) * cells + n] = -1;
-            store[static_cast<uint64_t>(d) * cells + n] = -1;
-        }
+        for (int d = 0; d < Q; ++d) load[static_cast<uint64_t>(d) * cells + n] = -1;
         return;
     }
-    put(load, 0, 0, n);
-    put(store, 0, 0, n);
-    static_for<(Q - 1) / 2>([&](auto P) {
-        constexpr int i = 2 * decltype(P)::value + 1;
-        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
-        const uint32_t dn = h.cell(L::cx[i], L::cy[i], L::cz[i]);
-        const bool wup = Bounded && is_wall(flags[up]);
-        const bool wdn = Bounded && is_wall(flags[dn]);
-        put(load, i, dir_a<Odd>(i, wup), up);
-        put(load, i + 1, dir_b<Odd>(i, wdn), n);
-        put(store, i, Odd ? i : i + 1, n);
-        put(store, i + 1, Odd ? i + 1 : i, up);
-    });
+    uint32_t fin[Q];
+    PairSlots<Q> s;
+    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, flags, h, n, fin, s);
+    for (int d = 0; d < Q; ++d) load[static_cast<uint64_t>(d) * cells + n] = fin[d];
+}
+
+template <int Q, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void probe_store(uint32_t* __restrict__ ids, const Geometry g,
+                                                   const Physics<uint32_t> p) {
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    if (Bounded && is_wall(p.flags[n])) return;
+    uint32_t fin[Q];
+    PairSlots<Q> s;
+    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, p.flags, h, n, fin, s);  // records the slots
+    uint32_t fout[Q];
+    for (int d = 0; d < Q; ++d) fout[d] = d * p.cells + n;
+    store_cell<Q, uint32_t, Odd, Bounded>(ids, g, p, n, 1U, fout, s);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -559,13 +598,24 @@ template <int Q, class T> struct Ops final : KernelOps {
         hipLaunchKernelGGL(diagnostics_final, dim3(1), dim3(256), 0, st.stream, partial, st.geo.tiles, out);
         check(hipGetLastError(), "diagnostics_final launch");
     }
-    void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, int64_t* load,
-                   int64_t* store) const override {
+    void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, uint32_t* ids,
+                   int64_t* load) const override {
+        hipLaunchKernelGGL(fill_ids, grid_of(st.geo), block_of(st.geo), 0, st.stream, ids, st.geo, st.cells, Q, 0U);
         per_parity(st, odd, bounded, [&](auto O, auto B) {
-            hipLaunchKernelGGL((cfd_hip::index_map<Q, T, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
-                               block_of(st.geo), 0, st.stream, st.geo, flags, st.cells, load, store);
+            hipLaunchKernelGGL((probe_load<Q, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
+                               block_of(st.geo), 0, st.stream, ids, st.geo, flags, st.cells, load);
         });
-        check(hipGetLastError(), "index_map launch");
+        check(hipGetLastError(), "probe_load launch");
+        hipLaunchKernelGGL(fill_ids, grid_of(st.geo), block_of(st.geo), 0, st.stream, ids, st.geo, st.cells, Q,
+                           0xFFFFFFFFU);
+        Physics<uint32_t> p{};
+        p.flags = flags;
+        p.cells = st.cells;
+        per_parity(st, odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((probe_store<Q, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
+                               block_of(st.geo), 0, st.stream, ids, st.geo, p);
+        });
+        check(hipGetLastError(), "probe_store launch");
     }
 };
 

@@ -32,8 +32,10 @@ struct KernelOps {
     virtual void macroscopic(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
     virtual void download(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
     virtual void diagnostics(const LaunchState& st, bool odd, bool bounded, double* partial, double* out) const = 0;
-    virtual void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, int64_t* load,
-                           int64_t* store) const = 0;
+    // ids: q*stride uint32 scratch; on return it holds, per slot, the identifier d*cells + n of the
+    // population stored there (0xFFFFFFFF if none); load: q*cells identifiers of the slots read.
+    virtual void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, uint32_t* ids,
+                           int64_t* load) const = 0;
 };
 
 const KernelOps* kernel_ops(int q, int storage);

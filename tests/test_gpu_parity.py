@@ -278,8 +278,10 @@ def test_walls_removed_restart_at_rest(hip, oracle_mod):
     o.set_flags(None)
     g.step(5)
     o.step(5)
-    # The frozen density is re-expanded from its float sum: equal to ~1 ulp, not bit for bit.
-    assert_fields(gpu_fields(g), o.macroscopic(), bit_exact=False, tol=1e-5)
+    # The frozen density is re-expanded from its float sum (1 + 2.4e-7 for D3Q19 at rest), so the
+    # restarted cells differ by ulps, not bits: rho and u agree to 1e-6 absolute.
+    gf, of = gpu_fields(g), np.stack(o.macroscopic()).reshape(4, -1)
+    assert np.abs(gf - of).max() < 1e-6
 
 
 def test_local_acceleration_parity(hip, oracle_mod):
@@ -321,7 +323,8 @@ def test_fp64_storage_parity(hip, oracle_mod, q):
     n = (14, 12, 10)
     g, o = make_pair(hip, oracle_mod, q, n, 0.7, (1e-6, 0, 0), storage="fp64")
     g.initialize_taylor_green(0.03, 1e-4, 3)
-    o.init_taylor_green(0.03, 1e-4, 3)
+    # The C ABI takes float arguments; the FP64 oracle gets the same float-rounded values.
+    o.init_taylor_green(float(np.float32(0.03)), float(np.float32(1e-4)), 3)
     g.step(11)
     o.step(11)
     assert_fields(gpu_fields(g, f64=True), o.macroscopic())
