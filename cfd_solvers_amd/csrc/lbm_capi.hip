@@ -66,7 +66,7 @@ struct cfd_lbm_solver {
     Geometry geo{};
     hipStream_t stream = nullptr;
     const KernelOps* ops = nullptr;
-    DeviceBuffer<unsigned char> f, flags, solid_rho, gamma, lacc, macro, partial, reduce_out;
+    DeviceBuffer<unsigned char> f, flags, links, link_vel, interior, solid_rho, gamma, lacc, macro, partial, reduce_out;
     float wall_u[8][3] = {};
     float u_peak = 0.0F, rho_out = 1.0F;
     uint64_t t = 0;
@@ -111,11 +111,12 @@ struct cfd_lbm_solver {
         const uint64_t tiles = static_cast<uint64_t>(geo.tiles_x) * geo.tiles_y * nz;
         if (tiles >= (uint64_t{1} << 31)) throw std::invalid_argument("lattice too large for one launch");
         geo.tiles = static_cast<uint32_t>(tiles);
-        geo.xcd_remap = (geo.tiles % 8U == 0U && geo.tiles >= 64U) ? 1U : 0U;
-        if (const char* v = std::getenv("CFD_LBM_XCD_REMAP")) geo.xcd_remap = geo.xcd_remap && std::atoi(v) != 0;
-        // Planes start on 256-element boundaries; CFD_LBM_PLANE_PAD (elements, rounded up to 256)
-        // staggers the planes so that they do not sit a power of two apart.
-        uint64_t pad = 0;
+        geo.xcd_remap = 0U;
+        if (const char* v = std::getenv("CFD_LBM_XCD_REMAP"))
+            geo.xcd_remap = (std::atoi(v) != 0 && geo.tiles % 8U == 0U && geo.tiles >= 64U) ? 1U : 0U;
+        // Planes start on 256-element boundaries and are staggered by 4352 elements so that they
+        // never sit a power of two apart (measured +2 % on 256^3); CFD_LBM_PLANE_PAD overrides.
+        uint64_t pad = 4352;
         if (const char* v = std::getenv("CFD_LBM_PLANE_PAD")) pad = std::strtoull(v, nullptr, 10);
         geo.stride = (static_cast<uint64_t>(cells) + pad + 255U) / 256U * 256U;
         // Kernels address a population as plane base + 32-bit byte offset, reaching one plane further
@@ -149,6 +150,8 @@ struct cfd_lbm_solver {
     void use_device() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
     bool odd() const { return (t & 1U) != 0U; }
     bool bounded() const { return static_cast<bool>(flags); }
+    uint32_t seg_x() const { return static_cast<uint32_t>((nx + 63) / 64); }
+    size_t vel_words() const { return static_cast<size_t>((q - 1 + 9) / 10); }
 
     template <class Real> void build_physics() {
         Physics<Real> p{};
@@ -164,6 +167,10 @@ struct cfd_lbm_solver {
         p.gamma = reinterpret_cast<const Real*>(gamma.ptr);
         p.lax = reinterpret_cast<const Real*>(lacc.ptr);
         p.solid_rho = reinterpret_cast<const Real*>(solid_rho.ptr);
+        p.links = reinterpret_cast<const uint32_t*>(links.ptr);
+        p.link_vel = reinterpret_cast<const uint32_t*>(link_vel.ptr);
+        p.interior = interior.ptr;
+        p.seg_x = seg_x();
         p.cells = static_cast<uint32_t>(cells);
         std::memcpy(physics, &p, sizeof(p));
     }
@@ -338,9 +345,17 @@ struct cfd_lbm_solver {
         if (!host_flags) {
             sync();
             flags.reset();
+            links.reset();
+            link_vel.reset();
+            interior.reset();
             solid_rho.reset();
             return;
         }
+        if (!interior) interior.allocate(static_cast<size_t>(seg_x()) * ny * nz);
+        ops->interior(st, fresh.ptr, seg_x(), interior.ptr);
+        if (!links) links.allocate(cells * sizeof(uint32_t));
+        if (!link_vel) link_vel.allocate(cells * vel_words() * sizeof(uint32_t));
+        ops->links(st, fresh.ptr, reinterpret_cast<uint32_t*>(links.ptr), reinterpret_cast<uint32_t*>(link_vel.ptr));
         if (!solid_rho) solid_rho.allocate(cells * real);
         ops->freeze(st, odd(), fresh.ptr, flags.ptr, solid_rho.ptr);
         ops->bounce_back(st, odd(), true, fresh.ptr);
@@ -377,9 +392,10 @@ struct cfd_lbm_solver {
     void step(uint64_t count) {
         use_device();
         const bool b = bounded();
+        const bool forced = ax != 0.0F || ay != 0.0F || az != 0.0F || static_cast<bool>(lacc);
         LaunchState st = state();
         for (uint64_t k = 0; k < count; ++k) {
-            ops->step(st, odd(), b);
+            ops->step(st, odd(), b, forced);
             ++t;
         }
     }
@@ -457,7 +473,7 @@ struct cfd_lbm_solver {
         loads.allocate(n * sizeof(int64_t));
         const LaunchState st = state();
         // Wall-velocity indices do not move slots; the probe sees plain walls.
-        DeviceBuffer<unsigned char> probe_flags;
+        DeviceBuffer<unsigned char> probe_flags, probe_links, probe_vel;
         if (bounded()) {
             std::vector<uint8_t> host(cells);
             hip_check(hipMemcpy(host.data(), flags.ptr, cells, hipMemcpyDeviceToHost), "hipMemcpy flags");
@@ -465,8 +481,13 @@ struct cfd_lbm_solver {
                 if (v & 0x80U) v = 0x80U;
             probe_flags.allocate(cells);
             hip_check(hipMemcpy(probe_flags.ptr, host.data(), cells, hipMemcpyHostToDevice), "hipMemcpy flags");
+            probe_links.allocate(cells * sizeof(uint32_t));
+            probe_vel.allocate(cells * vel_words() * sizeof(uint32_t));
+            ops->links(st, probe_flags.ptr, reinterpret_cast<uint32_t*>(probe_links.ptr),
+                       reinterpret_cast<uint32_t*>(probe_vel.ptr));
         }
-        ops->index_map(st, (parity & 1) != 0, bounded(), probe_flags.ptr, reinterpret_cast<uint32_t*>(ids.ptr),
+        ops->index_map(st, (parity & 1) != 0, bounded(), probe_flags.ptr,
+                       reinterpret_cast<const uint32_t*>(probe_links.ptr), reinterpret_cast<uint32_t*>(ids.ptr),
                        reinterpret_cast<int64_t*>(loads.ptr));
         std::vector<uint32_t> slots(static_cast<size_t>(q) * geo.stride);
         d2h(load, loads.ptr, n * sizeof(int64_t));
@@ -483,7 +504,7 @@ struct cfd_lbm_solver {
 
     size_t population_bytes() const { return static_cast<size_t>(q) * cells * elem; }
     size_t resident_bytes() const {
-        return f.bytes + flags.bytes + solid_rho.bytes + gamma.bytes + lacc.bytes + macro.bytes + partial.bytes +
+        return f.bytes + flags.bytes + links.bytes + link_vel.bytes + interior.bytes + solid_rho.bytes + gamma.bytes + lacc.bytes + macro.bytes + partial.bytes +
                reduce_out.bytes;
     }
 };

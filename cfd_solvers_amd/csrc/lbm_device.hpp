@@ -75,6 +75,7 @@ constexpr uint8_t kInlet = 0x01;
 constexpr uint8_t kOutlet = 0x02;
 constexpr uint8_t kWall = 0x80;
 __device__ __forceinline__ bool is_wall(uint8_t f) { return (f & kWall) != 0; }
+__device__ __forceinline__ bool is_wall(uint32_t f) { return (f & kWall) != 0; }
 
 // ------------------------------------------------------------------ population storage
 // fp32: the value itself.  fp16s: binary16 of (f - w_d) * 4096 (round to nearest even, the
@@ -112,8 +113,9 @@ template <> struct Storage<double> {
 
 // ------------------------------------------------------------------ launch geometry
 // A block is a bx*by = 256-thread tile of one z plane (bx spans x, multiple of 32).  Tiles are
-// numbered x-fastest; with xcd_remap the hardware's round-robin block->XCD dispatch is undone so
-// each XCD (and its L2) walks one contiguous slab of tiles.
+// numbered x-fastest.  The hardware deals consecutive blocks round-robin to the 8 XCDs, so the
+// chip streams through neighbouring rows together; xcd_remap (off by default: it measured slower
+// on MI355X) instead gives each XCD one contiguous slab of tiles.
 struct Geometry {
     uint32_t nx, ny, nz;
     uint32_t bx, by;
@@ -164,12 +166,23 @@ __device__ __forceinline__ Neighbourhood neighbourhood(const Geometry& g, uint32
 // cell stored the opposite population one step earlier.  Wall cells do nothing, and the two slots
 // of a pair at a wall cell are touched only by the one fluid cell facing it, so the update stays
 // race-free with a single pass over the lattice.
+// Per-cell link descriptor of a bounded lattice, built from the flags whenever they change:
+//   bits 0..q-2   link l is a wall link: link 2p is population i = 2p+1 (upstream cell n - c_i),
+//                 link 2p+1 is population i+1 (upstream cell n + c_i)
+//   bits 26..27   cell type: 0 fluid, 1 velocity inlet, 2 pressure outlet, 3 wall
+//   bit 31        at least one wall link has a non-zero wall velocity
+// One 32-bit load replaces the cell's own flag and its q-1 neighbour flags in the hot kernel.
+constexpr uint32_t kTypeShift = 26;
+constexpr uint32_t kTypeWall = 3;
+constexpr uint32_t kMovingBit = 31;
+
+// Per-cell slot state carried from the loads to the stores, kept small so the bounded kernel
+// stays close to the periodic kernel's register budget.
 template <int Q> struct PairSlots {
-    uint32_t up[Q];        // byte offset of cell up = n - c_i in a plane, index by i (odd)
-    uint8_t wall_up[Q];    // upstream wall flag of direction i (cell up)
-    uint8_t wall_dn[Q];    // upstream wall flag of direction i+1 (cell dn)
-    uint8_t flag_up[Q];    // full flags of up / dn (wall velocity index)
-    uint8_t flag_dn[Q];
+    static constexpr int kVelWords = (Q - 1 + 9) / 10;  // 3-bit wall-velocity index per link, 10 per word
+    uint32_t up[Q];  // byte offset of cell up = n - c_i in a plane, index by i (odd)
+    uint32_t desc;   // link descriptor (bounded lattices only)
+    __device__ __forceinline__ bool is_wall_link(int link) const { return ((desc >> link) & 1U) != 0U; }
 };
 
 // Direction plane holding fin[i] (a) and fin[i+1] (b) of pair i.
@@ -239,6 +252,10 @@ template <class Real> struct Physics {
     const Real* gamma;          // null: no partial bounce-back
     const Real* lax;            // null: no local body acceleration (lay = lax + cells, laz = lay + cells)
     const Real* solid_rho;      // density frozen in wall cells
+    const uint32_t* links;      // per-cell link descriptor (bounded lattices)
+    const uint32_t* link_vel;   // per cell, (q-1+9)/10 words of 3-bit wall-velocity indices per link
+    const uint8_t* interior;    // per 64-cell row segment: 1 if its 3x3x66 neighbourhood is all fluid
+    uint32_t seg_x;             // segments per row, ceil(nx / 64)
     uint32_t cells;
 };
 
@@ -304,7 +321,10 @@ __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometr
 
 // Collision of one non-wall cell in registers.  Returns the density used for moving-wall
 // momentum corrections (src/lbm/d2q9.cpp:217-223 uses the fluid cell's density).
-template <int Q, class Real>
+// Forced selects the Guo-forcing variant at compile time: the unforced variant carries no force
+// temporaries and fits twice the waves per SIMD.  With Forced, a zero acceleration still takes the
+// unforced arithmetic (the reference's guo_force returns 0 then), so both variants agree bitwise.
+template <int Q, bool Forced, class Real>
 __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
                                         uint32_t y, uint32_t z, Real (&f)[Q]) {
     if (flag == kInlet || flag == kOutlet) {
@@ -327,22 +347,31 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
     } else {
         ux = uy = uz = Real(0);
     }
-    Real ax = p.ax, ay = p.ay, az = p.az;
-    if (p.lax) {
-        ax = ax + p.lax[n];
-        ay = ay + p.lax[p.cells + n];
-        az = az + p.lax[2U * p.cells + n];
+    Real ax = Real(0), ay = Real(0), az = Real(0);
+    if constexpr (Forced) {
+        ax = p.ax;
+        ay = p.ay;
+        az = p.az;
+        if (p.lax) {
+            ax = ax + p.lax[n];
+            ay = ay + p.lax[p.cells + n];
+            az = az + p.lax[2U * p.cells + n];
+        }
     }
-    ux = ux + Real(0.5) * ax;
-    uy = uy + Real(0.5) * ay;
-    uz = uz + Real(0.5) * az;
+    // The reference always adds a/2 (esoteric_pull.hpp:418-420); u + 0 is u, so the unforced
+    // variant skips it.
+    if constexpr (Forced) {
+        ux = ux + Real(0.5) * ax;
+        uy = uy + Real(0.5) * ay;
+        uz = uz + Real(0.5) * az;
+    }
     const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
     const Real omega = p.omega;
     Real fin_copy[Q];
     if (p.gamma) {
         static_for<Q>([&](auto DD) { fin_copy[decltype(DD)::value] = f[decltype(DD)::value]; });
     }
-    if (ax == Real(0) && ay == Real(0) && az == Real(0)) {
+    if (!Forced || (ax == Real(0) && ay == Real(0) && az == Real(0))) {
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
             f[d] = f[d] - omega * (f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu));

@@ -36,6 +36,9 @@ template <class T> __device__ __forceinline__ T load_at(const void* base, uint32
 #endif
 }
 template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t byte_off, T v) {
+    // Opaque to the optimiser: otherwise it materialises all 64-bit store addresses before the
+    // collision and keeps them live (2 VGPRs each), which halves the occupancy.
+    asm volatile("" : "+v"(byte_off));
     T* p = reinterpret_cast<T*>(static_cast<unsigned char*>(base) + byte_off);
 #if defined(CFD_LBM_NT_STORE)
     __builtin_nontemporal_store(v, p);
@@ -46,13 +49,14 @@ template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t
 
 // Loads the logical populations of a non-wall cell (bounce-back aware) and records the slots.
 template <int Q, class T, bool Odd, bool Bounded>
-__device__ __forceinline__ void load_cell(const void* f, const Geometry& g, const uint8_t* flags,
+__device__ __forceinline__ void load_cell(const void* f, const Geometry& g, const uint32_t* links,
                                           const Neighbourhood& h, uint32_t n,
                                           typename Storage<T>::Real (&fin)[Q], PairSlots<Q>& s) {
     using L = Lattice<Q>;
     using S = Storage<T>;
     const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
     const uint32_t plane_bytes = static_cast<uint32_t>(g.stride * sizeof(T));
+    if constexpr (Bounded) s.desc = links[n];
     fin[0] = S::template decode<Q, 0>(load_at<T>(plane<T>(f, g, 0), off_n));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
@@ -60,13 +64,8 @@ __device__ __forceinline__ void load_cell(const void* f, const Geometry& g, cons
         const uint32_t off_up = up * static_cast<uint32_t>(sizeof(T));
         s.up[i] = off_up;
         if constexpr (Bounded) {
-            const uint32_t dn = h.cell(L::cx[i], L::cy[i], L::cz[i]);
-            s.flag_up[i] = flags[up];
-            s.flag_dn[i] = flags[dn];
-            const bool wup = is_wall(s.flag_up[i]);
-            const bool wdn = is_wall(s.flag_dn[i]);
-            s.wall_up[i] = wup;
-            s.wall_dn[i] = wdn;
+            const bool wup = s.is_wall_link(i - 1);  // population i comes from a wall at up
+            const bool wdn = s.is_wall_link(i);      // population i+1 comes from a wall at dn
             // Plane i holds the pair; a wall link selects the partner plane (one plane further).
             const uint32_t sel_a = (Odd != wup) ? plane_bytes : 0U;
             const uint32_t sel_b = (Odd != wdn) ? 0U : plane_bytes;
@@ -83,65 +82,162 @@ __device__ __forceinline__ void load_cell(const void* f, const Geometry& g, cons
 
 template <int Q, class T, bool Odd, bool Bounded>
 __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Physics<typename Storage<T>::Real>& p,
-                                           uint32_t n, typename Storage<T>::Real rho,
-                                           typename Storage<T>::Real (&fout)[Q], const PairSlots<Q>& s) {
+                                           uint32_t n, uint32_t x, uint32_t y, uint32_t z,
+                                           typename Storage<T>::Real rho, typename Storage<T>::Real (&fout)[Q],
+                                           const PairSlots<Q>& s) {
     using L = Lattice<Q>;
     using S = Storage<T>;
     using Real = typename Storage<T>::Real;
+    if constexpr (Bounded) {
+        if ((s.desc >> kMovingBit) & 1U) {
+            // Moving wall: a population bounced off a wall of velocity u_w is stored as
+            // f - 6 w rho_w (c . u_w), the push form of src/lbm/d2q9.cpp:217-223.  Only cells next
+            // to a moving wall come here; they read their per-link wall-velocity indices.
+            constexpr int kW = PairSlots<Q>::kVelWords;
+            uint32_t vel[kW];
+            for (int w = 0; w < kW; ++w) vel[w] = p.link_vel[static_cast<uint64_t>(n) * kW + w];
+            const Real rho_wall = rho > Real(1.0e-8) ? rho : Real(1.0e-8);
+            static_for<(Q - 1) / 2>([&](auto P) {
+                constexpr int i = 2 * decltype(P)::value + 1;
+                // fout[i] travels to dn = n + c_i (link i); fout[i+1] to up = n - c_i (link i-1).
+                const uint32_t ka = (vel[i / 10] >> (3 * (i % 10))) & 7U;
+                if (ka != 0U) {
+                    const Real cu = static_cast<Real>(L::cx[i]) * p.wall_u[ka][0] +
+                                    static_cast<Real>(L::cy[i]) * p.wall_u[ka][1] +
+                                    static_cast<Real>(L::cz[i]) * p.wall_u[ka][2];
+                    fout[i] = fout[i] - Real(6) * static_cast<Real>(L::w[i]) * rho_wall * cu;
+                }
+                const uint32_t kb = (vel[(i - 1) / 10] >> (3 * ((i - 1) % 10))) & 7U;
+                if (kb != 0U) {
+                    const Real cu = static_cast<Real>(L::cx[i + 1]) * p.wall_u[kb][0] +
+                                    static_cast<Real>(L::cy[i + 1]) * p.wall_u[kb][1] +
+                                    static_cast<Real>(L::cz[i + 1]) * p.wall_u[kb][2];
+                    fout[i + 1] = fout[i + 1] - Real(6) * static_cast<Real>(L::w[i + 1]) * rho_wall * cu;
+                }
+            });
+        }
+    }
     const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
     store_at<T>(plane<T>(f, g, 0), off_n, S::template encode<Q, 0>(fout[0]));
-    const Real rho_wall = rho > Real(1.0e-8) ? rho : Real(1.0e-8);
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
-        Real a = fout[i];      // travels to dn = n + c_i
-        Real b = fout[i + 1];  // travels to up = n - c_i
-        if constexpr (Bounded) {
-            // Moving wall: the bounced population loses 6 w rho_w (c . u_w) (push form of
-            // src/lbm/d2q9.cpp:217-223).
-            const int ka = s.flag_dn[i] & 7;
-            if (s.wall_dn[i] && ka != 0) {
-                const Real cu = static_cast<Real>(L::cx[i]) * p.wall_u[ka][0] +
-                                static_cast<Real>(L::cy[i]) * p.wall_u[ka][1] +
-                                static_cast<Real>(L::cz[i]) * p.wall_u[ka][2];
-                a = a - Real(6) * static_cast<Real>(L::w[i]) * rho_wall * cu;
-            }
-            const int kb = s.flag_up[i] & 7;
-            if (s.wall_up[i] && kb != 0) {
-                const Real cu = static_cast<Real>(L::cx[i + 1]) * p.wall_u[kb][0] +
-                                static_cast<Real>(L::cy[i + 1]) * p.wall_u[kb][1] +
-                                static_cast<Real>(L::cz[i + 1]) * p.wall_u[kb][2];
-                b = b - Real(6) * static_cast<Real>(L::w[i + 1]) * rho_wall * cu;
-            }
-        }
         if constexpr (Odd) {
-            store_at<T>(plane<T>(f, g, i), off_n, S::template encode<Q, i>(a));
-            store_at<T>(plane<T>(f, g, i + 1), s.up[i], S::template encode<Q, i + 1>(b));
+            store_at<T>(plane<T>(f, g, i), off_n, S::template encode<Q, i>(fout[i]));
+            store_at<T>(plane<T>(f, g, i + 1), s.up[i], S::template encode<Q, i + 1>(fout[i + 1]));
         } else {
-            store_at<T>(plane<T>(f, g, i + 1), off_n, S::template encode<Q, i>(a));
-            store_at<T>(plane<T>(f, g, i), s.up[i], S::template encode<Q, i + 1>(b));
+            store_at<T>(plane<T>(f, g, i + 1), off_n, S::template encode<Q, i>(fout[i]));
+            store_at<T>(plane<T>(f, g, i), s.up[i], S::template encode<Q, i + 1>(fout[i + 1]));
         }
     });
 }
 
 // ------------------------------------------------------------------ hot kernel
-template <int Q, class T, bool Odd, bool Bounded>
-__global__ __launch_bounds__(256) void stream_collide(void* __restrict__ f, const Geometry g,
-                                                      const Physics<typename Storage<T>::Real> p) {
+template <int Q, class T, bool Odd, bool Bounded, bool Forced>
+__device__ __forceinline__ void update_cell(void* f, const Geometry& g, const Physics<typename Storage<T>::Real>& p,
+                                            const Neighbourhood& h, uint8_t flag, uint32_t n, uint32_t x,
+                                            uint32_t y, uint32_t z) {
     using Real = typename Storage<T>::Real;
+    Real fin[Q];
+    PairSlots<Q> s;
+    load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+    const Real rho = collide<Q, Forced>(p, g, flag, n, y, z, fin);
+    store_cell<Q, T, Odd, Bounded>(f, g, p, n, x, y, z, rho, fin, s);
+}
+
+#if defined(CFD_LBM_WAVES_PER_EU)
+#define CFD_LBM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(CFD_LBM_WAVES_PER_EU)))
+#else
+#define CFD_LBM_OCCUPANCY
+#endif
+
+template <int Q, class T, bool Odd, bool Bounded, bool Forced>
+__global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(void* __restrict__ f, const Geometry g,
+                                                                        const Physics<typename Storage<T>::Real> p) {
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
     const uint32_t n = h.cell(0, 0, 0);
-    uint8_t flag = kFluid;
     if constexpr (Bounded) {
-        flag = p.flags[n];
-        if (is_wall(flag)) return;
+        // Wavefronts whose whole neighbourhood is plain fluid take the periodic path: no flag
+        // loads, no bounce-back selects.  A wavefront covers one 64-cell row segment when
+        // bx >= 64, so the branch is uniform there.
+        if (p.interior[(z * g.ny + y) * p.seg_x + (x >> 6)] != 0U) {
+            update_cell<Q, T, Odd, false, Forced>(f, g, p, h, kFluid, n, x, y, z);
+            return;
+        }
+        const uint32_t type = (p.links[n] >> kTypeShift) & 3U;
+        if (type == kTypeWall) return;
+        update_cell<Q, T, Odd, true, Forced>(f, g, p, h, static_cast<uint8_t>(type), n, x, y, z);
+    } else {
+        update_cell<Q, T, Odd, false, Forced>(f, g, p, h, kFluid, n, x, y, z);
     }
-    Real fin[Q];
-    PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
-    const Real rho = collide<Q>(p, g, flag, n, y, z, fin);
-    store_cell<Q, T, Odd, Bounded>(f, g, p, n, rho, fin, s);
+}
+
+// 1 for a 64-cell row segment whose cells and all their neighbours (x from 64s-1 to 64s+64,
+// y +- 1, z +- 1, periodic) are plain fluid.
+__global__ __launch_bounds__(256) void mark_interior(const uint8_t* __restrict__ flags, const Geometry g,
+                                                     uint32_t seg_x, uint8_t* __restrict__ interior) {
+    const uint32_t total = seg_x * g.ny * g.nz;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const uint32_t sx = k % seg_x;
+    const uint32_t r = k / seg_x;
+    const uint32_t y = r % g.ny;
+    const uint32_t z = r / g.ny;
+    const int x0 = static_cast<int>(sx * 64U) - 1;
+    const int x1 = static_cast<int>(min(sx * 64U + 64U, g.nx));  // inclusive: last cell + 1
+    uint8_t all_fluid = 1U;
+    for (int dz = -1; dz <= 1 && all_fluid; ++dz) {
+        const uint32_t zz = (z + g.nz + static_cast<uint32_t>(dz)) % g.nz;
+        for (int dy = -1; dy <= 1 && all_fluid; ++dy) {
+            const uint32_t yy = (y + g.ny + static_cast<uint32_t>(dy)) % g.ny;
+            const uint32_t row = (zz * g.ny + yy) * g.nx;
+            for (int xx = x0; xx <= x1; ++xx) {
+                const uint32_t xw = static_cast<uint32_t>((xx + static_cast<int>(g.nx)) % static_cast<int>(g.nx));
+                if (flags[row + xw] != kFluid) {
+                    all_fluid = 0U;
+                    break;
+                }
+            }
+        }
+    }
+    interior[k] = all_fluid;
+}
+
+// Link descriptor of every cell from the flags (layout in lbm_device.hpp).
+template <int Q>
+__global__ __launch_bounds__(256) void build_links(const uint8_t* __restrict__ flags, const Geometry g,
+                                                   uint32_t* __restrict__ links, uint32_t* __restrict__ link_vel) {
+    using L = Lattice<Q>;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    const uint8_t own = flags[n];
+    const uint32_t type = is_wall(own) ? kTypeWall : (own == kInlet ? 1U : (own == kOutlet ? 2U : 0U));
+    constexpr int kW = PairSlots<Q>::kVelWords;
+    uint32_t desc = type << kTypeShift;
+    uint32_t vel[kW] = {};
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint8_t fu = flags[h.cell(-L::cx[i], -L::cy[i], -L::cz[i])];
+        const uint8_t fd = flags[h.cell(L::cx[i], L::cy[i], L::cz[i])];
+        if (is_wall(fu)) {
+            desc |= 1U << (i - 1);
+            vel[(i - 1) / 10] |= static_cast<uint32_t>(fu & 7U) << (3 * ((i - 1) % 10));
+        }
+        if (is_wall(fd)) {
+            desc |= 1U << i;
+            vel[i / 10] |= static_cast<uint32_t>(fd & 7U) << (3 * (i % 10));
+        }
+    });
+    bool moving = false;
+    for (int w = 0; w < kW; ++w) {
+        moving = moving || vel[w] != 0U;
+        link_vel[static_cast<uint64_t>(n) * kW + w] = vel[w];
+    }
+    if (moving) desc |= 1U << kMovingBit;
+    links[n] = desc;
 }
 
 // ------------------------------------------------------------------ initialisation
@@ -338,7 +434,7 @@ __global__ __launch_bounds__(256) void macroscopic(const void* __restrict__ f, c
     }
     Real fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
+    load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
     Real rho, ux, uy, uz;
     cell_state<Q>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
     out[n] = rho;
@@ -368,7 +464,7 @@ __global__ __launch_bounds__(256) void download_logical(const void* __restrict__
     }
     Real fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
+    load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
     static_for<Q>([&](auto DD) {
         constexpr int d = decltype(DD)::value;
         out[static_cast<uint64_t>(d) * p.cells + n] = fin[d];
@@ -401,7 +497,7 @@ __global__ __launch_bounds__(256) void diagnostics_partial(const void* __restric
         } else {
             Real fin[Q];
             PairSlots<Q> s;
-            load_cell<Q, T, Odd, Bounded>(f, g, p.flags, h, n, fin, s);
+            load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
             Real sum = Real(0);
             static_for<Q>([&](auto DD) { sum += fin[decltype(DD)::value]; });
             mass = static_cast<double>(sum);
@@ -474,7 +570,8 @@ __global__ __launch_bounds__(256) void fill_ids(uint32_t* __restrict__ ids, cons
 
 template <int Q, bool Odd, bool Bounded>
 __global__ __launch_bounds__(256) void probe_load(const uint32_t* __restrict__ ids, const Geometry g,
-                                                  const uint8_t* __restrict__ flags, uint32_t cells,
+                                                  const uint8_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ links, uint32_t cells,
                                                   int64_t* __restrict__ load) {
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
@@ -486,7 +583,7 @@ __global__ __launch_bounds__(256) void probe_load(const uint32_t* __restrict__ i
     }
     uint32_t fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, flags, h, n, fin, s);
+    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, links, h, n, fin, s);
     for (int d = 0; d < Q; ++d) load[static_cast<uint64_t>(d) * cells + n] = fin[d];
 }
 
@@ -500,10 +597,10 @@ __global__ __launch_bounds__(256) void probe_store(uint32_t* __restrict__ ids, c
     if (Bounded && is_wall(p.flags[n])) return;
     uint32_t fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, p.flags, h, n, fin, s);  // records the slots
+    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, p.links, h, n, fin, s);  // records the slots
     uint32_t fout[Q];
     for (int d = 0; d < Q; ++d) fout[d] = d * p.cells + n;
-    store_cell<Q, uint32_t, Odd, Bounded>(ids, g, p, n, 1U, fout, s);
+    store_cell<Q, uint32_t, Odd, Bounded>(ids, g, p, n, x, y, z, 1U, fout, s);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -517,13 +614,15 @@ inline dim3 grid_of(const Geometry& g) { return dim3(g.tiles, 1, 1); }
 template <int Q, class T> struct Ops final : KernelOps {
     using Real = typename Storage<T>::Real;
 
-    template <bool Odd, bool Bounded> static void step_t(const LaunchState& st) {
-        hipLaunchKernelGGL((stream_collide<Q, T, Odd, Bounded>), grid_of(st.geo), block_of(st.geo), 0, st.stream,
-                           st.f, st.geo, *static_cast<const Physics<Real>*>(st.physics));
+    template <bool Odd, bool Bounded, bool Forced> static void step_t(const LaunchState& st) {
+        hipLaunchKernelGGL((stream_collide<Q, T, Odd, Bounded, Forced>), grid_of(st.geo), block_of(st.geo), 0,
+                           st.stream, st.f, st.geo, *static_cast<const Physics<Real>*>(st.physics));
     }
-    void step(const LaunchState& st, bool odd, bool bounded) const override {
-        if (odd) bounded ? step_t<true, true>(st) : step_t<true, false>(st);
-        else bounded ? step_t<false, true>(st) : step_t<false, false>(st);
+    void step(const LaunchState& st, bool odd, bool bounded, bool forced) const override {
+        per_parity(st, odd, bounded, [&](auto O, auto B) {
+            if (forced) step_t<decltype(O)::value, decltype(B)::value, true>(st);
+            else step_t<decltype(O)::value, decltype(B)::value, false>(st);
+        });
         check(hipGetLastError(), "stream_collide launch");
     }
     void initialize(const LaunchState& st, const void* init_args, const uint8_t* flags) const override {
@@ -557,6 +656,12 @@ template <int Q, class T> struct Ops final : KernelOps {
             hipLaunchKernelGGL((freeze_wall_density<Q, T, false>), grid_of(st.geo), block_of(st.geo), 0, st.stream,
                                st.f, st.geo, flags, old_flags, static_cast<Real*>(solid_rho));
         check(hipGetLastError(), "freeze_wall_density launch");
+    }
+    void interior(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint8_t* out) const override {
+        const uint32_t total = seg_x * st.geo.ny * st.geo.nz;
+        hipLaunchKernelGGL(mark_interior, dim3((total + 255U) / 256U), dim3(256), 0, st.stream, flags, st.geo, seg_x,
+                           out);
+        check(hipGetLastError(), "mark_interior launch");
     }
     void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
               const void* solid_rho) const override {
@@ -598,19 +703,25 @@ template <int Q, class T> struct Ops final : KernelOps {
         hipLaunchKernelGGL(diagnostics_final, dim3(1), dim3(256), 0, st.stream, partial, st.geo.tiles, out);
         check(hipGetLastError(), "diagnostics_final launch");
     }
-    void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, uint32_t* ids,
-                   int64_t* load) const override {
+    void links(const LaunchState& st, const uint8_t* flags, uint32_t* out, uint32_t* vel) const override {
+        hipLaunchKernelGGL((build_links<Q>), grid_of(st.geo), block_of(st.geo), 0, st.stream, flags, st.geo, out,
+                           vel);
+        check(hipGetLastError(), "build_links launch");
+    }
+    void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, const uint32_t* links,
+                   uint32_t* ids, int64_t* load) const override {
         hipLaunchKernelGGL(fill_ids, grid_of(st.geo), block_of(st.geo), 0, st.stream, ids, st.geo, st.cells, Q, 0U);
         per_parity(st, odd, bounded, [&](auto O, auto B) {
             hipLaunchKernelGGL((probe_load<Q, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
-                               block_of(st.geo), 0, st.stream, ids, st.geo, flags, st.cells, load);
+                               block_of(st.geo), 0, st.stream, ids, st.geo, flags, links, st.cells, load);
         });
         check(hipGetLastError(), "probe_load launch");
         hipLaunchKernelGGL(fill_ids, grid_of(st.geo), block_of(st.geo), 0, st.stream, ids, st.geo, st.cells, Q,
                            0xFFFFFFFFU);
         Physics<uint32_t> p{};
         p.flags = flags;
-        p.cells = st.cells;
+        p.links = links;
+        p.cells = st.cells;  // probe flags carry no wall velocities: link_vel is never read
         per_parity(st, odd, bounded, [&](auto O, auto B) {
             hipLaunchKernelGGL((probe_store<Q, decltype(O)::value, decltype(B)::value>), grid_of(st.geo),
                                block_of(st.geo), 0, st.stream, ids, st.geo, p);

@@ -21,12 +21,13 @@ struct LaunchState {
 // One instance per (lattice, storage); every launch is asynchronous on st.stream.
 struct KernelOps {
     virtual ~KernelOps() = default;
-    virtual void step(const LaunchState& st, bool odd, bool bounded) const = 0;
+    virtual void step(const LaunchState& st, bool odd, bool bounded, bool forced) const = 0;
     virtual void initialize(const LaunchState& st, const void* init_args, const uint8_t* flags) const = 0;
     virtual void upload(const LaunchState& st, bool odd, const void* logical) const = 0;
     virtual void bounce_back(const LaunchState& st, bool odd, bool to_bounce_back, const uint8_t* flags) const = 0;
     virtual void freeze(const LaunchState& st, bool odd, const uint8_t* flags, const uint8_t* old_flags,
                         void* solid_rho) const = 0;
+    virtual void interior(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint8_t* out) const = 0;
     virtual void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
                       const void* solid_rho) const = 0;
     virtual void macroscopic(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
@@ -34,8 +35,10 @@ struct KernelOps {
     virtual void diagnostics(const LaunchState& st, bool odd, bool bounded, double* partial, double* out) const = 0;
     // ids: q*stride uint32 scratch; on return it holds, per slot, the identifier d*cells + n of the
     // population stored there (0xFFFFFFFF if none); load: q*cells identifiers of the slots read.
-    virtual void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags, uint32_t* ids,
-                           int64_t* load) const = 0;
+    virtual void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags,
+                           const uint32_t* links, uint32_t* ids, int64_t* load) const = 0;
+    // out: per-cell link descriptor; vel: (q-1+9)/10 words per cell of wall-velocity indices
+    virtual void links(const LaunchState& st, const uint8_t* flags, uint32_t* out, uint32_t* vel) const = 0;
 };
 
 const KernelOps* kernel_ops(int q, int storage);
