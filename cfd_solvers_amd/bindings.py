@@ -62,6 +62,10 @@ _SIGNATURES = {
     "cfd_lbm_stream": [C.c_void_p, C.POINTER(C.c_void_p)],
     "cfd_lbm_transfer_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.c_int],
     "cfd_lbm_index_map": [C.c_void_p, C.c_int, _i64p, _i64p],
+    "cfd_lbm_objective_pressure_drop": [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)],
+    "cfd_lbm_adjoint_run": [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                            C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_double)],
+    "cfd_lbm_adjoint_timing": [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)],
 }
 
 
@@ -282,6 +286,24 @@ class EsotericPullHipSolver:
         self.lib.call("cfd_lbm_transfer_stats", self._h, v, int(reset))
         return {"host_to_device_bytes": v[0], "device_to_host_bytes": v[1],
                 "device_to_device_bytes": v[2], "synchronization_points": v[3]}
+
+    def objective_pressure_drop(self, x_in, x_out):
+        v = C.c_double()
+        self.lib.call("cfd_lbm_objective_pressure_drop", self._h, x_in, x_out, C.byref(v))
+        return v.value
+
+    def adjoint_run(self, steps, window, checkpoint_every, x_in, x_out, want_gamma=True):
+        """J, dJ/dgamma (shape (nz, ny, nx), FP64) and dJ/du_peak; the state advances by `steps`."""
+        J, du = C.c_double(), C.c_double()
+        grad = np.zeros(self.cells) if want_gamma else None
+        self.lib.call("cfd_lbm_adjoint_run", self._h, steps, window, checkpoint_every, x_in, x_out, C.byref(J),
+                      grad.ctypes.data if grad is not None else None, C.byref(du))
+        return J.value, (grad.reshape(self.shape) if grad is not None else None), du.value
+
+    def adjoint_timing(self):
+        fw, rv = C.c_float(), C.c_float()
+        self.lib.call("cfd_lbm_adjoint_timing", self._h, C.byref(fw), C.byref(rv))
+        return fw.value, rv.value
 
     def index_map(self, parity):
         load = np.empty(self.q * self.cells, dtype=np.int64)

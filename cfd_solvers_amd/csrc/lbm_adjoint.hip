@@ -1,0 +1,351 @@
+// lbm_adjoint.hip -- discrete adjoint of the in-place esoteric-pull step, and the pressure-drop
+// objective it differentiates.
+//
+// In slot space one time step is cell-local: cell n reads its slots L(n) (esoteric map with
+// bounce-back) and writes its slots W(n), and no other cell touches them.  The transposed step is
+// therefore cell-local and in place as well: cell n reads lambda_{t+1} from W(n), applies the
+// transposed collision Jacobian at the forward state f_t (read from L(n) of a stored state), and
+// writes lambda_t to L(n).  A bounce-back link reads slot X (opposite parity) and writes slot Y;
+// X is written one step later and never read again, Y is overwritten without being read, so the
+// adjoint writes the Jacobian term to X and zero to Y -- no pass-through terms are needed.
+//
+// Differentiated physics: unforced BGK with partial bounce-back gamma, stationary halfway walls,
+// the velocity inlet (parabolic, Zou-He density) and the pressure outlet (Zou-He velocity).
+// Gradients: dJ/dgamma per cell and dJ/du_peak, accumulated in FP64.  The reference has no adjoint;
+// this is checked against a logical-space adjoint (reverse streaming + transposed collision) and
+// central finite differences in the tests.
+#include "lbm_adjoint.hpp"
+#include "lbm_cell.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+namespace cfd_hip {
+
+namespace {
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// d feq_d / d rho = e and d feq_d / d u = gu for the second-order equilibrium.
+template <int Q, int D, class Real>
+__device__ __forceinline__ void feq_grad(Real rho, Real ux, Real uy, Real uz, Real uu_plain, Real& e, Real (&gu)[3]) {
+    using L = Lattice<Q>;
+    const Real w = static_cast<Real>(L::w[D]);
+    const Real cu = cdot<Q, D>(ux, uy, uz);
+    e = w * (Real(1) + Real(3) * cu + Real(4.5) * cu * cu - Real(1.5) * uu_plain);
+    const Real wr = w * rho;
+    gu[0] = wr * (Real(3) * static_cast<Real>(L::cx[D]) + Real(9) * cu * static_cast<Real>(L::cx[D]) - Real(3) * ux);
+    gu[1] = wr * (Real(3) * static_cast<Real>(L::cy[D]) + Real(9) * cu * static_cast<Real>(L::cy[D]) - Real(3) * uy);
+    gu[2] = wr * (Real(3) * static_cast<Real>(L::cz[D]) + Real(9) * cu * static_cast<Real>(L::cz[D]) - Real(3) * uz);
+}
+
+// mu = (d fout / d fin)^T lam for one cell; also d/dgamma and d/du_peak of lam . fout.
+template <int Q, class Real>
+__device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
+                                              uint32_t y, uint32_t z, const Real (&f)[Q], const Real (&lam)[Q],
+                                              Real (&mu)[Q], double& dgamma, double& dpeak) {
+    using L = Lattice<Q>;
+    dgamma = 0.0;
+    dpeak = 0.0;
+    if (flag == kInlet) {
+        const Real uin = inlet_velocity(p, g, y, z, L::dims == 3);
+        const Real eta = (static_cast<Real>(y) - Real(0.5)) / static_cast<Real>(g.ny - 2U);
+        Real P = Real(4) * eta * (Real(1) - eta);
+        if (L::dims == 3) {
+            const Real zeta = (static_cast<Real>(z) - Real(0.5)) / static_cast<Real>(g.nz - 2U);
+            P = P * (Real(4) * zeta * (Real(1) - zeta));
+        }
+        const Real inv = Real(1) / (Real(1) - uin);
+        const Real rho = (sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, -1>(f)) * inv;
+        Real g_rho = Real(0), g_ux = Real(0);
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            Real e, gu[3];
+            feq_grad<Q, d>(rho, uin, Real(0), Real(0), uin * uin, e, gu);
+            g_rho += lam[d] * e;
+            g_ux += lam[d] * gu[0];
+        });
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            constexpr int c = L::cx[d];
+            mu[d] = c == 0 ? g_rho * inv : (c < 0 ? Real(2) * g_rho * inv : Real(0));
+        });
+        dpeak = static_cast<double>((g_ux + g_rho * rho * inv) * P);
+        return;
+    }
+    if (flag == kOutlet) {
+        const Real rho = p.rho_out;
+        const Real ux = Real(-1) + (sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, 1>(f)) / rho;
+        Real g_ux = Real(0);
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            Real e, gu[3];
+            feq_grad<Q, d>(rho, ux, Real(0), Real(0), ux * ux, e, gu);
+            g_ux += lam[d] * gu[0];
+        });
+        const Real s = g_ux / rho;
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            constexpr int c = L::cx[d];
+            mu[d] = c == 0 ? s : (c > 0 ? Real(2) * s : Real(0));
+        });
+        return;
+    }
+    Real rho, mx, my, mz;
+    moments<Q>(f, rho, mx, my, mz);
+    const Real ux = mx / rho, uy = my / rho, uz = mz / rho;
+    const Real uu = ux * ux + uy * uy + uz * uz;
+    const Real omega = p.omega;
+    const Real gm = p.gamma ? p.gamma[n] : Real(0);
+    const Real keep = Real(1) - gm;
+    Real g_rho = Real(0), G0 = Real(0), G1 = Real(0), G2 = Real(0), dg = Real(0);
+    Real a[Q];
+    static_for<Q>([&](auto DD) {
+        constexpr int d = decltype(DD)::value;
+        Real e, gu[3];
+        feq_grad<Q, d>(rho, ux, uy, uz, uu, e, gu);
+        a[d] = keep * lam[d];
+        const Real oa = omega * a[d];
+        g_rho += oa * e;
+        G0 += oa * gu[0];
+        G1 += oa * gu[1];
+        G2 += oa * gu[2];
+        const Real fstar = f[d] - omega * (f[d] - rho * e);
+        dg += lam[d] * (f[opposite(d)] - fstar);
+    });
+    G0 = G0 / rho;
+    G1 = G1 / rho;
+    G2 = G2 / rho;
+    const Real Gu = G0 * ux + G1 * uy + G2 * uz;
+    const Real base = g_rho - Gu;
+    static_for<Q>([&](auto DD) {
+        constexpr int d = decltype(DD)::value;
+        const Real Gc = cdot<Q, d>(G0, G1, G2);
+        mu[d] = (Real(1) - omega) * a[d] + gm * lam[opposite(d)] + base + Gc;
+    });
+    dgamma = p.gamma ? static_cast<double>(dg) : 0.0;
+}
+
+// Objective plane weight of a cell: c_in on plane x_in, c_out on plane x_out (fluid cells only).
+template <class Real>
+__device__ __forceinline__ Real plane_weight(const AdjointArgs& a, uint32_t x, bool fluid) {
+    if (!fluid) return Real(0);
+    if (static_cast<int>(x) == a.x_in) return static_cast<Real>(a.c_in);
+    if (static_cast<int>(x) == a.x_out) return static_cast<Real>(a.c_out);
+    return Real(0);
+}
+
+// One transposed step at the given parity, in place on lam.  fstate holds the forward state f_t.
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fstate, void* __restrict__ lam_buf,
+                                                    const Geometry g, const Physics<typename Storage<T>::Real> p,
+                                                    const AdjointArgs a) {
+    using L = Lattice<Q>;
+    using Real = typename Storage<T>::Real;
+    using S = Storage<Real>;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    uint8_t flag = kFluid;
+    if constexpr (Bounded) {
+        const uint32_t type = (p.links[n] >> kTypeShift) & 3U;
+        if (type == kTypeWall) return;
+        flag = static_cast<uint8_t>(type);
+    }
+    Real f[Q];
+    PairSlots<Q> s;
+    load_cell<Q, T, Odd, Bounded>(fstate, g, p.links, h, n, f, s);
+    // lambda_{t+1} at the slots this cell wrote (its store slots).
+    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(Real));
+    const uint32_t rb = static_cast<uint32_t>(sizeof(Real));
+    const uint32_t plane_bytes = static_cast<uint32_t>(g.stride * sizeof(Real));
+    Real lam[Q];
+    lam[0] = load_at<Real>(plane<Real>(lam_buf, g, 0), off_n);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t off_up = s.up[i] / static_cast<uint32_t>(sizeof(T)) * rb;
+        lam[i] = load_at<Real>(plane<Real>(lam_buf, g, Odd ? i : i + 1), off_n);
+        lam[i + 1] = load_at<Real>(plane<Real>(lam_buf, g, Odd ? i + 1 : i), off_up);
+    });
+    Real mu[Q];
+    double dgamma, dpeak;
+    collision_vjp<Q>(p, g, flag, n, y, z, f, lam, mu, dgamma, dpeak);
+    if (a.add_objective) {
+        const Real c = plane_weight<Real>(a, x, flag == kFluid);
+        if (c != Real(0)) static_for<Q>([&](auto DD) { mu[decltype(DD)::value] += c; });
+    }
+    // lambda_t at the slots this cell read; zero on bounce-back store slots that were not read.
+    store_at<Real>(plane<Real>(lam_buf, g, 0), off_n, mu[0]);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t off_up = s.up[i] / static_cast<uint32_t>(sizeof(T)) * rb;
+        bool wup = false, wdn = false;
+        if constexpr (Bounded) {
+            wup = s.is_wall_link(i - 1);
+            wdn = s.is_wall_link(i);
+        }
+        const uint32_t sel_a = (Odd != wup) ? plane_bytes : 0U;
+        const uint32_t sel_b = (Odd != wdn) ? 0U : plane_bytes;
+        store_at<Real>(plane<Real>(lam_buf, g, i), off_up + sel_a, mu[i]);
+        store_at<Real>(plane<Real>(lam_buf, g, i), off_n + sel_b, mu[i + 1]);
+        if (wup) store_at<Real>(plane<Real>(lam_buf, g, Odd ? i + 1 : i), off_up, Real(0));
+        if (wdn) store_at<Real>(plane<Real>(lam_buf, g, Odd ? i : i + 1), off_n, Real(0));
+    });
+    if (p.gamma && flag == kFluid) a.grad_gamma[n] += dgamma;
+    if (flag == kInlet) a.grad_peak[n] += dpeak;
+}
+
+// lambda_T = dJ/df_T: zero everywhere plus the objective's weight at the slots read at parity T.
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void adjoint_seed(void* __restrict__ lam_buf, const Geometry g,
+                                                    const Physics<typename Storage<T>::Real> p, const AdjointArgs a) {
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    for (int d = 0; d < Q; ++d) static_cast<Real*>(lam_buf)[static_cast<uint64_t>(d) * g.stride + n] = Real(0);
+    (void)h;
+}
+
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void adjoint_seed_source(void* __restrict__ lam_buf, const Geometry g,
+                                                           const Physics<typename Storage<T>::Real> p,
+                                                           const AdjointArgs a) {
+    using L = Lattice<Q>;
+    using Real = typename Storage<T>::Real;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    const uint32_t n = h.cell(0, 0, 0);
+    bool fluid = true;
+    uint32_t desc = 0;
+    if constexpr (Bounded) {
+        desc = p.links[n];
+        fluid = ((desc >> kTypeShift) & 3U) == 0U;
+    }
+    const Real c = plane_weight<Real>(a, x, fluid);
+    if (c == Real(0)) return;
+    Real* lam = static_cast<Real*>(lam_buf);
+    lam[n] += c;
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+        const bool wup = Bounded && ((desc >> (i - 1)) & 1U);
+        const bool wdn = Bounded && ((desc >> i) & 1U);
+        lam[static_cast<uint64_t>(dir_a<Odd>(i, wup)) * g.stride + up] += c;
+        lam[static_cast<uint64_t>(dir_b<Odd>(i, wdn)) * g.stride + n] += c;
+    });
+}
+
+// j(f) = sum_{x = x_in} rho * c_in + sum_{x = x_out} rho * c_out over fluid cells, in FP64, by one
+// block in a fixed order (deterministic); acc += weight * j.
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(1024) void plane_objective(const void* __restrict__ f, const Geometry g,
+                                                        const Physics<typename Storage<T>::Real> p,
+                                                        const AdjointArgs a, double weight, double* acc) {
+    using Real = typename Storage<T>::Real;
+    __shared__ double sm[1024];
+    const uint32_t per_plane = g.ny * g.nz;
+    double sum = 0.0;
+    for (uint32_t k = threadIdx.x; k < 2U * per_plane; k += blockDim.x) {
+        const uint32_t which = k / per_plane;
+        const uint32_t r = k % per_plane;
+        const uint32_t y = r % g.ny, z = r / g.ny;
+        const uint32_t x = static_cast<uint32_t>(which == 0 ? a.x_in : a.x_out);
+        const Neighbourhood h = neighbourhood(g, x, y, z);
+        const uint32_t n = h.cell(0, 0, 0);
+        bool fluid = true;
+        if constexpr (Bounded) fluid = ((p.links[n] >> kTypeShift) & 3U) == 0U;
+        if (!fluid) continue;
+        Real fin[Q];
+        PairSlots<Q> s;
+        load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+        Real rho = Real(0);
+        static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
+        sum += static_cast<double>(rho) * (which == 0 ? a.c_in : a.c_out);
+    }
+    sm[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *acc += weight * sm[0];
+}
+
+__global__ __launch_bounds__(256) void fill_zero_f64(double* __restrict__ v, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = 0.0;
+}
+
+template <int Q, class T> struct AdjOps final : AdjointOps {
+    using Real = typename Storage<T>::Real;
+    template <class K> static void per_parity(bool odd, bool bounded, K k) {
+        if (odd) bounded ? k(std::true_type{}, std::true_type{}) : k(std::true_type{}, std::false_type{});
+        else bounded ? k(std::false_type{}, std::true_type{}) : k(std::false_type{}, std::false_type{});
+    }
+    static dim3 block(const Geometry& g) { return dim3(g.bx, g.by, 1); }
+    static dim3 grid(const Geometry& g) { return dim3(g.tiles, 1, 1); }
+
+    void step(const LaunchState& st, bool odd, bool bounded, const void* fstate, void* lam,
+              const AdjointArgs& a) const override {
+        per_parity(odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((adjoint_step<Q, T, decltype(O)::value, decltype(B)::value>), grid(st.geo),
+                               block(st.geo), 0, st.stream, fstate, lam, st.geo,
+                               *static_cast<const Physics<Real>*>(st.physics), a);
+        });
+        check(hipGetLastError(), "adjoint_step launch");
+    }
+    void seed(const LaunchState& st, bool odd, bool bounded, void* lam, const AdjointArgs& a) const override {
+        per_parity(odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((adjoint_seed<Q, T, decltype(O)::value, decltype(B)::value>), grid(st.geo),
+                               block(st.geo), 0, st.stream, lam, st.geo,
+                               *static_cast<const Physics<Real>*>(st.physics), a);
+            hipLaunchKernelGGL((adjoint_seed_source<Q, T, decltype(O)::value, decltype(B)::value>), grid(st.geo),
+                               block(st.geo), 0, st.stream, lam, st.geo,
+                               *static_cast<const Physics<Real>*>(st.physics), a);
+        });
+        check(hipGetLastError(), "adjoint_seed launch");
+    }
+    void objective(const LaunchState& st, bool odd, bool bounded, const void* f, const AdjointArgs& a, double weight,
+                   double* acc) const override {
+        per_parity(odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((plane_objective<Q, T, decltype(O)::value, decltype(B)::value>), dim3(1), dim3(1024),
+                               0, st.stream, f, st.geo, *static_cast<const Physics<Real>*>(st.physics), a, weight,
+                               acc);
+        });
+        check(hipGetLastError(), "plane_objective launch");
+    }
+    void zero(const LaunchState& st, double* v, uint64_t n) const override {
+        hipLaunchKernelGGL(fill_zero_f64, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st.stream, v,
+                           n);
+        check(hipGetLastError(), "fill_zero launch");
+    }
+};
+
+template <int Q> const AdjointOps* adj_for_storage(int storage) {
+    static const AdjOps<Q, float> f32;
+    static const AdjOps<Q, double> f64;
+    if (storage == 0) return &f32;
+    if (storage == 2) return &f64;
+    throw std::invalid_argument("the adjoint runs on FP32 or FP64 population storage");
+}
+
+}  // namespace
+
+const AdjointOps* adjoint_ops(int q, int storage) {
+    switch (q) {
+    case 9: return adj_for_storage<9>(storage);
+    case 19: return adj_for_storage<19>(storage);
+    case 27: return adj_for_storage<27>(storage);
+    default: throw std::invalid_argument("lattice must be D2Q9 (9), D3Q19 (19) or D3Q27 (27)");
+    }
+}
+
+}  // namespace cfd_hip

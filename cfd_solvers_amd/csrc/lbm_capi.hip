@@ -6,6 +6,7 @@
 // ledger (include/cfd/core/device_residency.hpp:11-42).  Error convention of
 // src/python/c_api.cpp:8-15: 0 ok, 1 std::exception, 2 unknown, message in a caller buffer.
 #include "../../include/cfd_lbm/c_api.h"
+#include "lbm_adjoint.hpp"
 #include "lbm_kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -15,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -70,6 +72,9 @@ struct cfd_lbm_solver {
     float wall_u[8][3] = {};
     float u_peak = 0.0F, rho_out = 1.0F;
     uint64_t t = 0;
+    uint32_t step_lds = 0;
+    std::vector<uint8_t> host_flags;  // copy of the flags (empty: periodic)
+    float last_forward_ms = 0.0F, last_reverse_ms = 0.0F;
     mutable TransferStats stats;
     // Host copy of the Physics<Real> passed by value to every launch.
     alignas(16) unsigned char physics[sizeof(Physics<double>)] = {};
@@ -101,6 +106,18 @@ struct cfd_lbm_solver {
         // Launch geometry: 256-thread tiles, bx a power of two in [32, 256] covering x.
         uint32_t bx = 32;
         while (bx < nx && bx < 256) bx <<= 1;
+        if (const char* v = std::getenv("CFD_LBM_BX")) {  // tuning: tile width 32, 64, 128 or 256
+            const long w = std::atol(v);
+            if (w == 32 || w == 64 || w == 128 || w == 256) bx = static_cast<uint32_t>(w);
+        }
+        // Resident wavefronts per SIMD of the step kernel, capped by reserving LDS per block (160 KiB
+        // per CU; a 256-thread block puts one wave on each of the 4 SIMDs).  Measured on MI355X at
+        // 256^3: FP32 D3Q19 is fastest at 5 waves (4 % over 8), FP32 D3Q27 at 4; FP16 wants all 8.
+        // CFD_LBM_WAVES (3..8) overrides.
+        long waves = 8;
+        if (storage == 0) waves = lattice == 27 ? 4 : 5;
+        if (const char* v = std::getenv("CFD_LBM_WAVES")) waves = std::atol(v);
+        if (waves >= 3 && waves < 8) step_lds = static_cast<uint32_t>(163840 / waves - 1024);
         geo.nx = static_cast<uint32_t>(nx);
         geo.ny = static_cast<uint32_t>(ny);
         geo.nz = static_cast<uint32_t>(nz);
@@ -178,7 +195,7 @@ struct cfd_lbm_solver {
     LaunchState state() {
         if (storage == 2) build_physics<double>();
         else build_physics<float>();
-        return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells)};
+        return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells), step_lds};
     }
 
     void sync() const {
@@ -337,6 +354,8 @@ struct cfd_lbm_solver {
             fresh.allocate(cells);
             h2d(fresh.ptr, host_flags, cells);
         }
+        if (host_flags) host_flags_assign(host_flags);
+        else this->host_flags.clear();
         LaunchState st = state();
         if (bounded()) {
             ops->bounce_back(st, odd(), false, flags.ptr);  // back to the plain layout
@@ -362,6 +381,144 @@ struct cfd_lbm_solver {
         sync();
         std::swap(flags.ptr, fresh.ptr);
         std::swap(flags.bytes, fresh.bytes);
+    }
+
+    void host_flags_assign(const uint8_t* v) { this->host_flags.assign(v, v + cells); }
+
+    // Pressure drop between fluid cells of planes x_in and x_out at the current state.
+    cfd_hip::AdjointArgs objective_args(int x_in, int x_out, uint64_t window) const {
+        if (x_in < 0 || x_out < 0 || static_cast<size_t>(x_in) >= nx || static_cast<size_t>(x_out) >= nx || x_in == x_out)
+            throw std::invalid_argument("objective planes must be two different x indices inside the lattice");
+        size_t a = 0, b = 0;
+        for (size_t z = 0; z < nz; ++z)
+            for (size_t y = 0; y < ny; ++y) {
+                const size_t row = (z * ny + y) * nx;
+                const bool fa = host_flags.empty() || host_flags[row + static_cast<size_t>(x_in)] == 0;
+                const bool fb = host_flags.empty() || host_flags[row + static_cast<size_t>(x_out)] == 0;
+                a += fa;
+                b += fb;
+            }
+        if (a == 0 || b == 0) throw std::invalid_argument("objective planes contain no fluid cells");
+        cfd_hip::AdjointArgs args{};
+        args.x_in = x_in;
+        args.x_out = x_out;
+        args.c_in = 1.0 / (3.0 * static_cast<double>(a) * static_cast<double>(window));
+        args.c_out = -1.0 / (3.0 * static_cast<double>(b) * static_cast<double>(window));
+        return args;
+    }
+
+    double objective(int x_in, int x_out) {
+        use_device();
+        const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage == 1 ? 0 : storage);
+        DeviceBuffer<unsigned char> acc;
+        acc.allocate(sizeof(double));
+        const LaunchState st = state();
+        adj->zero(st, reinterpret_cast<double*>(acc.ptr), 1);
+        if (storage == 1) throw std::invalid_argument("the objective runs on FP32 or FP64 population storage");
+        adj->objective(st, odd(), bounded(), f.ptr, objective_args(x_in, x_out, 1), 1.0, reinterpret_cast<double*>(acc.ptr));
+        double v = 0.0;
+        d2h(&v, acc.ptr, sizeof(double));
+        sync();
+        return v;
+    }
+
+    // Forward `steps` steps from the current state, J = mean of j over the last `window` states,
+    // then the discrete adjoint back to the current state with two-level checkpointing: a snapshot
+    // every `every` steps on the way forward, and per segment an out-of-place recomputation of its
+    // states into a replay buffer before the transposed steps run through it.
+    double adjoint_run(uint64_t steps, uint64_t window, uint64_t every, int x_in, int x_out, double* grad_gamma_out,
+                       double* grad_peak_out) {
+        use_device();
+        if (storage == 1) throw std::invalid_argument("the adjoint runs on FP32 or FP64 population storage");
+        if (steps < 1 || window < 1 || window > steps) throw std::invalid_argument("need 1 <= window <= steps");
+        if (every < 1) throw std::invalid_argument("checkpoint interval must be >= 1");
+        if (ax != 0.0F || ay != 0.0F || az != 0.0F || lacc) throw std::invalid_argument("the adjoint supports unforced lattices only");
+        for (uint8_t v : host_flags)
+            if ((v & 0x80U) && (v & 7U) && (wall_u[v & 7U][0] != 0.0F || wall_u[v & 7U][1] != 0.0F || wall_u[v & 7U][2] != 0.0F))
+                throw std::invalid_argument("the adjoint supports walls at rest only");
+        const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage);
+        const uint64_t segments = (steps + every - 1) / every;
+        const size_t state_bytes = static_cast<size_t>(q) * geo.stride * elem;
+        const size_t replay_states = static_cast<size_t>(std::min<uint64_t>(every, steps) - 1);
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const size_t need = (segments + replay_states) * state_bytes + state_bytes / elem * real + 2 * cells * 8 + 64;
+        if (need > free_b)
+            throw std::invalid_argument("checkpoints and replay buffer do not fit in device memory (" +
+                                        std::to_string(need >> 20) + " MiB needed, " + std::to_string(free_b >> 20) +
+                                        " MiB free): raise the checkpoint interval");
+        std::vector<std::unique_ptr<DeviceBuffer<unsigned char>>> snaps(segments), replay(replay_states);
+        for (auto& b : snaps) {
+            b = std::make_unique<DeviceBuffer<unsigned char>>();
+            b->allocate(state_bytes);
+        }
+        for (auto& b : replay) {
+            b = std::make_unique<DeviceBuffer<unsigned char>>();
+            b->allocate(state_bytes);
+        }
+        DeviceBuffer<unsigned char> lam, gg, gp, acc;
+        lam.allocate(static_cast<size_t>(q) * geo.stride * real);
+        gg.allocate(cells * sizeof(double));
+        gp.allocate(cells * sizeof(double));
+        acc.allocate(sizeof(double));
+        cfd_hip::AdjointArgs args = objective_args(x_in, x_out, window);
+        args.grad_gamma = reinterpret_cast<double*>(gg.ptr);
+        args.grad_peak = reinterpret_cast<double*>(gp.ptr);
+        const LaunchState st = state();
+        adj->zero(st, args.grad_gamma, cells);
+        adj->zero(st, args.grad_peak, cells);
+        adj->zero(st, reinterpret_cast<double*>(acc.ptr), 1);
+        const bool b = bounded();
+        const uint64_t t0 = t;
+        hipEvent_t e0, e1, e2;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventCreate(&e2), "hipEventCreate");
+        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        for (uint64_t k = 0; k < steps; ++k) {
+            if (k % every == 0)
+                hip_check(hipMemcpyAsync(snaps[k / every]->ptr, f.ptr, state_bytes, hipMemcpyDeviceToDevice, stream),
+                          "hipMemcpyAsync snapshot");
+            ops->step(st, ((t0 + k) & 1U) != 0U, b, false, f.ptr, f.ptr);
+            if (k + 1 >= steps - window + 1)
+                adj->objective(st, ((t0 + k + 1) & 1U) != 0U, b, f.ptr, args, 1.0, reinterpret_cast<double*>(acc.ptr));
+        }
+        t = t0 + steps;
+        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+        args.add_objective = 0;
+        adj->seed(st, (t & 1U) != 0U, b, lam.ptr, args);
+        for (uint64_t c = segments; c-- > 0;) {
+            const uint64_t s0 = c * every, e = std::min(steps, s0 + every);
+            std::vector<const void*> states(e - s0);
+            states[0] = snaps[c]->ptr;
+            for (uint64_t k = 1; k < e - s0; ++k) {
+                ops->step(st, ((t0 + s0 + k - 1) & 1U) != 0U, b, false, states[k - 1], replay[k - 1]->ptr);
+                states[k] = replay[k - 1]->ptr;
+            }
+            for (uint64_t k = e - s0; k-- > 0;) {
+                const uint64_t tt = s0 + k;
+                args.add_objective = tt + window >= steps + 1 ? 1 : 0;
+                adj->step(st, ((t0 + tt) & 1U) != 0U, b, states[k], lam.ptr, args);
+            }
+        }
+        hip_check(hipEventRecord(e2, stream), "hipEventRecord");
+        double J = 0.0;
+        d2h(&J, acc.ptr, sizeof(double));
+        std::vector<double> peak(cells);
+        d2h(peak.data(), gp.ptr, cells * sizeof(double));
+        if (grad_gamma_out) d2h(grad_gamma_out, gg.ptr, cells * sizeof(double));
+        sync();
+        (void)hipEventElapsedTime(&last_forward_ms, e0, e1);
+        (void)hipEventElapsedTime(&last_reverse_ms, e1, e2);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        if (grad_peak_out) {
+            double sum = 0.0;
+            for (double v : peak) sum += v;
+            *grad_peak_out = sum;
+        }
+        return J;
     }
 
     void set_porosity(const float* g) {
@@ -395,7 +552,7 @@ struct cfd_lbm_solver {
         const bool forced = ax != 0.0F || ay != 0.0F || az != 0.0F || static_cast<bool>(lacc);
         LaunchState st = state();
         for (uint64_t k = 0; k < count; ++k) {
-            ops->step(st, odd(), b, forced);
+            ops->step(st, odd(), b, forced, f.ptr, f.ptr);
             ++t;
         }
     }
@@ -717,6 +874,29 @@ int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int reset, char*
             stats4[3] = st.syncs;
         }
         if (reset) st = TransferStats{};
+    });
+}
+
+int cfd_lbm_objective_pressure_drop(cfd_lbm_solver* s, int x_in, int x_out, double* value, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        const double v = need(s).objective(x_in, x_out);
+        if (value) *value = v;
+    });
+}
+
+int cfd_lbm_adjoint_run(cfd_lbm_solver* s, uint64_t steps, uint64_t window, uint64_t checkpoint_every, int x_in,
+                        int x_out, double* objective, double* grad_gamma, double* grad_u_peak, char* error,
+                        size_t cap) {
+    return guarded(error, cap, [&] {
+        const double J = need(s).adjoint_run(steps, window, checkpoint_every, x_in, x_out, grad_gamma, grad_u_peak);
+        if (objective) *objective = J;
+    });
+}
+
+int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_ms, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (forward_ms) *forward_ms = need(s).last_forward_ms;
+        if (reverse_ms) *reverse_ms = s->last_reverse_ms;
     });
 }
 

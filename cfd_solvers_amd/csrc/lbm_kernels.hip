@@ -6,6 +6,7 @@
 // once per setup or per diagnostic: initialisation, bounce-back slot preparation, macroscopic
 // fields, FP64 reductions, the index-map probe.
 #include "lbm_kernels.hpp"
+#include "lbm_cell.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -17,131 +18,17 @@ namespace cfd_hip {
 
 namespace {
 
-template <class T> __device__ __forceinline__ T* plane(void* base, const Geometry& g, int d) {
-    return static_cast<T*>(base) + static_cast<uint64_t>(d) * g.stride;
-}
-template <class T> __device__ __forceinline__ const T* plane(const void* base, const Geometry& g, int d) {
-    return static_cast<const T*>(base) + static_cast<uint64_t>(d) * g.stride;
-}
-
-// Byte-offset access: a uniform 64-bit plane base (SGPRs) plus a 32-bit per-lane byte offset
-// (one VGPR), which maps onto the global_load/store "saddr" form.  Two consecutive planes must
-// fit in 4 GiB (checked at solver creation).
-template <class T> __device__ __forceinline__ T load_at(const void* base, uint32_t byte_off) {
-    const T* p = reinterpret_cast<const T*>(static_cast<const unsigned char*>(base) + byte_off);
-#if defined(CFD_LBM_NT_LOAD)
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
-}
-template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t byte_off, T v) {
-    // Opaque to the optimiser: otherwise it materialises all 64-bit store addresses before the
-    // collision and keeps them live (2 VGPRs each), which halves the occupancy.
-    asm volatile("" : "+v"(byte_off));
-    T* p = reinterpret_cast<T*>(static_cast<unsigned char*>(base) + byte_off);
-#if defined(CFD_LBM_NT_STORE)
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
-
-// Loads the logical populations of a non-wall cell (bounce-back aware) and records the slots.
-template <int Q, class T, bool Odd, bool Bounded>
-__device__ __forceinline__ void load_cell(const void* f, const Geometry& g, const uint32_t* links,
-                                          const Neighbourhood& h, uint32_t n,
-                                          typename Storage<T>::Real (&fin)[Q], PairSlots<Q>& s) {
-    using L = Lattice<Q>;
-    using S = Storage<T>;
-    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
-    const uint32_t plane_bytes = static_cast<uint32_t>(g.stride * sizeof(T));
-    if constexpr (Bounded) s.desc = links[n];
-    fin[0] = S::template decode<Q, 0>(load_at<T>(plane<T>(f, g, 0), off_n));
-    static_for<(Q - 1) / 2>([&](auto P) {
-        constexpr int i = 2 * decltype(P)::value + 1;
-        const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
-        const uint32_t off_up = up * static_cast<uint32_t>(sizeof(T));
-        s.up[i] = off_up;
-        if constexpr (Bounded) {
-            const bool wup = s.is_wall_link(i - 1);  // population i comes from a wall at up
-            const bool wdn = s.is_wall_link(i);      // population i+1 comes from a wall at dn
-            // Plane i holds the pair; a wall link selects the partner plane (one plane further).
-            const uint32_t sel_a = (Odd != wup) ? plane_bytes : 0U;
-            const uint32_t sel_b = (Odd != wdn) ? 0U : plane_bytes;
-            fin[i] = S::template decode<Q, i>(load_at<T>(plane<T>(f, g, i), off_up + sel_a));
-            fin[i + 1] = S::template decode<Q, i + 1>(load_at<T>(plane<T>(f, g, i), off_n + sel_b));
-        } else {
-            // The slot of either plane of a pair holds a population of the direction it is
-            // loaded as (opposite directions share one weight, hence one FP16 shift).
-            fin[i] = S::template decode<Q, i>(load_at<T>(plane<T>(f, g, Odd ? i + 1 : i), off_up));
-            fin[i + 1] = S::template decode<Q, i + 1>(load_at<T>(plane<T>(f, g, Odd ? i : i + 1), off_n));
-        }
-    });
-}
-
-template <int Q, class T, bool Odd, bool Bounded>
-__device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Physics<typename Storage<T>::Real>& p,
-                                           uint32_t n, uint32_t x, uint32_t y, uint32_t z,
-                                           typename Storage<T>::Real rho, typename Storage<T>::Real (&fout)[Q],
-                                           const PairSlots<Q>& s) {
-    using L = Lattice<Q>;
-    using S = Storage<T>;
-    using Real = typename Storage<T>::Real;
-    if constexpr (Bounded) {
-        if ((s.desc >> kMovingBit) & 1U) {
-            // Moving wall: a population bounced off a wall of velocity u_w is stored as
-            // f - 6 w rho_w (c . u_w), the push form of src/lbm/d2q9.cpp:217-223.  Only cells next
-            // to a moving wall come here; they read their per-link wall-velocity indices.
-            constexpr int kW = PairSlots<Q>::kVelWords;
-            uint32_t vel[kW];
-            for (int w = 0; w < kW; ++w) vel[w] = p.link_vel[static_cast<uint64_t>(n) * kW + w];
-            const Real rho_wall = rho > Real(1.0e-8) ? rho : Real(1.0e-8);
-            static_for<(Q - 1) / 2>([&](auto P) {
-                constexpr int i = 2 * decltype(P)::value + 1;
-                // fout[i] travels to dn = n + c_i (link i); fout[i+1] to up = n - c_i (link i-1).
-                const uint32_t ka = (vel[i / 10] >> (3 * (i % 10))) & 7U;
-                if (ka != 0U) {
-                    const Real cu = static_cast<Real>(L::cx[i]) * p.wall_u[ka][0] +
-                                    static_cast<Real>(L::cy[i]) * p.wall_u[ka][1] +
-                                    static_cast<Real>(L::cz[i]) * p.wall_u[ka][2];
-                    fout[i] = fout[i] - Real(6) * static_cast<Real>(L::w[i]) * rho_wall * cu;
-                }
-                const uint32_t kb = (vel[(i - 1) / 10] >> (3 * ((i - 1) % 10))) & 7U;
-                if (kb != 0U) {
-                    const Real cu = static_cast<Real>(L::cx[i + 1]) * p.wall_u[kb][0] +
-                                    static_cast<Real>(L::cy[i + 1]) * p.wall_u[kb][1] +
-                                    static_cast<Real>(L::cz[i + 1]) * p.wall_u[kb][2];
-                    fout[i + 1] = fout[i + 1] - Real(6) * static_cast<Real>(L::w[i + 1]) * rho_wall * cu;
-                }
-            });
-        }
-    }
-    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
-    store_at<T>(plane<T>(f, g, 0), off_n, S::template encode<Q, 0>(fout[0]));
-    static_for<(Q - 1) / 2>([&](auto P) {
-        constexpr int i = 2 * decltype(P)::value + 1;
-        if constexpr (Odd) {
-            store_at<T>(plane<T>(f, g, i), off_n, S::template encode<Q, i>(fout[i]));
-            store_at<T>(plane<T>(f, g, i + 1), s.up[i], S::template encode<Q, i + 1>(fout[i + 1]));
-        } else {
-            store_at<T>(plane<T>(f, g, i + 1), off_n, S::template encode<Q, i>(fout[i]));
-            store_at<T>(plane<T>(f, g, i), s.up[i], S::template encode<Q, i + 1>(fout[i + 1]));
-        }
-    });
-}
-
 // ------------------------------------------------------------------ hot kernel
 template <int Q, class T, bool Odd, bool Bounded, bool Forced>
-__device__ __forceinline__ void update_cell(void* f, const Geometry& g, const Physics<typename Storage<T>::Real>& p,
-                                            const Neighbourhood& h, uint8_t flag, uint32_t n, uint32_t x,
-                                            uint32_t y, uint32_t z) {
+__device__ __forceinline__ void update_cell(const void* src, void* dst, const Geometry& g,
+                                            const Physics<typename Storage<T>::Real>& p, const Neighbourhood& h,
+                                            uint8_t flag, uint32_t n, uint32_t x, uint32_t y, uint32_t z) {
     using Real = typename Storage<T>::Real;
     Real fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+    load_cell<Q, T, Odd, Bounded>(src, g, p.links, h, n, fin, s);
     const Real rho = collide<Q, Forced>(p, g, flag, n, y, z, fin);
-    store_cell<Q, T, Odd, Bounded>(f, g, p, n, x, y, z, rho, fin, s);
+    store_cell<Q, T, Odd, Bounded>(dst, g, p, n, x, y, z, rho, fin, s);
 }
 
 #if defined(CFD_LBM_WAVES_PER_EU)
@@ -150,26 +37,28 @@ __device__ __forceinline__ void update_cell(void* f, const Geometry& g, const Ph
 #define CFD_LBM_OCCUPANCY
 #endif
 
+// In place when src == dst (the time step); src != dst writes the next state into another buffer
+// (recomputation of stored states for the adjoint).  Every slot the next step reads is written.
 template <int Q, class T, bool Odd, bool Bounded, bool Forced>
-__global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(void* __restrict__ f, const Geometry g,
+__global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const void* src, void* dst, const Geometry g,
                                                                         const Physics<typename Storage<T>::Real> p) {
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
     const uint32_t n = h.cell(0, 0, 0);
     if constexpr (Bounded) {
-        // Wavefronts whose whole neighbourhood is plain fluid take the periodic path: no flag
+        // Wavefronts whose whole neighbourhood is plain fluid take the periodic path: no link
         // loads, no bounce-back selects.  A wavefront covers one 64-cell row segment when
         // bx >= 64, so the branch is uniform there.
         if (p.interior[(z * g.ny + y) * p.seg_x + (x >> 6)] != 0U) {
-            update_cell<Q, T, Odd, false, Forced>(f, g, p, h, kFluid, n, x, y, z);
+            update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, n, x, y, z);
             return;
         }
         const uint32_t type = (p.links[n] >> kTypeShift) & 3U;
         if (type == kTypeWall) return;
-        update_cell<Q, T, Odd, true, Forced>(f, g, p, h, static_cast<uint8_t>(type), n, x, y, z);
+        update_cell<Q, T, Odd, true, Forced>(src, dst, g, p, h, static_cast<uint8_t>(type), n, x, y, z);
     } else {
-        update_cell<Q, T, Odd, false, Forced>(f, g, p, h, kFluid, n, x, y, z);
+        update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, n, x, y, z);
     }
 }
 
@@ -614,14 +503,15 @@ inline dim3 grid_of(const Geometry& g) { return dim3(g.tiles, 1, 1); }
 template <int Q, class T> struct Ops final : KernelOps {
     using Real = typename Storage<T>::Real;
 
-    template <bool Odd, bool Bounded, bool Forced> static void step_t(const LaunchState& st) {
-        hipLaunchKernelGGL((stream_collide<Q, T, Odd, Bounded, Forced>), grid_of(st.geo), block_of(st.geo), 0,
-                           st.stream, st.f, st.geo, *static_cast<const Physics<Real>*>(st.physics));
+    template <bool Odd, bool Bounded, bool Forced>
+    static void step_t(const LaunchState& st, const void* src, void* dst) {
+        hipLaunchKernelGGL((stream_collide<Q, T, Odd, Bounded, Forced>), grid_of(st.geo), block_of(st.geo),
+                           st.step_lds, st.stream, src, dst, st.geo, *static_cast<const Physics<Real>*>(st.physics));
     }
-    void step(const LaunchState& st, bool odd, bool bounded, bool forced) const override {
+    void step(const LaunchState& st, bool odd, bool bounded, bool forced, const void* src, void* dst) const override {
         per_parity(st, odd, bounded, [&](auto O, auto B) {
-            if (forced) step_t<decltype(O)::value, decltype(B)::value, true>(st);
-            else step_t<decltype(O)::value, decltype(B)::value, false>(st);
+            if (forced) step_t<decltype(O)::value, decltype(B)::value, true>(st, src, dst);
+            else step_t<decltype(O)::value, decltype(B)::value, false>(st, src, dst);
         });
         check(hipGetLastError(), "stream_collide launch");
     }

@@ -16,12 +16,15 @@ struct LaunchState {
     const void* physics;     // Physics<float> or Physics<double> (host copy, passed by value)
     hipStream_t stream;
     uint32_t cells;
+    uint32_t step_lds;       // dynamic LDS bytes per step-kernel block (occupancy cap; 0 = none)
 };
 
 // One instance per (lattice, storage); every launch is asynchronous on st.stream.
 struct KernelOps {
     virtual ~KernelOps() = default;
-    virtual void step(const LaunchState& st, bool odd, bool bounded, bool forced) const = 0;
+    // One time step from src into dst (src == dst: in place).
+    virtual void step(const LaunchState& st, bool odd, bool bounded, bool forced, const void* src,
+                      void* dst) const = 0;
     virtual void initialize(const LaunchState& st, const void* init_args, const uint8_t* flags) const = 0;
     virtual void upload(const LaunchState& st, bool odd, const void* logical) const = 0;
     virtual void bounce_back(const LaunchState& st, bool odd, bool to_bounce_back, const uint8_t* flags) const = 0;

@@ -137,6 +137,23 @@ CFD_LBM_API int cfd_lbm_stream(cfd_lbm_solver* s, void** stream, char* error, si
 CFD_LBM_API int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int reset, char* error,
                                        size_t error_capacity);
 
+/* Pressure-drop objective at the current state (FP64): j = mean rho/3 over the fluid cells of plane
+ * x = x_in minus the same over plane x = x_out.  No reference counterpart (the reference has no
+ * objectives; docs/RELEASE_0_10_9.md:39).  FP32 or FP64 storage. */
+CFD_LBM_API int cfd_lbm_objective_pressure_drop(cfd_lbm_solver* s, int x_in, int x_out, double* value, char* error,
+                                                size_t error_capacity);
+/* Discrete adjoint (no reference counterpart: the reference has no adjoint).  Runs `steps` steps
+ * from the current state, J = mean of the pressure drop over the last `window` states, then the
+ * transposed steps back with a snapshot every `checkpoint_every` steps; returns J, dJ/dgamma per
+ * cell (cells doubles, may be NULL) and dJ/du_peak.  The state is left at time + steps.  Unforced
+ * lattices with walls at rest, PBB, inlet and outlet; FP32 or FP64 storage. */
+CFD_LBM_API int cfd_lbm_adjoint_run(cfd_lbm_solver* s, uint64_t steps, uint64_t window, uint64_t checkpoint_every,
+                                    int x_in, int x_out, double* objective, double* grad_gamma, double* grad_u_peak,
+                                    char* error, size_t error_capacity);
+/* GPU time of the last adjoint run's forward phase and reverse (recompute + transposed) phase. */
+CFD_LBM_API int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_ms, char* error,
+                                       size_t error_capacity);
+
 /* Debug: the slots (d'*cells + n') read (load) and written (store) for every (d, n) by the
  * step kernel at the given parity, computed by the kernel's own index code (q*cells each).
  * Wall cells report -1.  Periodic lattices follow esoteric_pull.hpp:349-382. */

@@ -437,3 +437,29 @@ class RefD2Q9:
 
     def mass(self):
         return self.L.ref_d2q9_mass(self.h)
+
+
+def _adjoint_bindings(L):
+    L.orc_objective.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.orc_adjoint_run.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                  _dp, C.POINTER(C.c_double)]
+
+
+def _objective(self, x_in, x_out):
+    _adjoint_bindings(self.L)
+    out = C.c_double()
+    _check(self.L.orc_objective(self.h, x_in, x_out, C.byref(out)))
+    return out.value
+
+
+def _adjoint_run(self, steps, window, x_in, x_out):
+    """Forward `steps` steps, J over the last `window` states, and dJ/dgamma (per cell), dJ/du_peak."""
+    _adjoint_bindings(self.L)
+    J, du = C.c_double(), C.c_double()
+    grad = np.zeros(self.cells)
+    _check(self.L.orc_adjoint_run(self.h, steps, window, x_in, x_out, C.byref(J), grad, C.byref(du)))
+    return J.value, grad.reshape(self.shape), du.value
+
+
+Oracle.objective = _objective
+Oracle.adjoint_run = _adjoint_run

@@ -184,6 +184,9 @@ public:
     virtual uint64_t time_step() const = 0;
     virtual void set_time_step(uint64_t t) = 0;
     virtual size_t cells() const = 0;
+    virtual double objective(int x_in, int x_out) const = 0;
+    virtual double adjoint_run(uint64_t steps, uint64_t window, int x_in, int x_out, double* grad_gamma,
+                               double* grad_u_peak) = 0;
 };
 
 template <int Q, class Real, class Store> class Oracle final : public OracleBase {
@@ -713,6 +716,239 @@ private:
         ++t_;
     }
 
+
+public:
+    // ------------------------------------------------------------ objective and adjoint
+    // Pressure drop between the fluid cells of plane x_in and plane x_out:
+    //   j(f) = mean_{x = x_in} rho / 3 - mean_{x = x_out} rho / 3,   rho = sum_d f_d (logical).
+    double objective(int x_in, int x_out) const override {
+        std::vector<Real> fin(static_cast<size_t>(Q) * N_);
+        logical_state(fin.data());
+        return plane_objective(fin.data(), x_in, x_out);
+    }
+
+    // Runs `steps` forward steps from the current state and returns
+    //   J = (1/window) * sum_{t = steps-window+1}^{steps} j(f_t),
+    // with dJ/dgamma per cell and dJ/du_peak from the textbook discrete adjoint in logical space:
+    // reverse streaming (the transpose of the pull/bounce-back permutation) and the transposed
+    // collision Jacobian, evaluated in double precision on the stored forward states.  Supports
+    // unforced BGK with partial bounce-back, stationary walls, velocity inlet and pressure outlet.
+    double adjoint_run(uint64_t steps, uint64_t window, int x_in, int x_out, double* grad_gamma,
+                       double* grad_u_peak) override {
+        if (window < 1 || window > steps) throw std::invalid_argument("objective window must be in [1, steps]");
+        if (ax_ != Real(0) || ay_ != Real(0) || az_ != Real(0) || !lax_.empty())
+            throw std::invalid_argument("the adjoint supports unforced lattices only");
+        for (int k = 1; k < 8; ++k)
+            if (wall_u_[k][0] != Real(0) || wall_u_[k][1] != Real(0) || wall_u_[k][2] != Real(0))
+                if (!flags_.empty())
+                    for (uint8_t f : flags_)
+                        if (is_wall(f) && (f & 7) == k) throw std::invalid_argument("the adjoint supports walls at rest only");
+        const size_t QN = static_cast<size_t>(Q) * N_;
+        std::vector<std::vector<Real>> states(steps + 1, std::vector<Real>(QN));
+        logical_state(states[0].data());
+        double J = 0.0;
+        for (uint64_t t = 0; t < steps; ++t) {
+            step(1);
+            logical_state(states[t + 1].data());
+            if (t + 1 >= steps - window + 1) J += plane_objective(states[t + 1].data(), x_in, x_out);
+        }
+        J /= static_cast<double>(window);
+
+        // Adjoint of the objective's density sum on each plane.
+        size_t count_in = 0, count_out = 0;
+        for (size_t n = 0; n < N_; ++n) {
+            if (!flags_.empty() && flags_[n] != kFluid) continue;
+            const int x = static_cast<int>(n % nx_);
+            count_in += x == x_in;
+            count_out += x == x_out;
+        }
+        const double c_in = 1.0 / (3.0 * static_cast<double>(count_in) * static_cast<double>(window));
+        const double c_out = -1.0 / (3.0 * static_cast<double>(count_out) * static_cast<double>(window));
+        auto add_source = [&](std::vector<double>& lam) {
+            for (size_t n = 0; n < N_; ++n) {
+                if (!flags_.empty() && flags_[n] != kFluid) continue;
+                const int x = static_cast<int>(n % nx_);
+                const double c = x == x_in ? c_in : (x == x_out ? c_out : 0.0);
+                if (c != 0.0)
+                    for (int d = 0; d < Q; ++d) lam[static_cast<size_t>(d) * N_ + n] += c;
+            }
+        };
+        std::vector<double> lam(QN, 0.0), lam_out(QN, 0.0);
+        add_source(lam);  // lambda_T = dj(f_T)/df_T
+        std::fill(grad_gamma, grad_gamma + N_, 0.0);
+        double du = 0.0;
+        const double omega = 1.0 / static_cast<double>(tau_);
+        for (uint64_t t = steps; t-- > 0;) {
+            // Reverse streaming: f_{t+1}[d][n + c_d] = fout_t[d][n], or f_{t+1}[opp d][n] = fout_t[d][n]
+            // when n + c_d is a wall (halfway bounce-back).  Wall cells keep their populations.
+            for (size_t n = 0; n < N_; ++n) {
+                size_t x, y, z;
+                coords(n, x, y, z);
+                const bool wall = !flags_.empty() && is_wall(flags_[n]);
+                for (int d = 0; d < Q; ++d) {
+                    const size_t k = static_cast<size_t>(d) * N_ + n;
+                    if (wall) {
+                        lam_out[k] = lam[k];
+                        continue;
+                    }
+                    const size_t target = neighbor(x, y, z, d);
+                    if (!flags_.empty() && is_wall(flags_[target])) lam_out[k] = lam[static_cast<size_t>(opposite(d)) * N_ + n];
+                    else lam_out[k] = lam[static_cast<size_t>(d) * N_ + target];
+                }
+            }
+            // Transposed collision Jacobian at every non-wall cell.
+            const std::vector<Real>& fs = states[t];
+            for (size_t n = 0; n < N_; ++n) {
+                const uint8_t flag = flags_.empty() ? kFluid : flags_[n];
+                if (is_wall(flag)) {
+                    for (int d = 0; d < Q; ++d) lam[static_cast<size_t>(d) * N_ + n] = lam_out[static_cast<size_t>(d) * N_ + n];
+                    continue;
+                }
+                size_t x, y, z;
+                coords(n, x, y, z);
+                double f[Q], lo[Q], mu[Q];
+                for (int d = 0; d < Q; ++d) {
+                    f[d] = static_cast<double>(fs[static_cast<size_t>(d) * N_ + n]);
+                    lo[d] = lam_out[static_cast<size_t>(d) * N_ + n];
+                }
+                double dgamma = 0.0, dpeak = 0.0;
+                collision_vjp(flag, n, y, z, f, lo, omega, mu, dgamma, dpeak);
+                for (int d = 0; d < Q; ++d) lam[static_cast<size_t>(d) * N_ + n] = mu[d];
+                grad_gamma[n] += dgamma;
+                du += dpeak;
+            }
+            if (t >= steps - window + 1) add_source(lam);  // dj(f_t)/df_t for t in the window
+        }
+        if (grad_u_peak) *grad_u_peak = du;
+        return J;
+    }
+
+private:
+    void logical_state(Real* out) const {
+        for (size_t n = 0; n < N_; ++n) {
+            Real fin[Q];
+            load(n, fin);
+            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = fin[d];
+        }
+    }
+
+    double plane_objective(const Real* fin, int x_in, int x_out) const {
+        double s_in = 0.0, s_out = 0.0;
+        size_t c_in = 0, c_out = 0;
+        for (size_t n = 0; n < N_; ++n) {
+            if (!flags_.empty() && flags_[n] != kFluid) continue;
+            const int x = static_cast<int>(n % nx_);
+            if (x != x_in && x != x_out) continue;
+            Real rho = Real(0);
+            for (int d = 0; d < Q; ++d) rho += fin[static_cast<size_t>(d) * N_ + n];
+            if (x == x_in) {
+                s_in += static_cast<double>(rho);
+                ++c_in;
+            } else {
+                s_out += static_cast<double>(rho);
+                ++c_out;
+            }
+        }
+        if (c_in == 0 || c_out == 0) throw std::invalid_argument("objective planes contain no fluid cells");
+        return s_in / (3.0 * static_cast<double>(c_in)) - s_out / (3.0 * static_cast<double>(c_out));
+    }
+
+    // d feq_d / d(rho, u) for the second-order equilibrium, in double.
+    static void feq_grad(int d, double rho, double ux, double uy, double uz, double& e, double (&gu)[3]) {
+        const double cx = L::cx[d], cy = L::cy[d], cz = L::cz[d], w = static_cast<double>(L::w[d]);
+        const double cu = cx * ux + cy * uy + cz * uz;
+        const double uu = ux * ux + uy * uy + uz * uz;
+        e = w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+        gu[0] = w * rho * (3.0 * cx + 9.0 * cu * cx - 3.0 * ux);
+        gu[1] = w * rho * (3.0 * cy + 9.0 * cu * cy - 3.0 * uy);
+        gu[2] = w * rho * (3.0 * cz + 9.0 * cu * cz - 3.0 * uz);
+    }
+
+    // mu = (d fout / d fin)^T lambda_out for one cell, plus d/dgamma and d/du_peak.
+    void collision_vjp(uint8_t flag, size_t n, size_t y, size_t z, const double* f, const double* lo, double omega,
+                       double* mu, double& dgamma, double& dpeak) const {
+        if (flag == kInlet) {
+            // u_in = u_peak * P(y, z) with P the parabolic profile of inlet_velocity().
+            const double eta = (static_cast<double>(y) - 0.5) / static_cast<double>(ny_ - 2);
+            double P = 4.0 * eta * (1.0 - eta);
+            if (L::dims == 3) {
+                const double zeta = (static_cast<double>(z) - 0.5) / static_cast<double>(nz_ - 2);
+                P *= 4.0 * zeta * (1.0 - zeta);
+            }
+            const double uin = static_cast<double>(inlet_velocity(y, z));
+            double s0 = 0.0, sn = 0.0;
+            for (int d = 0; d < Q; ++d) {
+                if (L::cx[d] == 0) s0 += f[d];
+                else if (L::cx[d] < 0) sn += f[d];
+            }
+            const double rho = (s0 + 2.0 * sn) / (1.0 - uin);
+            double g_rho = 0.0, g_ux = 0.0;
+            for (int d = 0; d < Q; ++d) {
+                double e, gu[3];
+                feq_grad(d, rho, uin, 0.0, 0.0, e, gu);
+                g_rho += lo[d] * e;
+                g_ux += lo[d] * gu[0];
+            }
+            for (int d = 0; d < Q; ++d) {
+                const double coef = L::cx[d] == 0 ? 1.0 : (L::cx[d] < 0 ? 2.0 : 0.0);
+                mu[d] = g_rho * coef / (1.0 - uin);
+            }
+            // rho depends on u_in through 1 / (1 - u_in); u_in = u_peak * P.
+            dpeak = (g_ux + g_rho * rho / (1.0 - uin)) * P;
+            dgamma = 0.0;
+            return;
+        }
+        if (flag == kOutlet) {
+            const double rho = static_cast<double>(rho_out_);
+            double s0 = 0.0, sp = 0.0;
+            for (int d = 0; d < Q; ++d) {
+                if (L::cx[d] == 0) s0 += f[d];
+                else if (L::cx[d] > 0) sp += f[d];
+            }
+            const double ux = -1.0 + (s0 + 2.0 * sp) / rho;
+            double g_ux = 0.0;
+            for (int d = 0; d < Q; ++d) {
+                double e, gu[3];
+                feq_grad(d, rho, ux, 0.0, 0.0, e, gu);
+                g_ux += lo[d] * gu[0];
+            }
+            for (int d = 0; d < Q; ++d) {
+                const double coef = L::cx[d] == 0 ? 1.0 : (L::cx[d] > 0 ? 2.0 : 0.0);
+                mu[d] = g_ux * coef / rho;
+            }
+            dgamma = dpeak = 0.0;
+            return;
+        }
+        double rho = 0.0, m[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < Q; ++d) {
+            rho += f[d];
+            m[0] += f[d] * L::cx[d];
+            m[1] += f[d] * L::cy[d];
+            m[2] += f[d] * L::cz[d];
+        }
+        const double u[3] = {m[0] / rho, m[1] / rho, m[2] / rho};
+        const double g = gamma_.empty() ? 0.0 : static_cast<double>(gamma_[n]);
+        double a[Q], fstar[Q];
+        double g_rho = 0.0, G[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < Q; ++d) {
+            double e, gu[3];
+            feq_grad(d, rho, u[0], u[1], u[2], e, gu);
+            fstar[d] = f[d] - omega * (f[d] - e * rho);
+            a[d] = (1.0 - g) * lo[d];
+            g_rho += omega * a[d] * e;
+            for (int c = 0; c < 3; ++c) G[c] += omega * a[d] * gu[c] / rho;
+        }
+        const double Gu = G[0] * u[0] + G[1] * u[1] + G[2] * u[2];
+        dgamma = 0.0;
+        for (int k = 0; k < Q; ++k) {
+            const double Gc = G[0] * L::cx[k] + G[1] * L::cy[k] + G[2] * L::cz[k];
+            mu[k] = (1.0 - omega) * a[k] + g * lo[opposite(k)] + g_rho + Gc - Gu;
+            dgamma += lo[k] * (f[opposite(k)] - fstar[k]);
+        }
+        dpeak = 0.0;
+    }
+
+private:
     // Macroscopic state reported per cell (esoteric_pull.hpp:180-209): fluid u = m/rho + a/2,
     // wall cells rho = sum of frozen populations and u = 0, inlet/outlet cells report the state
     // their boundary rule imposes.
@@ -833,6 +1069,13 @@ ORC_API int orc_bb_load_map(void* h, int p, int64_t* out) { return guarded([&] {
 ORC_API uint64_t orc_time_step(void* h) { return H(h)->time_step(); }
 ORC_API void orc_set_time_step(void* h, uint64_t t) { H(h)->set_time_step(t); }
 ORC_API size_t orc_cells(void* h) { return H(h)->cells(); }
+ORC_API int orc_objective(void* h, int x_in, int x_out, double* out) {
+    return guarded([&] { *out = H(h)->objective(x_in, x_out); });
+}
+ORC_API int orc_adjoint_run(void* h, uint64_t steps, uint64_t window, int x_in, int x_out, double* J,
+                            double* grad_gamma, double* grad_u_peak) {
+    return guarded([&] { *J = H(h)->adjoint_run(steps, window, x_in, x_out, grad_gamma, grad_u_peak); });
+}
 ORC_API uint16_t orc_f32_to_f16(float f) { return orc::f32_to_f16_rne(f); }
 ORC_API float orc_f16_to_f32(uint16_t h) { return orc::f16_to_f32(h); }
 ORC_API double orc_uniform01(uint64_t seed, uint64_t cell, uint64_t comp) {
