@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "D3Q19 MLUPS fwd FP32/FP16-store + adjoint, % of 8 TB/s HBM roofline"
 PEAK_GBS = 8000.0
 BYTES_PER_POP = {"fp32": 4, "fp16s": 2, "fp64": 8}
+ADJ_BYTES = 3 * 19 * 4 + 1 + 4 + 16
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_stream_collide_d3q19_256_cavity.json"
 
 
@@ -81,6 +82,63 @@ def run_leg(hip, name, lattice, storage, n, tau, steps, warmup):
     log(f"leg {name}: {res['mlups']:.0f} MLUPS, kernel {res['kernel_ms']:.3f} ms, {gbs:.0f} GB/s "
         f"({100 * gbs / PEAK_GBS:.1f}% of 8 TB/s), mass drift {res['mass_rel_drift']:.2e}")
     s.close()
+    return res
+
+
+def duct_flags(n):
+    f = np.zeros((n, n, n), dtype=np.uint8)
+    f[:, 0, :] = f[:, -1, :] = 0x80
+    f[0, :, :] = f[-1, :, :] = 0x80
+    f[1:-1, 1:-1, 0] = 0x01
+    f[1:-1, 1:-1, -1] = 0x02
+    return f.reshape(-1)
+
+
+def adjoint_leg(hip, n=128, steps=2000, window=200, every=100, kernel_steps=100):
+    """Config-4 workload: D3Q19 n^3 duct, velocity inlet (peak 0.05), pressure outlet, walls in y/z,
+    partial bounce-back blend 0.01 * U[0.3, 0.7] (seed 4); J = mean pressure drop over the last
+    `window` of `steps` states; snapshots every `every` steps.  A second, short run with a snapshot
+    every step (no recomputation) times the transposed-step kernel alone."""
+    log(f"leg adjoint: D3Q19 {n}^3 duct, T {steps}, window {window}, snapshots every {every}")
+    flags = duct_flags(n)
+    rng = np.random.default_rng(4)
+    blend = np.where(flags == 0, 0.01 * rng.uniform(0.3, 0.7, flags.size), 0.0).astype(np.float32)
+
+    def solver():
+        s = hip.EsotericPullHipSolver("d3q19", n, n, n, 0.6, storage="fp32")
+        s.set_flags(flags)
+        s.set_inlet_outlet(0.05, 1.0)
+        s.set_porosity(blend)
+        s.initialize_uniform()
+        return s
+
+    s = solver()
+    t0 = time.perf_counter()
+    J, grad, du = s.adjoint_run(steps, window, every, 1, n - 2)
+    wall = time.perf_counter() - t0
+    fwd_ms, rev_ms = s.adjoint_timing()
+    s.close()
+    s = solver()
+    s.adjoint_run(kernel_steps, 1, 1, 1, n - 2, want_gamma=False)
+    _, kern_ms = s.adjoint_timing()
+    s.close()
+    cells = n ** 3
+    fluid = int(np.count_nonzero(flags == 0)) + int(np.count_nonzero((flags == 1) | (flags == 2)))
+    adj_step_ms = kern_ms / kernel_steps
+    adj_bytes = fluid * ADJ_BYTES  # lambda read + write, f read, flag, gamma, FP64 gradient read + write
+    res = {
+        "workload": f"D3Q19 {n}^3 duct, inlet peak 0.05, outlet rho 1, PBB blend 0.01*U[0.3,0.7], FP32",
+        "steps": steps, "window": window, "snapshot_every": every, "objective": J, "du_peak": du,
+        "wall_seconds": wall, "forward_ms": fwd_ms, "reverse_ms": rev_ms,
+        "reverse_over_forward": rev_ms / fwd_ms, "total_over_forward": (fwd_ms + rev_ms) / fwd_ms,
+        "forward_mlups": cells * steps / (fwd_ms * 1e3), "reverse_mlups": cells * steps / (rev_ms * 1e3),
+        "adjoint_kernel_ms": adj_step_ms, "adjoint_kernel_mlups": cells / (adj_step_ms * 1e3),
+        "adjoint_bytes_per_cell_update": ADJ_BYTES, "adjoint_kernel_gbs": adj_bytes / (adj_step_ms * 1e-3) / 1e9,
+    }
+    res["adjoint_kernel_roofline_frac"] = res["adjoint_kernel_gbs"] / PEAK_GBS
+    log(f"leg adjoint: forward {fwd_ms:.1f} ms, reverse {rev_ms:.1f} ms ({rev_ms / fwd_ms:.2f}x), "
+        f"transposed step {adj_step_ms * 1e3:.1f} us = {res['adjoint_kernel_mlups']:.0f} MLUPS, "
+        f"{res['adjoint_kernel_gbs']:.0f} GB/s")
     return res
 
 
@@ -146,7 +204,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--legs", default="fp32,fp16,d3q27")
+    ap.add_argument("--legs", default="fp32,fp16,d3q27,adjoint")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
     args = ap.parse_args()
@@ -162,10 +220,13 @@ def main():
     table = {"fp32": ("d3q19", "fp32"), "fp16": ("d3q19", "fp16s"), "d3q27": ("d3q27", "fp32"),
              "fp64": ("d3q19", "fp64")}
     for name in wanted:
+        if name == "adjoint":
+            legs[name] = adjoint_leg(hip)
+            continue
         lattice, storage = table[name]
         legs[name] = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
     probe = None if args.no_probe else copy_probe()
-    head = legs.get("fp32") or next(iter(legs.values()))
+    head = legs.get("fp32") or next(v for k, v in legs.items() if k != "adjoint")
 
     traffic = None
     if TRAFFIC_FILE.exists() and args.n == 256:

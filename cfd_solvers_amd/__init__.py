@@ -5,8 +5,11 @@ in ``include/cfd_lbm/c_api.h``); this package is the ctypes host layer over it, 
 reference's ``python/cfd_solvers`` bindings.  Build the library with
 ``python -m cfd_solvers_amd.build``.
 """
-from .bindings import (EsotericPullHipSolver, MacroscopicFields, SolverLibrary, load, FLUID, VELOCITY_INLET,
-                       PRESSURE_OUTLET, WALL, LATTICES, STORAGE)
+from .bindings import (EsotericPullHipSolver, MacroscopicFields, SolverLibrary, load, checkpoint_plan, FLUID,
+                       VELOCITY_INLET, PRESSURE_OUTLET, WALL, LATTICES, STORAGE, OBJECTIVES)
 
-__all__ = ["EsotericPullHipSolver", "MacroscopicFields", "SolverLibrary", "load", "FLUID", "VELOCITY_INLET",
-           "PRESSURE_OUTLET", "WALL", "LATTICES", "STORAGE"]
+__all__ = ["EsotericPullHipSolver", "MacroscopicFields", "SolverLibrary", "load", "checkpoint_plan", "FLUID",
+           "VELOCITY_INLET", "PRESSURE_OUTLET", "WALL", "LATTICES", "STORAGE", "OBJECTIVES"]
+
+# torch.autograd.Function over the adjoint: `from cfd_solvers_amd.autograd import LbmObjective`
+# (imported on demand so that the ctypes layer does not need torch).

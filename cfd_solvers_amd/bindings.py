@@ -66,7 +66,16 @@ _SIGNATURES = {
     "cfd_lbm_adjoint_run": [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
                             C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_double)],
     "cfd_lbm_adjoint_timing": [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "cfd_lbm_set_drag_body": [C.c_void_p, C.c_void_p, C.c_int],
+    "cfd_lbm_objective_drag": [C.c_void_p, C.POINTER(C.c_double)],
+    "cfd_lbm_adjoint_run_binomial": [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                                     C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_uint64)],
+    "cfd_lbm_checkpoint_plan": [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                C.POINTER(C.c_uint64)],
 }
+
+OBJECTIVES = {"pressure_drop": 0, "drag": 1}
 
 
 class SolverLibrary:
@@ -300,6 +309,31 @@ class EsotericPullHipSolver:
                       grad.ctypes.data if grad is not None else None, C.byref(du))
         return J.value, (grad.reshape(self.shape) if grad is not None else None), du.value
 
+    def set_drag_body(self, body, axis=0):
+        """Registers the wall cells of the drag body (nonzero entries of `body`, shape cells); None clears."""
+        if body is None:
+            self.lib.call("cfd_lbm_set_drag_body", self._h, None, axis)
+            return
+        b = np.ascontiguousarray(np.asarray(body).reshape(-1) != 0, dtype=np.uint8)
+        if b.size != self.cells:
+            raise ValueError("drag body must have one entry per cell")
+        self.lib.call("cfd_lbm_set_drag_body", self._h, b.ctypes.data, axis)
+
+    def objective_drag(self):
+        v = C.c_double()
+        self.lib.call("cfd_lbm_objective_drag", self._h, C.byref(v))
+        return v.value
+
+    def adjoint_run_binomial(self, steps, window, snapshots=0, objective="pressure_drop", x_in=1, x_out=None,
+                             want_gamma=True):
+        """Binomial checkpointing: J, dJ/dgamma (shape (nz, ny, nx)), dJ/du_peak, forward steps executed."""
+        J, du, fw = C.c_double(), C.c_double(), C.c_uint64()
+        grad = np.zeros(self.cells) if want_gamma else None
+        x_out = self.shape[-1] - 2 if x_out is None else x_out
+        self.lib.call("cfd_lbm_adjoint_run_binomial", self._h, OBJECTIVES[objective], steps, window, snapshots, x_in,
+                      x_out, C.byref(J), grad.ctypes.data if grad is not None else None, C.byref(du), C.byref(fw))
+        return J.value, (grad.reshape(self.shape) if grad is not None else None), du.value, fw.value
+
     def adjoint_timing(self):
         fw, rv = C.c_float(), C.c_float()
         self.lib.call("cfd_lbm_adjoint_timing", self._h, C.byref(fw), C.byref(rv))
@@ -310,3 +344,12 @@ class EsotericPullHipSolver:
         store = np.empty(self.q * self.cells, dtype=np.int64)
         self.lib.call("cfd_lbm_index_map", self._h, parity, load, store)
         return load.reshape(self.q, self.cells), store.reshape(self.q, self.cells)
+
+
+def checkpoint_plan(steps, snapshots, library=None):
+    """Binomial plan for `steps` steps with `snapshots` free state buffers (host only):
+    (forward steps by the closed form, repetition number, forward steps by walking the plan)."""
+    lib = load(library)
+    fw, r, walked = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    lib.call("cfd_lbm_checkpoint_plan", steps, snapshots, C.byref(fw), C.byref(r), C.byref(walked))
+    return fw.value, r.value, walked.value

@@ -11,9 +11,11 @@
 //
 // Differentiated physics: unforced BGK with partial bounce-back gamma, stationary halfway walls,
 // the velocity inlet (parabolic, Zou-He density) and the pressure outlet (Zou-He velocity).
-// Gradients: dJ/dgamma per cell and dJ/du_peak, accumulated in FP64.  The reference has no adjoint;
-// this is checked against a logical-space adjoint (reverse streaming + transposed collision) and
-// central finite differences in the tests.
+// Gradients: dJ/dgamma per cell and dJ/du_peak, accumulated in FP64.  With FP16 shifted storage
+// the forward states are the decoded FP16 states and lambda stays FP32: the adjoint is the one of
+// the FP32 collision evaluated at the stored states (the rounding of the store is not
+// differentiated).  The reference has no adjoint; this is checked against a logical-space adjoint
+// (reverse streaming + transposed collision) and central finite differences in the tests.
 #include "lbm_adjoint.hpp"
 #include "lbm_cell.hpp"
 
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
     Real mu[Q];
     double dgamma, dpeak;
     collision_vjp<Q>(p, g, flag, n, y, z, f, lam, mu, dgamma, dpeak);
-    if (a.add_objective) {
+    if (a.add_objective && a.kind == kPressureDrop) {
         const Real c = plane_weight<Real>(a, x, flag == kFluid);
         if (c != Real(0)) static_for<Q>([&](auto DD) { mu[decltype(DD)::value] += c; });
     }
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(256) void adjoint_seed(void* __restrict__ lam_buf, 
 }
 
 template <int Q, class T, bool Odd, bool Bounded>
-__global__ __launch_bounds__(256) void adjoint_seed_source(void* __restrict__ lam_buf, const Geometry g,
+__global__ __launch_bounds__(256) void plane_source(void* __restrict__ lam_buf, const Geometry g,
                                                            const Physics<typename Storage<T>::Real> p,
                                                            const AdjointArgs a) {
     using L = Lattice<Q>;
@@ -241,6 +243,79 @@ __global__ __launch_bounds__(256) void adjoint_seed_source(void* __restrict__ la
         lam[static_cast<uint64_t>(dir_a<Odd>(i, wup)) * g.stride + up] += c;
         lam[static_cast<uint64_t>(dir_b<Odd>(i, wdn)) * g.stride + n] += c;
     });
+}
+
+// Drag source: lam at the slots the listed cells read at this parity += c_drag * (-2 c_d[axis]) for
+// every population d that arrives from the body.  Each slot belongs to one cell: no races.
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void drag_source(void* __restrict__ lam_buf, const Geometry g, const AdjointArgs a) {
+    using L = Lattice<Q>;
+    using Real = typename Storage<T>::Real;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.drag_count) return;
+    const uint32_t n = a.drag_cells[k];
+    const uint32_t bits = a.drag_links[k];
+    const uint32_t x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+    const Neighbourhood h = neighbourhood(g, x, y, z);
+    Real* lam = static_cast<Real*>(lam_buf);
+    const Real c = static_cast<Real>(a.c_drag);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const Real ci = static_cast<Real>(a.drag_axis == 0 ? L::cx[i] : (a.drag_axis == 1 ? L::cy[i] : L::cz[i]));
+        if ((bits >> (i - 1)) & 1U) {
+            const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
+            lam[static_cast<uint64_t>(dir_a<Odd>(i, true)) * g.stride + up] += Real(-2) * ci * c;
+        }
+        if ((bits >> i) & 1U)  // population i+1 has c = -c_i
+            lam[static_cast<uint64_t>(dir_b<Odd>(i, true)) * g.stride + n] += Real(2) * ci * c;
+    });
+}
+
+// Per-block partial sums of the drag over the listed cells (FP64, fixed tree order).
+template <int Q, class T, bool Odd, bool Bounded>
+__global__ __launch_bounds__(256) void drag_partial(const void* __restrict__ f, const Geometry g,
+                                                    const Physics<typename Storage<T>::Real> p, const AdjointArgs a) {
+    using L = Lattice<Q>;
+    using Real = typename Storage<T>::Real;
+    __shared__ double sm[256];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (k < a.drag_count) {
+        const uint32_t n = a.drag_cells[k];
+        const uint32_t bits = a.drag_links[k];
+        const uint32_t x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+        const Neighbourhood h = neighbourhood(g, x, y, z);
+        Real fin[Q];
+        PairSlots<Q> s;
+        load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+        static_for<Q - 1>([&](auto DD) {
+            constexpr int d = decltype(DD)::value + 1;
+            const int cd = a.drag_axis == 0 ? L::cx[d] : (a.drag_axis == 1 ? L::cy[d] : L::cz[d]);
+            if ((bits >> (d - 1)) & 1U) v += -2.0 * static_cast<double>(cd) * static_cast<double>(fin[d]);
+        });
+    }
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.scratch[blockIdx.x] = sm[0];
+}
+
+// acc += weight * sum(partial[0..count)), one block in a fixed order.
+__global__ __launch_bounds__(1024) void sum_partials(const double* __restrict__ partial, uint32_t count, double weight,
+                                                     double* acc) {
+    __shared__ double sm[1024];
+    double v = 0.0;
+    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) v += partial[k];
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *acc += weight * sm[0];
 }
 
 // j(f) = sum_{x = x_in} rho * c_in + sum_{x = x_out} rho * c_out over fluid cells, in FP64, by one
@@ -293,6 +368,19 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
     static dim3 block(const Geometry& g) { return dim3(g.bx, g.by, 1); }
     static dim3 grid(const Geometry& g) { return dim3(g.tiles, 1, 1); }
 
+    static unsigned list_blocks(const AdjointArgs& a) { return (a.drag_count + 255U) / 256U; }
+
+    // The objective's dj/df source added onto lam at parity odd (the drag part; the pressure-drop
+    // part of adjoint_step is fused into the transposed step itself).
+    void drag(const LaunchState& st, bool odd, bool bounded, void* lam, const AdjointArgs& a) const {
+        if (a.drag_count == 0) return;
+        per_parity(odd, bounded, [&](auto O, auto B) {
+            hipLaunchKernelGGL((drag_source<Q, T, decltype(O)::value, decltype(B)::value>), dim3(list_blocks(a)),
+                               dim3(256), 0, st.stream, lam, st.geo, a);
+        });
+        check(hipGetLastError(), "drag_source launch");
+    }
+
     void step(const LaunchState& st, bool odd, bool bounded, const void* fstate, void* lam,
               const AdjointArgs& a) const override {
         per_parity(odd, bounded, [&](auto O, auto B) {
@@ -301,20 +389,34 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
                                *static_cast<const Physics<Real>*>(st.physics), a);
         });
         check(hipGetLastError(), "adjoint_step launch");
+        if (a.add_objective && a.kind == kDrag) drag(st, odd, bounded, lam, a);
     }
     void seed(const LaunchState& st, bool odd, bool bounded, void* lam, const AdjointArgs& a) const override {
         per_parity(odd, bounded, [&](auto O, auto B) {
             hipLaunchKernelGGL((adjoint_seed<Q, T, decltype(O)::value, decltype(B)::value>), grid(st.geo),
                                block(st.geo), 0, st.stream, lam, st.geo,
                                *static_cast<const Physics<Real>*>(st.physics), a);
-            hipLaunchKernelGGL((adjoint_seed_source<Q, T, decltype(O)::value, decltype(B)::value>), grid(st.geo),
-                               block(st.geo), 0, st.stream, lam, st.geo,
-                               *static_cast<const Physics<Real>*>(st.physics), a);
+            if (a.kind == kPressureDrop)
+                hipLaunchKernelGGL((plane_source<Q, T, decltype(O)::value, decltype(B)::value>), grid(st.geo),
+                                   block(st.geo), 0, st.stream, lam, st.geo,
+                                   *static_cast<const Physics<Real>*>(st.physics), a);
         });
         check(hipGetLastError(), "adjoint_seed launch");
+        if (a.kind == kDrag) drag(st, odd, bounded, lam, a);
     }
     void objective(const LaunchState& st, bool odd, bool bounded, const void* f, const AdjointArgs& a, double weight,
                    double* acc) const override {
+        if (a.kind == kDrag) {
+            if (a.drag_count == 0) return;
+            per_parity(odd, bounded, [&](auto O, auto B) {
+                hipLaunchKernelGGL((drag_partial<Q, T, decltype(O)::value, decltype(B)::value>), dim3(list_blocks(a)),
+                                   dim3(256), 0, st.stream, f, st.geo, *static_cast<const Physics<Real>*>(st.physics),
+                                   a);
+            });
+            hipLaunchKernelGGL(sum_partials, dim3(1), dim3(1024), 0, st.stream, a.scratch, list_blocks(a), weight, acc);
+            check(hipGetLastError(), "drag objective launch");
+            return;
+        }
         per_parity(odd, bounded, [&](auto O, auto B) {
             hipLaunchKernelGGL((plane_objective<Q, T, decltype(O)::value, decltype(B)::value>), dim3(1), dim3(1024),
                                0, st.stream, f, st.geo, *static_cast<const Physics<Real>*>(st.physics), a, weight,
@@ -331,10 +433,14 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
 
 template <int Q> const AdjointOps* adj_for_storage(int storage) {
     static const AdjOps<Q, float> f32;
+    static const AdjOps<Q, _Float16> f16;
     static const AdjOps<Q, double> f64;
-    if (storage == 0) return &f32;
-    if (storage == 2) return &f64;
-    throw std::invalid_argument("the adjoint runs on FP32 or FP64 population storage");
+    switch (storage) {
+    case 0: return &f32;
+    case 1: return &f16;
+    case 2: return &f64;
+    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1) or FP64 (2)");
+    }
 }
 
 }  // namespace

@@ -6,6 +6,7 @@
 // ledger (include/cfd/core/device_residency.hpp:11-42).  Error convention of
 // src/python/c_api.cpp:8-15: 0 ok, 1 std::exception, 2 unknown, message in a caller buffer.
 #include "../../include/cfd_lbm/c_api.h"
+#include "checkpoint_schedule.hpp"
 #include "lbm_adjoint.hpp"
 #include "lbm_kernels.hpp"
 
@@ -16,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -75,6 +77,7 @@ struct cfd_lbm_solver {
     uint32_t step_lds = 0;
     std::vector<uint8_t> host_flags;  // copy of the flags (empty: periodic)
     float last_forward_ms = 0.0F, last_reverse_ms = 0.0F;
+    uint64_t last_forward_steps = 0;
     mutable TransferStats stats;
     // Host copy of the Physics<Real> passed by value to every launch.
     alignas(16) unsigned char physics[sizeof(Physics<double>)] = {};
@@ -356,6 +359,7 @@ struct cfd_lbm_solver {
         }
         if (host_flags) host_flags_assign(host_flags);
         else this->host_flags.clear();
+        clear_drag_body();  // the body's fluid neighbours depend on the flags
         LaunchState st = state();
         if (bounded()) {
             ops->bounce_back(st, odd(), false, flags.ptr);  // back to the plain layout
@@ -385,8 +389,8 @@ struct cfd_lbm_solver {
 
     void host_flags_assign(const uint8_t* v) { this->host_flags.assign(v, v + cells); }
 
-    // Pressure drop between fluid cells of planes x_in and x_out at the current state.
-    cfd_hip::AdjointArgs objective_args(int x_in, int x_out, uint64_t window) const {
+    // Pressure drop between fluid cells of planes x_in and x_out, as a mean over `window` states.
+    cfd_hip::AdjointArgs pressure_args(int x_in, int x_out, uint64_t window) const {
         if (x_in < 0 || x_out < 0 || static_cast<size_t>(x_in) >= nx || static_cast<size_t>(x_out) >= nx || x_in == x_out)
             throw std::invalid_argument("objective planes must be two different x indices inside the lattice");
         size_t a = 0, b = 0;
@@ -400,6 +404,7 @@ struct cfd_lbm_solver {
             }
         if (a == 0 || b == 0) throw std::invalid_argument("objective planes contain no fluid cells");
         cfd_hip::AdjointArgs args{};
+        args.kind = cfd_hip::kPressureDrop;
         args.x_in = x_in;
         args.x_out = x_out;
         args.c_in = 1.0 / (3.0 * static_cast<double>(a) * static_cast<double>(window));
@@ -407,52 +412,153 @@ struct cfd_lbm_solver {
         return args;
     }
 
-    double objective(int x_in, int x_out) {
+    // Drag objective: the fluid cells next to the registered body and, per cell, the populations
+    // that arrive from a body cell (bit d-1 for population d).
+    DeviceBuffer<unsigned char> drag_cells, drag_links, drag_scratch;
+    uint32_t drag_count = 0;
+    int drag_axis = -1;  // -1: no body registered
+
+    void clear_drag_body() {
+        drag_cells.reset();
+        drag_links.reset();
+        drag_scratch.reset();
+        drag_count = 0;
+        drag_axis = -1;
+    }
+
+    template <int Q> void collect_drag_links(const uint8_t* body, std::vector<uint64_t>& pairs) const {
+        using L = cfd_hip::Lattice<Q>;
+        const auto wrap = [](size_t v, int c, size_t n) { return (v + n + static_cast<size_t>(c + 0)) % n; };
+        for (size_t cell = 0; cell < cells; ++cell) {
+            if (!body[cell]) continue;
+            if ((host_flags[cell] & 0x80U) == 0U) throw std::invalid_argument("drag body cells must be wall cells");
+            const size_t x = cell % nx, y = (cell / nx) % ny, z = cell / (nx * ny);
+            for (int d = 1; d < Q; ++d) {
+                // Population d arrives at n = body + c_d from the body cell.
+                const size_t xn = wrap(x, L::cx[d], nx), yn = wrap(y, L::cy[d], ny), zn = wrap(z, L::cz[d], nz);
+                const size_t n = (zn * ny + yn) * nx + xn;
+                if (host_flags[n] != 0) continue;  // fluid neighbours only
+                pairs.push_back((static_cast<uint64_t>(n) << 8) | static_cast<uint64_t>(d));
+            }
+        }
+    }
+
+    void set_drag_body(const uint8_t* body, int axis) {
         use_device();
-        const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage == 1 ? 0 : storage);
+        if (!body) {
+            sync();
+            clear_drag_body();
+            return;
+        }
+        if (axis < 0 || axis > 2 || (q == 9 && axis == 2)) throw std::invalid_argument("drag axis must be 0 (x), 1 (y) or 2 (z, 3-D)");
+        if (host_flags.empty()) throw std::invalid_argument("the drag objective needs wall flags (set_flags first)");
+        std::vector<uint64_t> pairs;
+        if (q == 9) collect_drag_links<9>(body, pairs);
+        else if (q == 19) collect_drag_links<19>(body, pairs);
+        else collect_drag_links<27>(body, pairs);
+        std::sort(pairs.begin(), pairs.end());
+        std::vector<uint32_t> list, bits;
+        for (uint64_t pr : pairs) {
+            const uint32_t n = static_cast<uint32_t>(pr >> 8);
+            const int d = static_cast<int>(pr & 0xFFU);
+            if (list.empty() || list.back() != n) {
+                list.push_back(n);
+                bits.push_back(0U);
+            }
+            bits.back() |= 1U << (d - 1);
+        }
+        if (list.empty()) throw std::invalid_argument("the drag body has no fluid neighbours");
+        sync();
+        clear_drag_body();
+        drag_cells.allocate(list.size() * 4);
+        drag_links.allocate(bits.size() * 4);
+        drag_scratch.allocate((list.size() + 255) / 256 * sizeof(double));
+        h2d(drag_cells.ptr, list.data(), list.size() * 4);
+        h2d(drag_links.ptr, bits.data(), bits.size() * 4);
+        sync();
+        drag_count = static_cast<uint32_t>(list.size());
+        drag_axis = axis;
+    }
+
+    cfd_hip::AdjointArgs drag_args(uint64_t window) const {
+        if (drag_axis < 0) throw std::invalid_argument("no drag body registered (set_drag_body first)");
+        cfd_hip::AdjointArgs args{};
+        args.kind = cfd_hip::kDrag;
+        args.drag_cells = reinterpret_cast<const uint32_t*>(drag_cells.ptr);
+        args.drag_links = reinterpret_cast<const uint32_t*>(drag_links.ptr);
+        args.drag_count = drag_count;
+        args.drag_axis = drag_axis;
+        args.c_drag = 1.0 / static_cast<double>(window);
+        args.scratch = reinterpret_cast<double*>(drag_scratch.ptr);
+        return args;
+    }
+
+    cfd_hip::AdjointArgs objective_args(int kind, int x_in, int x_out, uint64_t window) const {
+        if (kind == cfd_hip::kPressureDrop) return pressure_args(x_in, x_out, window);
+        if (kind == cfd_hip::kDrag) return drag_args(window);
+        throw std::invalid_argument("objective must be 0 (pressure drop) or 1 (drag)");
+    }
+
+    // j of the current state (FP64).
+    double objective(int kind, int x_in, int x_out) {
+        use_device();
+        const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage);
+        const cfd_hip::AdjointArgs args = objective_args(kind, x_in, x_out, 1);
         DeviceBuffer<unsigned char> acc;
         acc.allocate(sizeof(double));
         const LaunchState st = state();
         adj->zero(st, reinterpret_cast<double*>(acc.ptr), 1);
-        if (storage == 1) throw std::invalid_argument("the objective runs on FP32 or FP64 population storage");
-        adj->objective(st, odd(), bounded(), f.ptr, objective_args(x_in, x_out, 1), 1.0, reinterpret_cast<double*>(acc.ptr));
+        adj->objective(st, odd(), bounded(), f.ptr, args, 1.0, reinterpret_cast<double*>(acc.ptr));
         double v = 0.0;
         d2h(&v, acc.ptr, sizeof(double));
         sync();
         return v;
     }
 
-    // Forward `steps` steps from the current state, J = mean of j over the last `window` states,
-    // then the discrete adjoint back to the current state with two-level checkpointing: a snapshot
-    // every `every` steps on the way forward, and per segment an out-of-place recomputation of its
-    // states into a replay buffer before the transposed steps run through it.
-    double adjoint_run(uint64_t steps, uint64_t window, uint64_t every, int x_in, int x_out, double* grad_gamma_out,
-                       double* grad_peak_out) {
-        use_device();
-        if (storage == 1) throw std::invalid_argument("the adjoint runs on FP32 or FP64 population storage");
+    void adjoint_checks(uint64_t steps, uint64_t window) const {
         if (steps < 1 || window < 1 || window > steps) throw std::invalid_argument("need 1 <= window <= steps");
-        if (every < 1) throw std::invalid_argument("checkpoint interval must be >= 1");
         if (ax != 0.0F || ay != 0.0F || az != 0.0F || lacc) throw std::invalid_argument("the adjoint supports unforced lattices only");
         for (uint8_t v : host_flags)
             if ((v & 0x80U) && (v & 7U) && (wall_u[v & 7U][0] != 0.0F || wall_u[v & 7U][1] != 0.0F || wall_u[v & 7U][2] != 0.0F))
                 throw std::invalid_argument("the adjoint supports walls at rest only");
+    }
+
+    // Forward `steps` steps from the current state (J = mean of j over the last `window` states),
+    // then the discrete adjoint back to the current state.  Checkpointing:
+    //  * every > 0: two-level -- a snapshot every `every` steps on the way forward, and per segment
+    //    an out-of-place recomputation of its states into replay buffers before the transposed
+    //    steps run through them (each step recomputed once; memory steps/every + every - 1 states);
+    //  * every == 0: binomial -- `snapshots` state buffers (0: as many as fit, keeping 1/16 of the
+    //    free memory in reserve) plus the start state, optimal recomputation (checkpoint_schedule.hpp).
+    double adjoint_run(int kind, uint64_t steps, uint64_t window, uint64_t every, uint64_t snapshots, int x_in,
+                       int x_out, double* grad_gamma_out, double* grad_peak_out, uint64_t* forward_steps_out) {
+        use_device();
+        adjoint_checks(steps, window);
         const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage);
-        const uint64_t segments = (steps + every - 1) / every;
+        cfd_hip::AdjointArgs args = objective_args(kind, x_in, x_out, window);
         const size_t state_bytes = static_cast<size_t>(q) * geo.stride * elem;
-        const size_t replay_states = static_cast<size_t>(std::min<uint64_t>(every, steps) - 1);
+        const size_t fixed = static_cast<size_t>(q) * geo.stride * real + 2 * cells * 8 + 64;
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-        const size_t need = (segments + replay_states) * state_bytes + state_bytes / elem * real + 2 * cells * 8 + 64;
-        if (need > free_b)
-            throw std::invalid_argument("checkpoints and replay buffer do not fit in device memory (" +
-                                        std::to_string(need >> 20) + " MiB needed, " + std::to_string(free_b >> 20) +
-                                        " MiB free): raise the checkpoint interval");
-        std::vector<std::unique_ptr<DeviceBuffer<unsigned char>>> snaps(segments), replay(replay_states);
-        for (auto& b : snaps) {
-            b = std::make_unique<DeviceBuffer<unsigned char>>();
-            b->allocate(state_bytes);
+        size_t buffers = 0;  // state buffers besides f
+        if (every > 0) {
+            buffers = static_cast<size_t>((steps + every - 1) / every + std::min<uint64_t>(every, steps) - 1);
+        } else {
+            if (snapshots == 0) {
+                const size_t usable = free_b - free_b / 16;
+                snapshots = usable > fixed + 2 * state_bytes ? (usable - fixed) / state_bytes - 1 : 0;
+                if (snapshots == 0) throw std::invalid_argument("no room for a checkpoint buffer in device memory");
+            }
+            snapshots = std::min<uint64_t>(snapshots, std::max<uint64_t>(steps - 1, 1));
+            buffers = static_cast<size_t>(snapshots) + 1;
         }
-        for (auto& b : replay) {
+        const size_t need = buffers * state_bytes + fixed;
+        if (need > free_b)
+            throw std::invalid_argument("checkpoints do not fit in device memory (" + std::to_string(need >> 20) +
+                                        " MiB needed, " + std::to_string(free_b >> 20) +
+                                        " MiB free): raise the checkpoint interval or lower the snapshot count");
+        std::vector<std::unique_ptr<DeviceBuffer<unsigned char>>> pool(buffers);
+        for (auto& b : pool) {
             b = std::make_unique<DeviceBuffer<unsigned char>>();
             b->allocate(state_bytes);
         }
@@ -461,7 +567,6 @@ struct cfd_lbm_solver {
         gg.allocate(cells * sizeof(double));
         gp.allocate(cells * sizeof(double));
         acc.allocate(sizeof(double));
-        cfd_hip::AdjointArgs args = objective_args(x_in, x_out, window);
         args.grad_gamma = reinterpret_cast<double*>(gg.ptr);
         args.grad_peak = reinterpret_cast<double*>(gp.ptr);
         const LaunchState st = state();
@@ -470,36 +575,87 @@ struct cfd_lbm_solver {
         adj->zero(st, reinterpret_cast<double*>(acc.ptr), 1);
         const bool b = bounded();
         const uint64_t t0 = t;
+        const auto par = [t0](uint64_t k) { return ((t0 + k) & 1U) != 0U; };
+        uint64_t forward_steps = 0;
+        // Snapshots taken on the way forward: (step index, buffer).
+        std::vector<std::pair<uint64_t, void*>> shots;
+        std::vector<void*> free_list;
+        if (every > 0) {
+            for (uint64_t c = 0; c * every < steps; ++c) shots.emplace_back(c * every, pool[c]->ptr);
+        } else {
+            shots.emplace_back(0, pool[0]->ptr);
+            for (size_t k = 1; k < buffers; ++k) free_list.push_back(pool[k]->ptr);
+            // The first descent of the binomial plan: its advances are the forward sweep itself.
+            std::vector<void*> fl = free_list;
+            for (uint64_t a0 = 0; steps - a0 > 1;) {
+                const uint64_t m = a0 + cfd_hip::BinomialSchedule::split(steps - a0, fl.size());
+                shots.emplace_back(m, fl.back());
+                fl.pop_back();
+                a0 = m;
+            }
+        }
         hipEvent_t e0, e1, e2;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
         hip_check(hipEventCreate(&e2), "hipEventCreate");
         hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        size_t next = 0;
         for (uint64_t k = 0; k < steps; ++k) {
-            if (k % every == 0)
-                hip_check(hipMemcpyAsync(snaps[k / every]->ptr, f.ptr, state_bytes, hipMemcpyDeviceToDevice, stream),
+            if (next < shots.size() && shots[next].first == k) {
+                hip_check(hipMemcpyAsync(shots[next].second, f.ptr, state_bytes, hipMemcpyDeviceToDevice, stream),
                           "hipMemcpyAsync snapshot");
-            ops->step(st, ((t0 + k) & 1U) != 0U, b, false, f.ptr, f.ptr);
-            if (k + 1 >= steps - window + 1)
-                adj->objective(st, ((t0 + k + 1) & 1U) != 0U, b, f.ptr, args, 1.0, reinterpret_cast<double*>(acc.ptr));
+                ++next;
+            }
+            ops->step(st, par(k), b, false, f.ptr, f.ptr);
+            ++forward_steps;
+            if (k + 1 >= steps - window + 1) adj->objective(st, par(k + 1), b, f.ptr, args, 1.0, reinterpret_cast<double*>(acc.ptr));
         }
         t = t0 + steps;
         hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+        const auto transposed = [&](uint64_t tt, const void* state) {
+            args.add_objective = tt + window >= steps + 1 ? 1 : 0;
+            adj->step(st, par(tt), b, state, lam.ptr, args);
+        };
+        // States a+1 .. m of the chain into dst, starting from state a in src (out of place first).
+        const auto advance = [&](const void* src, void* dst, uint64_t a0, uint64_t m) {
+            ops->step(st, par(a0), b, false, src, dst);
+            for (uint64_t k = a0 + 1; k < m; ++k) ops->step(st, par(k), b, false, dst, dst);
+            forward_steps += m - a0;
+        };
         args.add_objective = 0;
-        adj->seed(st, (t & 1U) != 0U, b, lam.ptr, args);
-        for (uint64_t c = segments; c-- > 0;) {
-            const uint64_t s0 = c * every, e = std::min(steps, s0 + every);
-            std::vector<const void*> states(e - s0);
-            states[0] = snaps[c]->ptr;
-            for (uint64_t k = 1; k < e - s0; ++k) {
-                ops->step(st, ((t0 + s0 + k - 1) & 1U) != 0U, b, false, states[k - 1], replay[k - 1]->ptr);
-                states[k] = replay[k - 1]->ptr;
+        adj->seed(st, par(steps), b, lam.ptr, args);
+        if (every > 0) {
+            const uint64_t segments = shots.size();
+            for (uint64_t c = segments; c-- > 0;) {
+                const uint64_t s0 = c * every, e = std::min(steps, s0 + every);
+                std::vector<const void*> states(e - s0);
+                states[0] = shots[c].second;
+                for (uint64_t k = 1; k < e - s0; ++k) {
+                    void* dst = pool[segments + k - 1]->ptr;
+                    advance(states[k - 1], dst, s0 + k - 1, s0 + k);
+                    states[k] = dst;
+                }
+                for (uint64_t k = e - s0; k-- > 0;) transposed(s0 + k, states[k]);
             }
-            for (uint64_t k = e - s0; k-- > 0;) {
-                const uint64_t tt = s0 + k;
-                args.add_objective = tt + window >= steps + 1 ? 1 : 0;
-                adj->step(st, ((t0 + tt) & 1U) != 0U, b, states[k], lam.ptr, args);
-            }
+        } else {
+            // Reverse [a0, b0) with state a0 in src: advance to the split into a free buffer, reverse
+            // the right part with one buffer less, then the left part.  On the first descent the
+            // split states are already in place from the forward sweep.
+            std::function<void(uint64_t, uint64_t, const void*, bool)> reverse = [&](uint64_t a0, uint64_t b0,
+                                                                                   const void* src, bool first) {
+                while (b0 - a0 > 1) {
+                    const uint64_t m = a0 + cfd_hip::BinomialSchedule::split(b0 - a0, free_list.size());
+                    void* dst = free_list.back();
+                    free_list.pop_back();
+                    if (!first) advance(src, dst, a0, m);
+                    reverse(m, b0, dst, first);
+                    free_list.push_back(dst);
+                    b0 = m;
+                    first = false;
+                }
+                transposed(a0, src);
+            };
+            reverse(0, steps, shots[0].second, true);
         }
         hip_check(hipEventRecord(e2, stream), "hipEventRecord");
         double J = 0.0;
@@ -513,11 +669,13 @@ struct cfd_lbm_solver {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipEventDestroy(e2);
+        last_forward_steps = forward_steps;
         if (grad_peak_out) {
             double sum = 0.0;
             for (double v : peak) sum += v;
             *grad_peak_out = sum;
         }
+        if (forward_steps_out) *forward_steps_out = forward_steps;
         return J;
     }
 
@@ -879,7 +1037,18 @@ int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int reset, char*
 
 int cfd_lbm_objective_pressure_drop(cfd_lbm_solver* s, int x_in, int x_out, double* value, char* error, size_t cap) {
     return guarded(error, cap, [&] {
-        const double v = need(s).objective(x_in, x_out);
+        const double v = need(s).objective(cfd_hip::kPressureDrop, x_in, x_out);
+        if (value) *value = v;
+    });
+}
+
+int cfd_lbm_set_drag_body(cfd_lbm_solver* s, const uint8_t* body, int axis, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).set_drag_body(body, axis); });
+}
+
+int cfd_lbm_objective_drag(cfd_lbm_solver* s, double* value, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        const double v = need(s).objective(cfd_hip::kDrag, 0, 0);
         if (value) *value = v;
     });
 }
@@ -888,8 +1057,31 @@ int cfd_lbm_adjoint_run(cfd_lbm_solver* s, uint64_t steps, uint64_t window, uint
                         int x_out, double* objective, double* grad_gamma, double* grad_u_peak, char* error,
                         size_t cap) {
     return guarded(error, cap, [&] {
-        const double J = need(s).adjoint_run(steps, window, checkpoint_every, x_in, x_out, grad_gamma, grad_u_peak);
+        if (checkpoint_every < 1) throw std::invalid_argument("checkpoint interval must be >= 1");
+        const double J = need(s).adjoint_run(cfd_hip::kPressureDrop, steps, window, checkpoint_every, 0, x_in, x_out,
+                                             grad_gamma, grad_u_peak, nullptr);
         if (objective) *objective = J;
+    });
+}
+
+int cfd_lbm_adjoint_run_binomial(cfd_lbm_solver* s, int objective_kind, uint64_t steps, uint64_t window,
+                                 uint64_t snapshots, int x_in, int x_out, double* objective, double* grad_gamma,
+                                 double* grad_u_peak, uint64_t* forward_steps, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        const double J = need(s).adjoint_run(objective_kind, steps, window, 0, snapshots, x_in, x_out, grad_gamma,
+                                             grad_u_peak, forward_steps);
+        if (objective) *objective = J;
+    });
+}
+
+int cfd_lbm_checkpoint_plan(uint64_t steps, uint64_t snapshots, uint64_t* forward_steps, uint64_t* repetitions,
+                            uint64_t* walked_steps, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (steps < 1 || snapshots < 1) throw std::invalid_argument("need steps >= 1 and snapshots >= 1");
+        using B = cfd_hip::BinomialSchedule;
+        if (forward_steps) *forward_steps = B::forward_steps(steps, snapshots) + 1;  // + the step to the last state
+        if (repetitions) *repetitions = B::repetitions(steps, snapshots);
+        if (walked_steps) *walked_steps = B::walk(steps, snapshots) + 1;
     });
 }
 

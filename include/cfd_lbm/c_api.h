@@ -137,19 +137,49 @@ CFD_LBM_API int cfd_lbm_stream(cfd_lbm_solver* s, void** stream, char* error, si
 CFD_LBM_API int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int reset, char* error,
                                        size_t error_capacity);
 
+/* Objective kinds of the adjoint entry points. */
+enum { CFD_LBM_OBJECTIVE_PRESSURE_DROP = 0, CFD_LBM_OBJECTIVE_DRAG = 1 };
+
 /* Pressure-drop objective at the current state (FP64): j = mean rho/3 over the fluid cells of plane
  * x = x_in minus the same over plane x = x_out.  No reference counterpart (the reference has no
- * objectives; docs/RELEASE_0_10_9.md:39).  FP32 or FP64 storage. */
+ * objectives; docs/RELEASE_0_10_9.md:39).  Any storage. */
 CFD_LBM_API int cfd_lbm_objective_pressure_drop(cfd_lbm_solver* s, int x_in, int x_out, double* value, char* error,
                                                 size_t error_capacity);
+/* Registers the body of the drag objective: body[cells] != 0 marks wall cells that belong to it
+ * (NULL clears).  The drag is the momentum exchange of halfway bounce-back on the links from fluid
+ * cells into the body, along `axis` (0 x, 1 y, 2 z).  set_flags clears the registration.  No
+ * reference counterpart (the reference computes no forces on bodies). */
+CFD_LBM_API int cfd_lbm_set_drag_body(cfd_lbm_solver* s, const uint8_t* body, int axis, char* error,
+                                      size_t error_capacity);
+/* Drag on the registered body at the current state (FP64): sum over fluid cells n and populations d
+ * arriving from a body cell of -2 c_d[axis] f_d(n). */
+CFD_LBM_API int cfd_lbm_objective_drag(cfd_lbm_solver* s, double* value, char* error, size_t error_capacity);
 /* Discrete adjoint (no reference counterpart: the reference has no adjoint).  Runs `steps` steps
  * from the current state, J = mean of the pressure drop over the last `window` states, then the
- * transposed steps back with a snapshot every `checkpoint_every` steps; returns J, dJ/dgamma per
- * cell (cells doubles, may be NULL) and dJ/du_peak.  The state is left at time + steps.  Unforced
- * lattices with walls at rest, PBB, inlet and outlet; FP32 or FP64 storage. */
+ * transposed steps back with a snapshot every `checkpoint_every` steps (two-level: each step is
+ * recomputed once; memory ceil(steps/every) + every - 1 states); returns J, dJ/dgamma per cell
+ * (cells doubles, may be NULL) and dJ/du_peak.  The state is left at time + steps.  Unforced
+ * lattices with walls at rest, PBB, inlet and outlet; any storage (FP16: the adjoint of the FP32
+ * collision at the stored states, lambda in FP32). */
 CFD_LBM_API int cfd_lbm_adjoint_run(cfd_lbm_solver* s, uint64_t steps, uint64_t window, uint64_t checkpoint_every,
                                     int x_in, int x_out, double* objective, double* grad_gamma, double* grad_u_peak,
                                     char* error, size_t error_capacity);
+/* The same with binomial checkpointing: `snapshots` free state buffers besides the start state
+ * (0: as many as fit in device memory, 1/16 of it kept free), each step recomputed at most r times
+ * where C(snapshots + r, r) >= steps, with the minimal number of forward steps.  objective_kind
+ * CFD_LBM_OBJECTIVE_PRESSURE_DROP (planes x_in, x_out) or CFD_LBM_OBJECTIVE_DRAG (the registered
+ * body; x_in, x_out unused).  forward_steps (may be NULL) returns the forward steps executed, the
+ * first sweep included. */
+CFD_LBM_API int cfd_lbm_adjoint_run_binomial(cfd_lbm_solver* s, int objective_kind, uint64_t steps, uint64_t window,
+                                             uint64_t snapshots, int x_in, int x_out, double* objective,
+                                             double* grad_gamma, double* grad_u_peak, uint64_t* forward_steps,
+                                             char* error, size_t error_capacity);
+/* The binomial plan without a solver (host only, no device call): forward steps of a run of
+ * `steps` steps with `snapshots` buffers from the closed form r*l - C(s+r, r-1) + 1, the
+ * repetition number r, and the same count obtained by walking the plan. */
+CFD_LBM_API int cfd_lbm_checkpoint_plan(uint64_t steps, uint64_t snapshots, uint64_t* forward_steps,
+                                        uint64_t* repetitions, uint64_t* walked_steps, char* error,
+                                        size_t error_capacity);
 /* GPU time of the last adjoint run's forward phase and reverse (recompute + transposed) phase. */
 CFD_LBM_API int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_ms, char* error,
                                        size_t error_capacity);

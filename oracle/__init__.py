@@ -440,26 +440,37 @@ class RefD2Q9:
 
 
 def _adjoint_bindings(L):
-    L.orc_objective.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
-    L.orc_adjoint_run.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double),
-                                  _dp, C.POINTER(C.c_double)]
+    L.orc_objective.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.orc_set_drag_body.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    L.orc_adjoint_run.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                                  C.POINTER(C.c_double), _dp, C.POINTER(C.c_double)]
 
 
-def _objective(self, x_in, x_out):
+_KINDS = {"pressure_drop": 0, "drag": 1}
+
+
+def _objective(self, x_in=0, x_out=0, kind="pressure_drop"):
     _adjoint_bindings(self.L)
     out = C.c_double()
-    _check(self.L.orc_objective(self.h, x_in, x_out, C.byref(out)))
+    _check(self.L.orc_objective(self.h, _KINDS[kind], x_in, x_out, C.byref(out)))
     return out.value
 
 
-def _adjoint_run(self, steps, window, x_in, x_out):
+def _set_drag_body(self, body, axis=0):
+    _adjoint_bindings(self.L)
+    b = np.ascontiguousarray(np.asarray(body).reshape(-1) != 0, dtype=np.uint8)
+    _check(self.L.orc_set_drag_body(self.h, b.ctypes.data, axis))
+
+
+def _adjoint_run(self, steps, window, x_in=0, x_out=0, kind="pressure_drop"):
     """Forward `steps` steps, J over the last `window` states, and dJ/dgamma (per cell), dJ/du_peak."""
     _adjoint_bindings(self.L)
     J, du = C.c_double(), C.c_double()
     grad = np.zeros(self.cells)
-    _check(self.L.orc_adjoint_run(self.h, steps, window, x_in, x_out, C.byref(J), grad, C.byref(du)))
+    _check(self.L.orc_adjoint_run(self.h, _KINDS[kind], steps, window, x_in, x_out, C.byref(J), grad, C.byref(du)))
     return J.value, grad.reshape(self.shape), du.value
 
 
 Oracle.objective = _objective
+Oracle.set_drag_body = _set_drag_body
 Oracle.adjoint_run = _adjoint_run

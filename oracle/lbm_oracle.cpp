@@ -184,9 +184,10 @@ public:
     virtual uint64_t time_step() const = 0;
     virtual void set_time_step(uint64_t t) = 0;
     virtual size_t cells() const = 0;
-    virtual double objective(int x_in, int x_out) const = 0;
-    virtual double adjoint_run(uint64_t steps, uint64_t window, int x_in, int x_out, double* grad_gamma,
+    virtual double objective(int kind, int x_in, int x_out) const = 0;
+    virtual double adjoint_run(int kind, uint64_t steps, uint64_t window, int x_in, int x_out, double* grad_gamma,
                                double* grad_u_peak) = 0;
+    virtual void set_drag_body(const uint8_t* body, int axis) = 0;
 };
 
 template <int Q, class Real, class Store> class Oracle final : public OracleBase {
@@ -477,6 +478,8 @@ private:
     Real tau_, ax_, ay_, az_;
     std::vector<Store> F_;
     std::vector<uint8_t> flags_;
+    std::vector<uint8_t> body_;  // drag body (wall cells), empty: none
+    int drag_axis_{0};
     std::vector<Real> gamma_, lax_, lay_, laz_;
     Real wall_u_[8][3];
     Real u_peak_{0}, rho_out_{1};
@@ -719,12 +722,29 @@ private:
 
 public:
     // ------------------------------------------------------------ objective and adjoint
-    // Pressure drop between the fluid cells of plane x_in and plane x_out:
+    // kind 0, pressure drop between the fluid cells of plane x_in and plane x_out:
     //   j(f) = mean_{x = x_in} rho / 3 - mean_{x = x_out} rho / 3,   rho = sum_d f_d (logical).
-    double objective(int x_in, int x_out) const override {
+    // kind 1, drag on the registered body (momentum exchange of halfway bounce-back): the population
+    // f*_opp(d) that left fluid cell n towards a body cell at n - c_d returns as f_d(n), so the body
+    // receives 2 c_opp(d) f_d(n):
+    //   j(f) = sum over fluid n and d with n - c_d in the body of -2 c_d[axis] f_d(n) (logical).
+    double objective(int kind, int x_in, int x_out) const override {
         std::vector<Real> fin(static_cast<size_t>(Q) * N_);
         logical_state(fin.data());
-        return plane_objective(fin.data(), x_in, x_out);
+        return kind == 1 ? drag_objective(fin.data()) : plane_objective(fin.data(), x_in, x_out);
+    }
+
+    void set_drag_body(const uint8_t* body, int axis) override {
+        if (!body) {
+            body_.clear();
+            return;
+        }
+        if (axis < 0 || axis > 2) throw std::invalid_argument("drag axis must be 0, 1 or 2");
+        if (flags_.empty()) throw std::invalid_argument("the drag objective needs wall flags");
+        body_.assign(body, body + N_);
+        for (size_t n = 0; n < N_; ++n)
+            if (body_[n] && !is_wall(flags_[n])) throw std::invalid_argument("drag body cells must be wall cells");
+        drag_axis_ = axis;
     }
 
     // Runs `steps` forward steps from the current state and returns
@@ -733,9 +753,10 @@ public:
     // reverse streaming (the transpose of the pull/bounce-back permutation) and the transposed
     // collision Jacobian, evaluated in double precision on the stored forward states.  Supports
     // unforced BGK with partial bounce-back, stationary walls, velocity inlet and pressure outlet.
-    double adjoint_run(uint64_t steps, uint64_t window, int x_in, int x_out, double* grad_gamma,
+    double adjoint_run(int kind, uint64_t steps, uint64_t window, int x_in, int x_out, double* grad_gamma,
                        double* grad_u_peak) override {
         if (window < 1 || window > steps) throw std::invalid_argument("objective window must be in [1, steps]");
+        if (kind == 1 && body_.empty()) throw std::invalid_argument("no drag body registered");
         if (ax_ != Real(0) || ay_ != Real(0) || az_ != Real(0) || !lax_.empty())
             throw std::invalid_argument("the adjoint supports unforced lattices only");
         for (int k = 1; k < 8; ++k)
@@ -750,7 +771,8 @@ public:
         for (uint64_t t = 0; t < steps; ++t) {
             step(1);
             logical_state(states[t + 1].data());
-            if (t + 1 >= steps - window + 1) J += plane_objective(states[t + 1].data(), x_in, x_out);
+            if (t + 1 >= steps - window + 1)
+                J += kind == 1 ? drag_objective(states[t + 1].data()) : plane_objective(states[t + 1].data(), x_in, x_out);
         }
         J /= static_cast<double>(window);
 
@@ -762,9 +784,16 @@ public:
             count_in += x == x_in;
             count_out += x == x_out;
         }
-        const double c_in = 1.0 / (3.0 * static_cast<double>(count_in) * static_cast<double>(window));
-        const double c_out = -1.0 / (3.0 * static_cast<double>(count_out) * static_cast<double>(window));
+        const double c_in = count_in ? 1.0 / (3.0 * static_cast<double>(count_in) * static_cast<double>(window)) : 0.0;
+        const double c_out = count_out ? -1.0 / (3.0 * static_cast<double>(count_out) * static_cast<double>(window)) : 0.0;
         auto add_source = [&](std::vector<double>& lam) {
+            if (kind == 1) {
+                for (size_t n = 0; n < N_; ++n)
+                    for (int d = 1; d < Q; ++d)
+                        if (body_link(n, d))
+                            lam[static_cast<size_t>(d) * N_ + n] += -2.0 * c_axis(d) / static_cast<double>(window);
+                return;
+            }
             for (size_t n = 0; n < N_; ++n) {
                 if (!flags_.empty() && flags_[n] != kFluid) continue;
                 const int x = static_cast<int>(n % nx_);
@@ -830,6 +859,27 @@ private:
             load(n, fin);
             for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = fin[d];
         }
+    }
+
+    double c_axis(int d) const {
+        return static_cast<double>(drag_axis_ == 0 ? L::cx[d] : (drag_axis_ == 1 ? L::cy[d] : L::cz[d]));
+    }
+
+    // Population d of fluid cell n arrives from a body cell (the upstream cell n - c_d).
+    bool body_link(size_t n, int d) const {
+        if (flags_[n] != kFluid) return false;
+        size_t x, y, z;
+        coords(n, x, y, z);
+        return body_[neighbor(x, y, z, opposite(d))] != 0;
+    }
+
+    double drag_objective(const Real* fin) const {
+        if (body_.empty()) throw std::invalid_argument("no drag body registered");
+        double s = 0.0;
+        for (size_t n = 0; n < N_; ++n)
+            for (int d = 1; d < Q; ++d)
+                if (body_link(n, d)) s += -2.0 * c_axis(d) * static_cast<double>(fin[static_cast<size_t>(d) * N_ + n]);
+        return s;
     }
 
     double plane_objective(const Real* fin, int x_in, int x_out) const {
@@ -1069,12 +1119,15 @@ ORC_API int orc_bb_load_map(void* h, int p, int64_t* out) { return guarded([&] {
 ORC_API uint64_t orc_time_step(void* h) { return H(h)->time_step(); }
 ORC_API void orc_set_time_step(void* h, uint64_t t) { H(h)->set_time_step(t); }
 ORC_API size_t orc_cells(void* h) { return H(h)->cells(); }
-ORC_API int orc_objective(void* h, int x_in, int x_out, double* out) {
-    return guarded([&] { *out = H(h)->objective(x_in, x_out); });
+ORC_API int orc_objective(void* h, int kind, int x_in, int x_out, double* out) {
+    return guarded([&] { *out = H(h)->objective(kind, x_in, x_out); });
 }
-ORC_API int orc_adjoint_run(void* h, uint64_t steps, uint64_t window, int x_in, int x_out, double* J,
+ORC_API int orc_set_drag_body(void* h, const uint8_t* body, int axis) {
+    return guarded([&] { H(h)->set_drag_body(body, axis); });
+}
+ORC_API int orc_adjoint_run(void* h, int kind, uint64_t steps, uint64_t window, int x_in, int x_out, double* J,
                             double* grad_gamma, double* grad_u_peak) {
-    return guarded([&] { *J = H(h)->adjoint_run(steps, window, x_in, x_out, grad_gamma, grad_u_peak); });
+    return guarded([&] { *J = H(h)->adjoint_run(kind, steps, window, x_in, x_out, grad_gamma, grad_u_peak); });
 }
 ORC_API uint16_t orc_f32_to_f16(float f) { return orc::f32_to_f16_rne(f); }
 ORC_API float orc_f16_to_f32(uint16_t h) { return orc::f16_to_f32(h); }

@@ -160,3 +160,137 @@ def test_config4_gradient_vs_finite_differences(hip):
     report("config4_fd_upeak", adjoint=du64, fd=fd_u, fp32=du32, worst_cell_rel=worst)
     assert abs(du64 - fd_u) <= 1e-4 * abs(fd_u)
     assert abs(du32 - du64) <= 1e-3 * abs(du64)
+
+
+# ---------------------------------------------------------------- binomial checkpointing, drag, FP16
+def cube_duct(n=(16, 10, 10)):
+    nx, ny, nz = n
+    f = duct(nx, ny, nz).reshape(nz, ny, nx)
+    body = np.zeros_like(f)
+    body[4:6, 4:6, 6:8] = 1
+    f[body != 0] = 0x80
+    return f.reshape(-1), body.reshape(-1)
+
+
+def gpu_cube(hip, blend, storage, n=(16, 10, 10), u_peak=0.05):
+    flags, body = cube_duct(n)
+    s = hip.EsotericPullHipSolver(19, *n, 0.6, storage=storage)
+    s.set_flags(flags)
+    s.set_drag_body(body, 0)
+    s.set_inlet_outlet(u_peak, 1.0)
+    s.set_porosity(blend.astype(np.float32))
+    s.initialize_uniform()
+    return s
+
+
+def oracle_cube(oracle_mod, blend, storage, n=(16, 10, 10), u_peak=0.05):
+    flags, body = cube_duct(n)
+    o = oracle_mod.Oracle(19, *n, float(np.float32(0.6)), storage=storage)
+    o.set_flags(flags)
+    o.set_drag_body(body, 0)
+    o.set_inlet_outlet(float(np.float32(u_peak)), 1.0)
+    o.set_gamma(blend.astype(np.float32).astype(np.float64))
+    o.init_uniform()
+    return o
+
+
+def test_binomial_matches_two_level_bitwise(hip):
+    """Binomial plans with 1, 2, 3 and 'as many as fit' buffers run the same transposed steps on
+    bit-identical recomputed states: J and both gradients equal the two-level result bit for bit, and
+    the forward-step count is the plan's optimum."""
+    report("binomial_vs_two_level")
+    n = (16, 8, 8)
+    blend = 0.05 * design(duct(*n))
+    T, W = 40, 8
+    ref = gpu_duct(hip, n, blend, storage="fp32").adjoint_run(T, W, 1, 1, n[0] - 2)
+    for snaps in (1, 2, 3, 0):
+        g = gpu_duct(hip, n, blend, storage="fp32")
+        J, grad, du, fw = g.adjoint_run_binomial(T, W, snaps, "pressure_drop", 1, n[0] - 2)
+        assert J == ref[0] and du == ref[2]
+        assert np.array_equal(grad, ref[1])
+        expected = hip.checkpoint_plan(T, snaps if snaps else T - 1)[0]
+        assert fw == expected, (snaps, fw, expected)
+        assert g.time_step == T
+
+
+def test_drag_objective_matches_oracle(hip, oracle_mod):
+    report("drag_objective")
+    flags, _ = cube_duct()
+    blend = 0.05 * design(flags)
+    g = gpu_cube(hip, blend, "fp32")
+    o = oracle_cube(oracle_mod, blend, "fp32")
+    for steps in (0, 1, 7):
+        g.step(steps)
+        o.step(steps)
+        jg, jo = g.objective_drag(), o.objective(kind="drag")
+        assert jg == pytest.approx(jo, rel=1e-12, abs=1e-15)
+    assert g.objective_drag() > 0.0
+
+
+@pytest.mark.parametrize("storage,tol", [("fp64", 1e-9), ("fp32", 2e-4), ("fp16s", 2e-4)])
+def test_drag_adjoint_binomial_matches_oracle(hip, oracle_mod, storage, tol):
+    """The drag objective's adjoint under binomial checkpointing against the oracle's logical-space
+    adjoint on the same (bit-identical) forward states; FP16 storage keeps lambda in FP32."""
+    report("drag_adjoint", storage=storage)
+    flags, _ = cube_duct()
+    blend = 0.05 * design(flags)
+    T, W = 30, 6
+    g = gpu_cube(hip, blend, storage)
+    o = oracle_cube(oracle_mod, blend, storage)
+    J, grad, du, fw = g.adjoint_run_binomial(T, W, 3, "drag")
+    Jo, grado, duo = o.adjoint_run(T, W, kind="drag")
+    assert J == pytest.approx(Jo, rel=tol)
+    scale = np.abs(grado).max()
+    assert np.abs(grad - grado).max() <= tol * scale
+    assert du == pytest.approx(duo, rel=tol)
+    assert fw == hip.checkpoint_plan(T, 3)[0]
+
+
+def test_fp16_pressure_adjoint_matches_oracle(hip, oracle_mod):
+    report("adjoint_fp16")
+    n = (16, 8, 8)
+    blend = 0.05 * design(duct(*n))
+    T, W = 40, 8
+    g = gpu_duct(hip, n, blend, storage="fp16s")
+    o = oracle_duct(oracle_mod, n, blend, storage="fp16s")
+    J, grad, du = g.adjoint_run(T, W, 7, 1, n[0] - 2)
+    Jo, grado, duo = o.adjoint_run(T, W, 1, n[0] - 2)
+    assert J == pytest.approx(Jo, rel=1e-9)
+    assert np.abs(grad - grado).max() <= 2e-4 * np.abs(grado).max()
+    assert du == pytest.approx(duo, rel=2e-4)
+
+
+def test_torch_autograd_function(hip):
+    """LbmObjective: J.backward() returns the adjoint gradients, and a directional central
+    difference of J along a random design perturbation matches <dJ/dgamma, v> (FP64 storage)."""
+    import torch
+
+    from cfd_solvers_amd.autograd import LbmAdjointProblem, lbm_objective
+
+    report("torch_autograd")
+    n = (16, 8, 8)
+    flags = duct(*n)
+    blend = 0.05 * design(flags)
+    prob = LbmAdjointProblem(flags=flags, shape=n, storage="fp64", steps=40, window=8, checkpoint_every=0,
+                             snapshots=4, x_in=1, x_out=n[0] - 2)
+    dev = torch.device("cuda:0")
+    gamma = torch.tensor(blend.astype(np.float32), device=dev, requires_grad=True)
+    u_peak = torch.tensor(0.05, dtype=torch.float32, device=dev, requires_grad=True)
+    J = lbm_objective(gamma, u_peak, prob)
+    J.backward()
+    ref_J, ref_grad, ref_du = gpu_duct(hip, n, blend, storage="fp64").adjoint_run(40, 8, 1, 1, n[0] - 2)
+    assert J.item() == ref_J
+    assert torch.allclose(gamma.grad.double().cpu(), torch.from_numpy(ref_grad.reshape(-1)).float().double(),
+                          rtol=1e-6, atol=0)
+    assert u_peak.grad.item() == pytest.approx(ref_du, rel=1e-6)
+    rng = np.random.default_rng(12)
+    v = np.where(flags == 0, rng.standard_normal(flags.size), 0.0)
+    h = 1e-3
+    gp = (blend + h * 0.05 * v).clip(0, 1).astype(np.float32)
+    gm = (blend - h * 0.05 * v).clip(0, 1).astype(np.float32)
+    with torch.no_grad():
+        Jp = lbm_objective(torch.tensor(gp, device=dev), u_peak.detach(), prob).item()
+        Jm = lbm_objective(torch.tensor(gm, device=dev), u_peak.detach(), prob).item()
+    directional = float(np.dot(ref_grad.reshape(-1), gp.astype(np.float64) - gm.astype(np.float64)))
+    assert Jp - Jm == pytest.approx(directional, rel=1e-5)
+    prob.close()

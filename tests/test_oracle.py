@@ -427,3 +427,63 @@ def test_oracle_adjoint_matches_finite_differences():
           - _duct_oracle(gamma, 0.05 - h)[0].adjoint_run(T, W, 1, 14)[0]) / (2 * h)
     assert du == pytest.approx(fd, rel=1e-6)
     assert o.time_step == T
+
+
+def _cube_duct(gamma, u_peak, n=(16, 10, 10)):
+    """The duct above with a 2x2x2 wall cube (the drag body) in the middle."""
+    nx, ny, nz = n
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, 0, :] = f[:, -1, :] = f[0] = f[-1] = O.WALL
+    f[1:-1, 1:-1, 0] = O.INLET
+    f[1:-1, 1:-1, -1] = O.OUTLET
+    body = np.zeros_like(f)
+    body[4:6, 4:6, 6:8] = 1
+    f[body != 0] = O.WALL
+    o = O.Oracle(19, nx, ny, nz, 0.6, storage="fp64")
+    o.set_flags(f)
+    o.set_drag_body(body, 0)
+    o.set_inlet_outlet(u_peak, 1.0)
+    o.set_gamma(np.where(f.reshape(-1) == 0, gamma, 0.0))
+    o.init_uniform()
+    return o, f, body
+
+
+def test_oracle_drag_balances_the_body_force():
+    """Forced periodic channel at steady state: the momentum exchange on all walls equals the momentum
+    the Guo forcing puts in per step, sum over fluid cells of rho * a_x."""
+    nx, ny, nz, a = 6, 10, 10, 1e-5
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, 0, :] = f[:, -1, :] = f[0] = f[-1] = O.WALL
+    o = O.Oracle(19, nx, ny, nz, 0.8, accel=(a, 0.0, 0.0), storage="fp64")
+    o.set_flags(f)
+    o.set_drag_body(f != 0, 0)
+    o.init_uniform()
+    o.step(1500)
+    rho = o.macroscopic()[0]
+    expected = a * rho[f == 0].sum()
+    assert o.objective(kind="drag") == pytest.approx(expected, rel=1e-6)
+    assert abs(o.objective(kind="drag")) > 0.0
+    # Along y the walls are symmetric: no net force.
+    o.set_drag_body(f != 0, 1)
+    assert abs(o.objective(kind="drag")) < 1e-12 * expected
+
+
+def test_oracle_drag_adjoint_matches_finite_differences():
+    rng = np.random.default_rng(9)
+    gamma = 0.05 * rng.uniform(0.3, 0.7, 16 * 10 * 10)
+    T, W = 30, 6
+    o, f, body = _cube_duct(gamma, 0.05)
+    J, grad, du = o.adjoint_run(T, W, kind="drag")
+    assert J > 0.0  # flow along +x pushes the cube along +x
+    h = 1e-5
+    fluid = np.flatnonzero(f.reshape(-1) == 0)
+    for c in rng.choice(fluid, 4, replace=False):
+        gp, gm = gamma.copy(), gamma.copy()
+        gp[c] += h
+        gm[c] -= h
+        fd = (_cube_duct(gp, 0.05)[0].adjoint_run(T, W, kind="drag")[0]
+              - _cube_duct(gm, 0.05)[0].adjoint_run(T, W, kind="drag")[0]) / (2 * h)
+        assert grad.reshape(-1)[c] == pytest.approx(fd, rel=1e-6, abs=1e-13)
+    fd = (_cube_duct(gamma, 0.05 + h)[0].adjoint_run(T, W, kind="drag")[0]
+          - _cube_duct(gamma, 0.05 - h)[0].adjoint_run(T, W, kind="drag")[0]) / (2 * h)
+    assert du == pytest.approx(fd, rel=1e-6)
