@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,d3q27] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,d3q27,adjoint[,fp64,config5]] [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
 u = (0.1, 0, 0), tau 0.55, uint8 flags) with FP32 population storage; further legs run the same
-geometry with FP16 shifted-population storage and with D3Q27.  Prints progress on stderr and ONE
-JSON line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
+geometry with FP16 shifted-population storage and with D3Q27, and the adjoint leg runs the
+checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps).  The opt-in
+config5 leg runs forward + adjoint of a 768^3 FP16 duct past a sphere cluster (drag objective,
+binomial checkpointing) against the FP32-storage run.  Prints progress on stderr and ONE JSON
+line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
 (apps/cfd-bench.cpp:111-117); the roofline uses algorithmic HBM bytes per launch (each fluid
 population read and written once, plus one read of the flag array) over the kernel time measured
 with HIP events on the solver's stream.
@@ -28,7 +31,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "D3Q19 MLUPS fwd FP32/FP16-store + adjoint, % of 8 TB/s HBM roofline"
 PEAK_GBS = 8000.0
-BYTES_PER_POP = {"fp32": 4, "fp16s": 2, "fp64": 8}
+BYTES_PER_POP = {"fp32": 4, "fp16s": 2, "fp64": 8, "fp32s": 4}
 ADJ_BYTES = 3 * 19 * 4 + 1 + 4 + 16
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_stream_collide_d3q19_256_cavity.json"
 
@@ -142,6 +145,60 @@ def adjoint_leg(hip, n=128, steps=2000, window=200, every=100, kernel_steps=100)
     return res
 
 
+def config5_leg(hip, n=768, steps=1000, storages=("fp16s", "fp32")):
+    """Flow past a cluster: D3Q19 n^3 duct (inlet peak 0.04, outlet rho 1), 64 non-overlapping spheres
+    of radius U[10, 30] (seed 5) with centres in the middle 40 % of each axis, a partial-bounce-back
+    halo ramping from 0.5 at the sphere surfaces to 0 four cells out, tau 0.52; J = drag along x
+    averaged over all `steps` states; binomial checkpointing with as many state buffers as fit.
+    The FP16-storage run is compared with the FP32-storage run of the same problem."""
+    from cfd_solvers_amd import geometry as G
+
+    shape = (n, n, n)
+    lo, hi = 0.3 * n, 0.7 * n
+    centres, radii = G.random_spheres(shape, 10, 30, seed=5, count=64, box=((lo, hi),) * 3, periodic_x=False)
+    flags = G.channel_flags(n, n, n, inlet_outlet=True)
+    body = G.rasterize(shape, centres, radii, periodic_x=False)
+    flags[body] = G.WALL
+    halo = G.porosity_halo(shape, centres, radii, 4.0, 0.5)
+    halo[flags != 0] = 0.0
+    flags, body, halo = flags.reshape(-1), body.reshape(-1), halo.reshape(-1)
+    halo_cells = np.flatnonzero(halo > 0)
+    out = {"workload": f"D3Q19 {n}^3 duct, 64-sphere cluster + PBB halo, tau 0.52, inlet 0.04, drag, T {steps}",
+           "cells": n ** 3, "steps": steps, "spheres": 64, "body_cells": int(body.sum()),
+           "halo_cells": int(halo_cells.size)}
+    grads = {}
+    for storage in storages:
+        log(f"leg config5: {storage} {n}^3, T {steps}")
+        s = hip.EsotericPullHipSolver("d3q19", n, n, n, 0.52, storage=storage)
+        s.set_flags(flags)
+        s.set_drag_body(body, 0)
+        s.set_inlet_outlet(0.04, 1.0)
+        s.set_porosity(halo)
+        s.initialize_uniform()
+        t0 = time.perf_counter()
+        J, grad, du, fw = s.adjoint_run_binomial(steps, steps, 0, "drag")
+        wall = time.perf_counter() - t0
+        fwd_ms, rev_ms = s.adjoint_timing()
+        s.close()
+        grads[storage] = grad.reshape(-1)[halo_cells]
+        out[storage] = {"objective": J, "du_peak": du, "forward_steps": fw, "wall_seconds": wall,
+                        "forward_ms": fwd_ms, "reverse_ms": rev_ms,
+                        "forward_mlups": n ** 3 * steps / (fwd_ms * 1e3),
+                        "adjoint_mlups": n ** 3 * steps / (rev_ms * 1e3),
+                        "total_over_forward": (fwd_ms + rev_ms) / fwd_ms}
+        log(f"leg config5 {storage}: J {J:.6e}, du {du:.6e}, forward {fwd_ms:.0f} ms, reverse {rev_ms:.0f} ms "
+            f"({fw} forward steps)")
+        del grad
+    if "fp16s" in grads and "fp32" in grads:
+        a, b = grads["fp16s"], grads["fp32"]
+        out["fp16_vs_fp32"] = {
+            "objective_rel": abs(out["fp16s"]["objective"] - out["fp32"]["objective"]) / abs(out["fp32"]["objective"]),
+            "du_peak_rel": abs(out["fp16s"]["du_peak"] - out["fp32"]["du_peak"]) / abs(out["fp32"]["du_peak"]),
+            "grad_rel_l2_halo": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
+        log(f"leg config5: FP16 vs FP32 {out['fp16_vs_fp32']}")
+    return out
+
+
 def copy_probe():
     import ctypes as C
 
@@ -205,6 +262,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--legs", default="fp32,fp16,d3q27,adjoint")
+    ap.add_argument("--config5-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
     args = ap.parse_args()
@@ -223,10 +281,13 @@ def main():
         if name == "adjoint":
             legs[name] = adjoint_leg(hip)
             continue
+        if name == "config5":
+            legs[name] = config5_leg(hip, steps=args.config5_steps)
+            continue
         lattice, storage = table[name]
         legs[name] = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
     probe = None if args.no_probe else copy_probe()
-    head = legs.get("fp32") or next(v for k, v in legs.items() if k != "adjoint")
+    head = legs.get("fp32") or next(v for k, v in legs.items() if "mlups" in v)
 
     traffic = None
     if TRAFFIC_FILE.exists() and args.n == 256:

@@ -162,6 +162,11 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
     Real f[Q];
     PairSlots<Q> s;
     load_cell<Q, T, Odd, Bounded>(fstate, g, p.links, h, n, f, s);
+    if constexpr (Storage<T>::kShifted)  // the transposed collision takes the populations themselves
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            f[d] = full_population<true, Q, d>(f[d]);
+        });
     // lambda_{t+1} at the slots this cell wrote (its store slots).
     const uint32_t off_n = n * static_cast<uint32_t>(sizeof(Real));
     const uint32_t rb = static_cast<uint32_t>(sizeof(Real));
@@ -291,7 +296,9 @@ __global__ __launch_bounds__(256) void drag_partial(const void* __restrict__ f, 
         static_for<Q - 1>([&](auto DD) {
             constexpr int d = decltype(DD)::value + 1;
             const int cd = a.drag_axis == 0 ? L::cx[d] : (a.drag_axis == 1 ? L::cy[d] : L::cz[d]);
-            if ((bits >> (d - 1)) & 1U) v += -2.0 * static_cast<double>(cd) * static_cast<double>(fin[d]);
+            if ((bits >> (d - 1)) & 1U)
+                v += -2.0 * static_cast<double>(cd) *
+                     static_cast<double>(full_population<Storage<T>::kShifted, Q, d>(fin[d]));
         });
     }
     sm[threadIdx.x] = v;
@@ -343,7 +350,8 @@ __global__ __launch_bounds__(1024) void plane_objective(const void* __restrict__
         load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
         Real rho = Real(0);
         static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
-        sum += static_cast<double>(rho) * (which == 0 ? a.c_in : a.c_out);
+        const double rho_d = Storage<T>::kShifted ? 1.0 + static_cast<double>(rho) : static_cast<double>(rho);
+        sum += rho_d * (which == 0 ? a.c_in : a.c_out);
     }
     sm[threadIdx.x] = sum;
     __syncthreads();
@@ -413,7 +421,8 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
                                    dim3(256), 0, st.stream, f, st.geo, *static_cast<const Physics<Real>*>(st.physics),
                                    a);
             });
-            hipLaunchKernelGGL(sum_partials, dim3(1), dim3(1024), 0, st.stream, a.scratch, list_blocks(a), weight, acc);
+            hipLaunchKernelGGL(sum_partials, dim3(1), dim3(1024), 0, st.stream, a.scratch, list_blocks(a),
+                               weight * a.c_drag, acc);
             check(hipGetLastError(), "drag objective launch");
             return;
         }
@@ -435,11 +444,13 @@ template <int Q> const AdjointOps* adj_for_storage(int storage) {
     static const AdjOps<Q, float> f32;
     static const AdjOps<Q, _Float16> f16;
     static const AdjOps<Q, double> f64;
+    static const AdjOps<Q, F32Dev> f32s;
     switch (storage) {
     case 0: return &f32;
     case 1: return &f16;
     case 2: return &f64;
-    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1) or FP64 (2)");
+    case 3: return &f32s;
+    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2) or FP32S (3)");
     }
 }
 

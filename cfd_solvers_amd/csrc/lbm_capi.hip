@@ -86,7 +86,8 @@ struct cfd_lbm_solver {
                    int storage_) {
         if (lattice != 9 && lattice != 19 && lattice != 27)
             throw std::invalid_argument("lattice must be D2Q9 (9), D3Q19 (19) or D3Q27 (27)");
-        if (storage_ < 0 || storage_ > 2) throw std::invalid_argument("storage must be FP32 (0), FP16S (1) or FP64 (2)");
+        if (storage_ < 0 || storage_ > 3)
+            throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2) or FP32S (3)");
         if (nx_ < 2 || ny_ < 2) throw std::invalid_argument("LBM grid requires nx, ny >= 2");
         if (lattice == 9) nz_ = 1;
         else if (nz_ < 2) throw std::invalid_argument("3-D LBM grid requires nz >= 2");
@@ -103,7 +104,7 @@ struct cfd_lbm_solver {
         ax = ax_;
         ay = ay_;
         az = az_;
-        elem = storage == 0 ? 4 : (storage == 1 ? 2 : 8);
+        elem = storage == 1 ? 2 : (storage == 2 ? 8 : 4);
         real = storage == 2 ? 8 : 4;
 
         // Launch geometry: 256-thread tiles, bx a power of two in [32, 256] covering x.
@@ -118,7 +119,7 @@ struct cfd_lbm_solver {
         // 256^3: FP32 D3Q19 is fastest at 5 waves (4 % over 8), FP32 D3Q27 at 4; FP16 wants all 8.
         // CFD_LBM_WAVES (3..8) overrides.
         long waves = 8;
-        if (storage == 0) waves = lattice == 27 ? 4 : 5;
+        if (storage == 0 || storage == 3) waves = lattice == 27 ? 4 : 5;
         if (const char* v = std::getenv("CFD_LBM_WAVES")) waves = std::atol(v);
         if (waves >= 3 && waves < 8) step_lds = static_cast<uint32_t>(163840 / waves - 1024);
         geo.nx = static_cast<uint32_t>(nx);

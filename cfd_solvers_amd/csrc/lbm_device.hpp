@@ -78,38 +78,68 @@ __device__ __forceinline__ bool is_wall(uint8_t f) { return (f & kWall) != 0; }
 __device__ __forceinline__ bool is_wall(uint32_t f) { return (f & kWall) != 0; }
 
 // ------------------------------------------------------------------ population storage
-// fp32: the value itself.  fp16s: binary16 of (f - w_d) * 4096 (round to nearest even, the
-// hardware conversion); the shift keeps small deviations from equilibrium in the normal range.
-// fp64: verification build with double collision arithmetic.
+// fp32: the value itself, collision in the reference's arithmetic.  fp64: verification build.
+// fp16s: binary16 of (f - w_d) * 4096 (round to nearest even, the hardware conversion).  The
+// state of an fp16s lattice is the deviation f' = f - w_d from the rest equilibrium, and the
+// collision works on deviations throughout (rho = 1 + sum f'; see collide): at low velocities the
+// non-equilibrium part f - f_eq is many orders below f itself, and forming it from full
+// populations in FP32 loses it to cancellation (a 1 % bias in a porous channel at u ~ 3e-4),
+// while the deviations keep it to FP32 relative precision.
 constexpr float kHalfScale = 4096.0F;
 constexpr float kHalfInvScale = 1.0F / 4096.0F;
 
 template <class T> struct Storage;
 template <> struct Storage<float> {
     using Real = float;
+    static constexpr bool kShifted = false;
     template <int Q, int D> __device__ __forceinline__ static float decode(float v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static float encode(float v) { return v; }
 };
 template <> struct Storage<_Float16> {
     using Real = float;
+    static constexpr bool kShifted = true;
     template <int Q, int D> __device__ __forceinline__ static float decode(_Float16 h) {
-        return static_cast<float>(h) * kHalfInvScale + Lattice<Q>::w[D];
+        return static_cast<float>(h) * kHalfInvScale;
     }
     template <int Q, int D> __device__ __forceinline__ static _Float16 encode(float v) {
-        return static_cast<_Float16>((v - Lattice<Q>::w[D]) * kHalfScale);
+        return static_cast<_Float16>(v * kHalfScale);
     }
+};
+// fp32s: FP32 storage of the deviation f - w_d, collision on deviations (the accuracy of the
+// shifted arithmetic at the FP32 layout's cost; fp32 keeps the reference's arithmetic bit for bit).
+struct F32Dev {
+    float v;
+};
+template <> struct Storage<F32Dev> {
+    using Real = float;
+    static constexpr bool kShifted = true;
+    template <int Q, int D> __device__ __forceinline__ static float decode(F32Dev h) { return h.v; }
+    template <int Q, int D> __device__ __forceinline__ static F32Dev encode(float v) { return F32Dev{v}; }
 };
 // Slot identifiers, used only by the index-map probe.
 template <> struct Storage<uint32_t> {
     using Real = uint32_t;
+    static constexpr bool kShifted = false;
     template <int Q, int D> __device__ __forceinline__ static uint32_t decode(uint32_t v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static uint32_t encode(uint32_t v) { return v; }
 };
 template <> struct Storage<double> {
     using Real = double;
+    static constexpr bool kShifted = false;
     template <int Q, int D> __device__ __forceinline__ static double decode(double v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static double encode(double v) { return v; }
 };
+
+// Population d of the state representation -> the population itself (adds w_d when shifted).
+template <bool Shifted, int Q, int D, class Real> __device__ __forceinline__ Real full_population(Real v) {
+    if constexpr (Shifted) return v + static_cast<Real>(Lattice<Q>::w[D]);
+    else return v;
+}
+// The population itself -> the state representation.
+template <bool Shifted, int Q, int D, class Real> __device__ __forceinline__ Real state_population(Real v) {
+    if constexpr (Shifted) return v - static_cast<Real>(Lattice<Q>::w[D]);
+    else return v;
+}
 
 // ------------------------------------------------------------------ launch geometry
 // A block is a bx*by = 256-thread tile of one z plane (bx spans x, multiple of 32).  Tiles are
@@ -218,6 +248,20 @@ __device__ __forceinline__ Real equilibrium(Real rho, Real ux, Real uy, Real uz,
     return static_cast<Real>(Lattice<Q>::w[D]) * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
 }
 
+// f_eq - w_d from rho' = rho - 1 and rho: w_d (rho' + rho (cu + cu^2/2 - uu)), no cancellation.
+template <int Q, int D, class Real>
+__device__ __forceinline__ Real equilibrium_dev(Real rp, Real rho, Real ux, Real uy, Real uz, Real uu) {
+    const Real cu = Real(3) * cdot<Q, D>(ux, uy, uz);
+    return static_cast<Real>(Lattice<Q>::w[D]) * (rp + rho * (cu + Real(0.5) * cu * cu - uu));
+}
+
+// Equilibrium in the state representation (rp = rho - 1, used when shifted).
+template <bool Shifted, int Q, int D, class Real>
+__device__ __forceinline__ Real equilibrium_state(Real rp, Real rho, Real ux, Real uy, Real uz, Real uu) {
+    if constexpr (Shifted) return equilibrium_dev<Q, D>(rp, rho, ux, uy, uz, uu);
+    else return equilibrium<Q, D>(rho, ux, uy, uz, uu);
+}
+
 template <int Q, int D, class Real>
 __device__ __forceinline__ Real guo(Real ux, Real uy, Real uz, Real fx, Real fy, Real fz, Real uf, Real wfac) {
     const Real cu = cdot<Q, D>(ux, uy, uz);
@@ -281,26 +325,41 @@ __device__ __forceinline__ Real inlet_velocity(const Physics<Real>& p, const Geo
     return u;
 }
 
-// Macroscopic state a non-wall cell reports (esoteric_pull.hpp:180-209 for fluid cells).
-template <int Q, class Real>
-__device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
-                                           uint32_t y, uint32_t z, const Real (&f)[Q], Real& rho, Real& ux, Real& uy,
-                                           Real& uz) {
-    if (flag == kInlet) {
-        const Real uin = inlet_velocity(p, g, y, z, Lattice<Q>::dims == 3);
-        rho = (sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, -1>(f)) / (Real(1) - uin);
-        ux = uin;
-        uy = uz = Real(0);
-        return;
+// Density of a velocity-inlet cell (Zou-He, from the populations arriving from inside) as
+// rho' = rho - 1 and rho.  Unshifted: rho = (S0 + 2 S-) / (1 - u) as in src/lbm/d2q9.cpp:152-168.
+// Shifted (S0 + 2 S- of the weights is 1): rho' = (S0' + 2 S-' + u) / (1 - u).
+template <bool Shifted, int Q, class Real>
+__device__ __forceinline__ void inlet_density(const Real (&f)[Q], Real uin, Real& rp, Real& rho) {
+    const Real s = sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, -1>(f);
+    if constexpr (Shifted) {
+        rp = (s + uin) / (Real(1) - uin);
+        rho = Real(1) + rp;
+    } else {
+        rho = s / (Real(1) - uin);
+        rp = rho - Real(1);
     }
-    if (flag == kOutlet) {
-        rho = p.rho_out;
-        ux = Real(-1) + (sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, 1>(f)) / p.rho_out;
-        uy = uz = Real(0);
-        return;
+}
+// Normal velocity of a pressure-outlet cell: -1 + (S0 + 2 S+) / rho_out (shifted: the weights'
+// part of the sum is 1, so u = (1 - rho_out + S0' + 2 S+') / rho_out).
+template <bool Shifted, int Q, class Real>
+__device__ __forceinline__ Real outlet_velocity(const Physics<Real>& p, const Real (&f)[Q]) {
+    const Real s = sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, 1>(f);
+    if constexpr (Shifted) return ((Real(1) - p.rho_out) + s) / p.rho_out;
+    else return Real(-1) + s / p.rho_out;
+}
+
+// Density (and rho' = rho - 1) and velocity m / rho of a fluid cell from its state.
+template <bool Shifted, int Q, class Real>
+__device__ __forceinline__ void fluid_moments(const Real (&f)[Q], Real& rp, Real& rho, Real& ux, Real& uy, Real& uz) {
+    Real mx, my, mz, s;
+    moments<Q>(f, s, mx, my, mz);
+    if constexpr (Shifted) {
+        rp = s;
+        rho = Real(1) + s;
+    } else {
+        rho = s;
+        rp = s - Real(1);
     }
-    Real mx, my, mz;
-    moments<Q>(f, rho, mx, my, mz);
     if (rho > Real(0)) {
         ux = mx / rho;
         uy = my / rho;
@@ -308,6 +367,29 @@ __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometr
     } else {
         ux = uy = uz = Real(0);
     }
+}
+
+// Macroscopic state a non-wall cell reports (esoteric_pull.hpp:180-209 for fluid cells).
+template <int Q, bool Shifted, class Real>
+__device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
+                                           uint32_t y, uint32_t z, const Real (&f)[Q], Real& rho, Real& ux, Real& uy,
+                                           Real& uz) {
+    if (flag == kInlet) {
+        const Real uin = inlet_velocity(p, g, y, z, Lattice<Q>::dims == 3);
+        Real rp;
+        inlet_density<Shifted, Q>(f, uin, rp, rho);
+        ux = uin;
+        uy = uz = Real(0);
+        return;
+    }
+    if (flag == kOutlet) {
+        rho = p.rho_out;
+        ux = outlet_velocity<Shifted, Q>(p, f);
+        uy = uz = Real(0);
+        return;
+    }
+    Real rp;
+    fluid_moments<Shifted, Q>(f, rp, rho, ux, uy, uz);
     Real ax = p.ax, ay = p.ay, az = p.az;
     if (p.lax) {
         ax = ax + p.lax[n];
@@ -319,34 +401,34 @@ __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometr
     uz = uz + Real(0.5) * az;
 }
 
-// Collision of one non-wall cell in registers.  Returns the density used for moving-wall
-// momentum corrections (src/lbm/d2q9.cpp:217-223 uses the fluid cell's density).
+// Collision of one non-wall cell in registers, on the state representation (populations, or
+// deviations when Shifted).  Returns the density used for moving-wall momentum corrections
+// (src/lbm/d2q9.cpp:217-223 uses the fluid cell's density).
 // Forced selects the Guo-forcing variant at compile time: the unforced variant carries no force
 // temporaries and fits twice the waves per SIMD.  With Forced, a zero acceleration still takes the
 // unforced arithmetic (the reference's guo_force returns 0 then), so both variants agree bitwise.
-template <int Q, bool Forced, class Real>
+template <int Q, bool Forced, bool Shifted, class Real>
 __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
                                         uint32_t y, uint32_t z, Real (&f)[Q]) {
     if (flag == kInlet || flag == kOutlet) {
-        Real rho, ux, uy, uz;
-        cell_state<Q>(p, g, flag, n, y, z, f, rho, ux, uy, uz);
-        const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
+        Real rp, rho, ux;
+        if (flag == kInlet) {
+            ux = inlet_velocity(p, g, y, z, Lattice<Q>::dims == 3);
+            inlet_density<Shifted, Q>(f, ux, rp, rho);
+        } else {
+            rho = p.rho_out;
+            rp = rho - Real(1);
+            ux = outlet_velocity<Shifted, Q>(p, f);
+        }
+        const Real uu = Real(1.5) * (ux * ux);
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            f[d] = equilibrium<Q, d>(rho, ux, uy, uz, uu);
+            f[d] = equilibrium_state<Shifted, Q, d>(rp, rho, ux, Real(0), Real(0), uu);
         });
         return rho;
     }
-    Real rho, mx, my, mz;
-    moments<Q>(f, rho, mx, my, mz);
-    Real ux, uy, uz;
-    if (rho > Real(0)) {
-        ux = mx / rho;
-        uy = my / rho;
-        uz = mz / rho;
-    } else {
-        ux = uy = uz = Real(0);
-    }
+    Real rp, rho, ux, uy, uz;
+    fluid_moments<Shifted, Q>(f, rp, rho, ux, uy, uz);
     Real ax = Real(0), ay = Real(0), az = Real(0);
     if constexpr (Forced) {
         ax = p.ax;
@@ -374,7 +456,7 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
     if (!Forced || (ax == Real(0) && ay == Real(0) && az == Real(0))) {
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            f[d] = f[d] - omega * (f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu));
+            f[d] = f[d] - omega * (f[d] - equilibrium_state<Shifted, Q, d>(rp, rho, ux, uy, uz, uu));
         });
     } else {
         const Real fx = rho * ax, fy = rho * ay, fz = rho * az;
@@ -382,13 +464,14 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
         const Real wfac = Real(1) - Real(0.5) * omega;
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            const Real post = f[d] - omega * (f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu));
+            const Real post = f[d] - omega * (f[d] - equilibrium_state<Shifted, Q, d>(rp, rho, ux, uy, uz, uu));
             f[d] = post + guo<Q, d>(ux, uy, uz, fx, fy, fz, uf, wfac);
         });
     }
     if (p.gamma) {
         // Partial bounce-back (Walsh, Burwinkle & Saar 2009): blend the BGK result with the
-        // pre-collision population of the opposite direction.
+        // pre-collision population of the opposite direction (w_d = w_opp(d): the blend is the
+        // same on deviations).
         const Real gm = p.gamma[n];
         const Real keep = Real(1) - gm;
         Real mixed[Q];

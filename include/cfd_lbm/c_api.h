@@ -36,11 +36,15 @@ typedef struct cfd_lbm_solver cfd_lbm_solver;
 /* Lattice ids are the velocity-set sizes q (descriptors.hpp:31-146). */
 enum { CFD_LBM_D2Q9 = 9, CFD_LBM_D3Q19 = 19, CFD_LBM_D3Q27 = 27 };
 
-/* Population storage.  Collision arithmetic is FP32 for FP32 and FP16S, FP64 for FP64. */
+/* Population storage.  Collision arithmetic is FP32 except for FP64.  FP32 follows the
+ * reference's arithmetic bit for bit; the shifted storages hold the deviation f - w_d and collide
+ * on deviations (rho = 1 + sum f'), which keeps the non-equilibrium part at FP32 precision at low
+ * velocities. */
 enum {
-    CFD_LBM_STORE_FP32 = 0,  /* 4 B / population (the reference's storage)                */
-    CFD_LBM_STORE_FP16S = 1, /* 2 B / population: binary16 of (f - w_d) * 4096, RNE       */
-    CFD_LBM_STORE_FP64 = 2   /* 8 B / population: verification build of the same kernels  */
+    CFD_LBM_STORE_FP32 = 0,  /* 4 B / population (the reference's storage and arithmetic)  */
+    CFD_LBM_STORE_FP16S = 1, /* 2 B / population: binary16 of (f - w_d) * 4096, RNE        */
+    CFD_LBM_STORE_FP64 = 2,  /* 8 B / population: verification build of the same kernels   */
+    CFD_LBM_STORE_FP32S = 3  /* 4 B / population: f - w_d in FP32                          */
 };
 
 /* uint8 cell flags (cfd_lbm_set_flags). */

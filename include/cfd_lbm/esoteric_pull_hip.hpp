@@ -35,6 +35,7 @@ enum class HipPopulationStorage : int {
     fp32 = CFD_LBM_STORE_FP32,
     fp16_shifted = CFD_LBM_STORE_FP16S,
     fp64 = CFD_LBM_STORE_FP64,
+    fp32_shifted = CFD_LBM_STORE_FP32S,
 };
 
 struct HipMacroscopicFields {

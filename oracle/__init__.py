@@ -19,7 +19,7 @@ ORACLE_SO = HERE / "liblbm_oracle.so"
 REF_SO = HERE / "_ref" / "libref_lbm.so"
 REF_NATIVE_SO = HERE / "_ref" / "libref_lbm_native.so"
 
-STORAGE = {"fp32": 0, "fp16s": 1, "fp64": 2}
+STORAGE = {"fp32": 0, "fp16s": 1, "fp64": 2, "fp32s": 3}
 FLUID, INLET, OUTLET, WALL = 0x00, 0x01, 0x02, 0x80
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

@@ -34,6 +34,7 @@
 #include <limits>
 #include <memory>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 namespace orc {
@@ -139,9 +140,19 @@ template <class Real, int Q> struct Codec<Real, Real, Q> {
     static Real load(Real v, int) { return v; }
     static Real store(Real v, int) { return v; }
 };
+// FP32 storage of the deviation f - w_d (shifted, like FP16S, at 4 B per population).
+struct F32Dev {
+    float v;
+};
+template <int Q> struct Codec<F32Dev, float, Q> {
+    static float load(F32Dev h, int) { return h.v; }
+    static F32Dev store(float v, int) { return F32Dev{v}; }
+};
+// FP16 shifted storage holds binary16((f - w_d) * 4096); its state is the deviation f - w_d and
+// the collision runs on deviations (Shifted below), so the codec only scales.
 template <int Q> struct Codec<Half, float, Q> {
-    static float load(Half h, int d) { return f16_to_f32(h.bits) * kHalfInvScale + Lat<Q>::w[d]; }
-    static Half store(float v, int d) { return Half{f32_to_f16_rne((v - Lat<Q>::w[d]) * kHalfScale)}; }
+    static float load(Half h, int) { return f16_to_f32(h.bits) * kHalfInvScale; }
+    static Half store(float v, int) { return Half{f32_to_f16_rne(v * kHalfScale)}; }
 };
 
 // ---------------------------------------------------------------- seeded noise (SplitMix64)
@@ -193,6 +204,12 @@ public:
 template <int Q, class Real, class Store> class Oracle final : public OracleBase {
     using L = Lat<Q>;
     using C = Codec<Store, Real, Q>;
+    // State = deviation f - w_d from the rest equilibrium (FP16 shifted storage).  The collision
+    // then works on deviations: rho = 1 + sum f', f_eq - w = w (rho' + rho (cu + cu^2/2 - uu)),
+    // which keeps the non-equilibrium part at FP32 relative precision at low velocities.
+    static constexpr bool kShifted = std::is_same<Store, Half>::value || std::is_same<Store, F32Dev>::value;
+    static Real to_full(Real v, int d) { return kShifted ? v + w(d) : v; }
+    static Real to_state(Real v, int d) { return kShifted ? v - w(d) : v; }
 
 public:
     Oracle(size_t nx, size_t ny, size_t nz, double tau, double ax, double ay, double az)
@@ -267,11 +284,12 @@ public:
             const Real ux = solid ? Real(0) : s[1];
             const Real uy = solid ? Real(0) : s[2];
             const Real uz = solid ? Real(0) : s[3];
-            at(0, n) = C::store(feq(0, s[0], ux, uy, uz), 0);
+            const Real rp = s[0] - Real(1);
+            at(0, n) = C::store(feq_state(0, rp, s[0], ux, uy, uz), 0);
             for (int i = 1; i < Q; i += 2) {
                 const size_t nb = neighbor(x, y, z, i + 1);
-                at(i, nb) = C::store(feq(i, s[0], ux, uy, uz), i);
-                at(i + 1, n) = C::store(feq(i + 1, s[0], ux, uy, uz), i + 1);
+                at(i, nb) = C::store(feq_state(i, rp, s[0], ux, uy, uz), i);
+                at(i + 1, n) = C::store(feq_state(i + 1, rp, s[0], ux, uy, uz), i + 1);
             }
         }
     }
@@ -351,7 +369,7 @@ public:
             load(n, fin);
             Real rho = Real(0);
             for (int d = 0; d < Q; ++d) rho += fin[d];
-            total += static_cast<double>(rho);
+            total += kShifted ? 1.0 + static_cast<double>(rho) : static_cast<double>(rho);
         }
         return total;
     }
@@ -385,7 +403,7 @@ public:
         for (size_t n = 0; n < N_; ++n) {
             Real fin[Q];
             load(n, fin);
-            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = static_cast<double>(fin[d]);
+            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = static_cast<double>(to_full(fin[d], d));
         }
     }
     // Writes logical populations (the inverse of load) at the current parity.
@@ -394,11 +412,11 @@ public:
         for (size_t n = 0; n < N_; ++n) {
             size_t x, y, z;
             coords(n, x, y, z);
-            at(0, n) = C::store(static_cast<Real>(in[n]), 0);
+            at(0, n) = C::store(to_state(static_cast<Real>(in[n]), 0), 0);
             for (int i = 1; i < Q; i += 2) {
                 const size_t nb = neighbor(x, y, z, i + 1);
-                const Real a = static_cast<Real>(in[static_cast<size_t>(i) * N_ + n]);
-                const Real b = static_cast<Real>(in[static_cast<size_t>(i + 1) * N_ + n]);
+                const Real a = to_state(static_cast<Real>(in[static_cast<size_t>(i) * N_ + n]), i);
+                const Real b = to_state(static_cast<Real>(in[static_cast<size_t>(i + 1) * N_ + n]), i + 1);
                 if (odd) {
                     at(i + 1, nb) = C::store(a, i + 1);
                     at(i, n) = C::store(b, i);
@@ -410,7 +428,10 @@ public:
         }
     }
     void raw(double* out) const override {
-        for (size_t k = 0; k < F_.size(); ++k) out[k] = static_cast<double>(C::load(F_[k], static_cast<int>(k / N_)));
+        for (size_t k = 0; k < F_.size(); ++k) {
+            const int d = static_cast<int>(k / N_);
+            out[k] = static_cast<double>(to_full(C::load(F_[k], d), d));
+        }
     }
 
     // Slot (d' * N + n') from which fin[d] of cell n is read at the given parity (esoteric_pull.hpp:349-365).
@@ -511,6 +532,16 @@ private:
         const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
         return w(d) * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
     }
+    // f_eq - w_d = w_d (rho' + rho (cu + cu^2/2 - uu)), rho' = rho - 1.
+    static Real feq_dev(int d, Real rp, Real rho, Real ux, Real uy, Real uz) {
+        const Real cu = Real(3) * (static_cast<Real>(L::cx[d]) * ux + static_cast<Real>(L::cy[d]) * uy +
+                                   static_cast<Real>(L::cz[d]) * uz);
+        const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
+        return w(d) * (rp + rho * (cu + Real(0.5) * cu * cu - uu));
+    }
+    static Real feq_state(int d, Real rp, Real rho, Real ux, Real uy, Real uz) {
+        return kShifted ? feq_dev(d, rp, rho, ux, uy, uz) : feq(d, rho, ux, uy, uz);
+    }
     static Real guo(int d, Real rho, Real ux, Real uy, Real uz, Real ax, Real ay, Real az, Real omega) {
         if (ax == Real(0) && ay == Real(0) && az == Real(0)) return Real(0);
         const Real cx = static_cast<Real>(L::cx[d]), cy = static_cast<Real>(L::cy[d]), cz = static_cast<Real>(L::cz[d]);
@@ -552,13 +583,26 @@ private:
     }
 
     static void accumulate(const Real* fin, Real& rho, Real& ux, Real& uy, Real& uz) {
-        rho = ux = uy = uz = Real(0);
+        Real rp;
+        accumulate(fin, rp, rho, ux, uy, uz);
+    }
+    // rho' = rho - 1 as well; shifted states sum to rho'.
+    static void accumulate(const Real* fin, Real& rp, Real& rho, Real& ux, Real& uy, Real& uz) {
+        Real s = Real(0);
+        ux = uy = uz = Real(0);
         for (int d = 0; d < Q; ++d) {
             const Real v = fin[d];
-            rho += v;
+            s += v;
             ux += v * static_cast<Real>(L::cx[d]);
             uy += v * static_cast<Real>(L::cy[d]);
             uz += v * static_cast<Real>(L::cz[d]);
+        }
+        if (kShifted) {
+            rp = s;
+            rho = Real(1) + s;
+        } else {
+            rho = s;
+            rp = s - Real(1);
         }
         if (rho > Real(0)) {
             ux /= rho;
@@ -586,13 +630,25 @@ private:
         return u;
     }
     // Zou-He density of a velocity-inlet cell at x = 0 from the populations that come from the interior.
+    // Shifted: the weights' share of S0 + 2 S- is 1, so rho' = (S0' + 2 S-' + u) / (1 - u).
     Real inlet_density(const Real* fin, Real uin) const {
+        Real rp;
+        return inlet_density(fin, uin, rp);
+    }
+    Real inlet_density(const Real* fin, Real uin, Real& rp) const {
         Real s0 = Real(0), sn = Real(0);
         for (int d = 0; d < Q; ++d) {
             if (L::cx[d] == 0) s0 += fin[d];
             else if (L::cx[d] < 0) sn += fin[d];
         }
-        return (s0 + Real(2) * sn) / (Real(1) - uin);
+        const Real s = s0 + Real(2) * sn;
+        if (kShifted) {
+            rp = (s + uin) / (Real(1) - uin);
+            return Real(1) + rp;
+        }
+        const Real rho = s / (Real(1) - uin);
+        rp = rho - Real(1);
+        return rho;
     }
     // Zou-He normal velocity of a pressure-outlet cell at x = nx-1.
     Real outlet_velocity(const Real* fin) const {
@@ -601,6 +657,7 @@ private:
             if (L::cx[d] == 0) s0 += fin[d];
             else if (L::cx[d] > 0) sp += fin[d];
         }
+        if (kShifted) return ((Real(1) - rho_out_) + (s0 + Real(2) * sp)) / rho_out_;
         return Real(-1) + (s0 + Real(2) * sp) / rho_out_;
     }
 
@@ -609,17 +666,19 @@ private:
         const uint8_t flag = flags_.empty() ? kFluid : flags_[n];
         if (flag == kInlet) {
             const Real uin = inlet_velocity(y, z);
-            const Real rho = inlet_density(fin, uin);
-            for (int d = 0; d < Q; ++d) fout[d] = feq(d, rho, uin, Real(0), Real(0));
+            Real rp;
+            const Real rho = inlet_density(fin, uin, rp);
+            for (int d = 0; d < Q; ++d) fout[d] = feq_state(d, rp, rho, uin, Real(0), Real(0));
             return rho;
         }
         if (flag == kOutlet) {
             const Real ux = outlet_velocity(fin);
-            for (int d = 0; d < Q; ++d) fout[d] = feq(d, rho_out_, ux, Real(0), Real(0));
+            const Real rp = rho_out_ - Real(1);
+            for (int d = 0; d < Q; ++d) fout[d] = feq_state(d, rp, rho_out_, ux, Real(0), Real(0));
             return rho_out_;
         }
-        Real rho, ux, uy, uz, ax, ay, az;
-        accumulate(fin, rho, ux, uy, uz);
+        Real rp, rho, ux, uy, uz, ax, ay, az;
+        accumulate(fin, rp, rho, ux, uy, uz);
         acceleration(n, ax, ay, az);
         ux += Real(0.5) * ax;
         uy += Real(0.5) * ay;
@@ -627,7 +686,7 @@ private:
         const Real omega = Real(1) / tau_;
         for (int d = 0; d < Q; ++d) {
             Real p = fin[d];
-            p -= omega * (p - feq(d, rho, ux, uy, uz));
+            p -= omega * (p - feq_state(d, rp, rho, ux, uy, uz));
             p += guo(d, rho, ux, uy, uz, ax, ay, az, omega);
             fout[d] = p;
         }
@@ -857,7 +916,7 @@ private:
         for (size_t n = 0; n < N_; ++n) {
             Real fin[Q];
             load(n, fin);
-            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = fin[d];
+            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = to_full(fin[d], d);
         }
     }
 
@@ -1009,7 +1068,7 @@ private:
         if (is_wall(flag)) {
             Real rho = Real(0);
             for (int d = 0; d < Q; ++d) rho += fin[d];
-            m[0] = rho;
+            m[0] = kShifted ? Real(1) + rho : rho;
             m[1] = m[2] = m[3] = Real(0);
             return;
         }
@@ -1044,7 +1103,8 @@ template <int Q> OracleBase* make_q(int storage, size_t nx, size_t ny, size_t nz
     case 0: return new Oracle<Q, float, float>(nx, ny, nz, tau, ax, ay, az);
     case 1: return new Oracle<Q, float, Half>(nx, ny, nz, tau, ax, ay, az);
     case 2: return new Oracle<Q, double, double>(nx, ny, nz, tau, ax, ay, az);
-    default: throw std::invalid_argument("storage must be 0 (fp32), 1 (fp16 shifted) or 2 (fp64)");
+    case 3: return new Oracle<Q, float, F32Dev>(nx, ny, nz, tau, ax, ay, az);
+    default: throw std::invalid_argument("storage must be 0 (fp32), 1 (fp16 shifted), 2 (fp64) or 3 (fp32 shifted)");
     }
 }
 

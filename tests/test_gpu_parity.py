@@ -301,11 +301,14 @@ def test_local_acceleration_parity(hip, oracle_mod):
 
 
 # ---------------------------------------------------------------------- storage precisions
+@pytest.mark.parametrize("storage", ["fp16s", "fp32s"])
 @pytest.mark.parametrize("bounded", [False, True])
-def test_fp16_storage_parity(hip, oracle_mod, bounded):
-    report("fp16", bounded=bounded)
+def test_shifted_storage_parity(hip, oracle_mod, storage, bounded):
+    """Shifted storages (state f - w_d, collision on deviations): fields and logical populations
+    bit for bit against the oracle's restatement of the same arithmetic."""
+    report("shifted", storage=storage, bounded=bounded)
     n = (20, 12, 10)
-    g, o = make_pair(hip, oracle_mod, 19, n, 0.6, (1e-5, 0, 0), storage="fp16s")
+    g, o = make_pair(hip, oracle_mod, 19, n, 0.6, (1e-5, 0, 0), storage=storage)
     if bounded:
         flags = channel_flags(*n, np.random.default_rng(2), 0.05)
         g.set_flags(flags)
@@ -315,6 +318,36 @@ def test_fp16_storage_parity(hip, oracle_mod, bounded):
     g.step(25)
     o.step(25)
     assert_fields(gpu_fields(g), o.macroscopic())
+    gl = g.logical_populations().astype(np.float64)
+    ol = o.logical().reshape(gl.shape)
+    live = np.ones(gl.shape[1], bool) if not bounded else flags == 0
+    assert np.array_equal(gl[:, live], ol[:, live])
+
+
+@pytest.mark.parametrize("storage", ["fp16s", "fp32s"])
+def test_shifted_boundaries_parity(hip, oracle_mod, storage):
+    """Inlet, outlet, partial bounce-back and a moving wall on shifted storage, bit for bit."""
+    report("shifted_boundaries", storage=storage)
+    rng = np.random.default_rng(6)
+    nx, ny, nz = 24, 12, 10
+    f = channel_flags(nx, ny, nz).reshape(nz, ny, nx)
+    f[:, :, 0][f[:, :, 0] == 0] = 0x01
+    f[:, :, -1][f[:, :, -1] == 0] = 0x02
+    f[-1, 1:-1, 1:-1] = 0x81  # top wall moves along x
+    flags = f.reshape(-1)
+    gamma = np.where(flags == 0, rng.uniform(0.0, 0.2, flags.size), 0.0).astype(np.float32)
+    g, o = make_pair(hip, oracle_mod, 19, (nx, ny, nz), 0.6, storage=storage)
+    for s in (g, o):
+        s.set_flags(flags)
+        s.set_inlet_outlet(0.05, 1.0)
+        s.set_wall_velocity(1, (0.03, 0.0, 0.0))
+    g.set_porosity(gamma)
+    o.set_gamma(gamma.astype(np.float64))
+    g.initialize_uniform()
+    o.init_uniform()
+    g.step(40)
+    o.step(40)
+    assert_fields(gpu_fields(g), o.macroscopic(), mask=flags != 0x81)
 
 
 @pytest.mark.parametrize("q", [19, 27])
