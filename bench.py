@@ -1,12 +1,13 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,d3q27,adjoint[,fp64,config5]] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint[,config5]] [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
 u = (0.1, 0, 0), tau 0.55, uint8 flags) with FP32 population storage; further legs run the same
-geometry with FP16 shifted-population storage and with D3Q27, and the adjoint leg runs the
+geometry with FP16 and FP32 shifted-population storage, FP64 (the reference for the velocity
+error of the others) and D3Q27, and the adjoint leg runs the
 checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps).  The opt-in
 config5 leg runs forward + adjoint of a 768^3 FP16 duct past a sphere cluster (drag objective,
 binomial checkpointing) against the FP32-storage run.  Prints progress on stderr and ONE JSON
@@ -67,6 +68,10 @@ def run_leg(hip, name, lattice, storage, n, tau, steps, warmup):
     s.wait()
     wall = time.perf_counter() - t0
     mass1, ke, vmax = s.diagnostics()
+    m = s.download_macroscopic()
+    fluid_mask = flags == 0
+    velocity = np.stack([m.ux.reshape(-1)[fluid_mask], m.uy.reshape(-1)[fluid_mask],
+                         m.uz.reshape(-1)[fluid_mask]]).astype(np.float64)
     cells = n ** 3
     fluid = int(np.count_nonzero(flags == 0))
     bytes_launch = fluid * 2 * q * BYTES_PER_POP[storage] + cells * 1
@@ -85,7 +90,7 @@ def run_leg(hip, name, lattice, storage, n, tau, steps, warmup):
     log(f"leg {name}: {res['mlups']:.0f} MLUPS, kernel {res['kernel_ms']:.3f} ms, {gbs:.0f} GB/s "
         f"({100 * gbs / PEAK_GBS:.1f}% of 8 TB/s), mass drift {res['mass_rel_drift']:.2e}")
     s.close()
-    return res
+    return res, velocity
 
 
 def duct_flags(n):
@@ -261,7 +266,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--legs", default="fp32,fp16,d3q27,adjoint")
+    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint")
     ap.add_argument("--config5-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
@@ -276,7 +281,8 @@ def main():
     legs = {}
     wanted = [l.strip() for l in args.legs.split(",") if l.strip()]
     table = {"fp32": ("d3q19", "fp32"), "fp16": ("d3q19", "fp16s"), "d3q27": ("d3q27", "fp32"),
-             "fp64": ("d3q19", "fp64")}
+             "fp64": ("d3q19", "fp64"), "fp32s": ("d3q19", "fp32s")}
+    velocities = {}
     for name in wanted:
         if name == "adjoint":
             legs[name] = adjoint_leg(hip)
@@ -285,7 +291,18 @@ def main():
             legs[name] = config5_leg(hip, steps=args.config5_steps)
             continue
         lattice, storage = table[name]
-        legs[name] = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
+        legs[name], vel = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
+        if lattice == "d3q19":
+            velocities[name] = vel
+    # Relative L2 error of the velocity field against the FP64 build after the same steps.
+    if "fp64" in velocities:
+        ref = velocities["fp64"]
+        for name, vel in velocities.items():
+            if name != "fp64":
+                err = float(np.sqrt(np.sum((vel - ref) ** 2) / np.sum(ref ** 2)))
+                legs[name]["velocity_rel_l2_vs_fp64"] = err
+                log(f"leg {name}: velocity relative L2 error vs FP64 {err:.2e}")
+    velocities.clear()
     probe = None if args.no_probe else copy_probe()
     head = legs.get("fp32") or next(v for k, v in legs.items() if "mlups" in v)
 

@@ -10,6 +10,11 @@ caller's interface here: the forward steps, the transposed steps and the checkpo
 
 The reference has no differentiable solver; the gradients are checked against central finite
 differences and the FP64 build of the same adjoint.
+
+One HIP runtime per process: torch's wheel bundles its own ``libamdhip64.so.7``.  Import torch
+before the library is first used (importing this module does so); the library then binds to
+torch's runtime by soname and both share the device.  A process that has already run the library
+on the GPU under another runtime cannot start torch's afterwards ("No HIP GPUs are available").
 """
 from __future__ import annotations
 

@@ -325,17 +325,18 @@ __global__ __launch_bounds__(1024) void sum_partials(const double* __restrict__ 
     if (threadIdx.x == 0) *acc += weight * sm[0];
 }
 
-// j(f) = sum_{x = x_in} rho * c_in + sum_{x = x_out} rho * c_out over fluid cells, in FP64, by one
-// block in a fixed order (deterministic); acc += weight * j.
+// Per-block partial sums of j(f) = sum_{x = x_in} rho * c_in + sum_{x = x_out} rho * c_out over fluid
+// cells, in FP64 (one thread per cell of the two planes; fixed tree order).
 template <int Q, class T, bool Odd, bool Bounded>
-__global__ __launch_bounds__(1024) void plane_objective(const void* __restrict__ f, const Geometry g,
-                                                        const Physics<typename Storage<T>::Real> p,
-                                                        const AdjointArgs a, double weight, double* acc) {
+__global__ __launch_bounds__(256) void plane_partial(const void* __restrict__ f, const Geometry g,
+                                                     const Physics<typename Storage<T>::Real> p,
+                                                     const AdjointArgs a) {
     using Real = typename Storage<T>::Real;
-    __shared__ double sm[1024];
+    __shared__ double sm[256];
     const uint32_t per_plane = g.ny * g.nz;
-    double sum = 0.0;
-    for (uint32_t k = threadIdx.x; k < 2U * per_plane; k += blockDim.x) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (k < 2U * per_plane) {
         const uint32_t which = k / per_plane;
         const uint32_t r = k % per_plane;
         const uint32_t y = r % g.ny, z = r / g.ny;
@@ -344,22 +345,23 @@ __global__ __launch_bounds__(1024) void plane_objective(const void* __restrict__
         const uint32_t n = h.cell(0, 0, 0);
         bool fluid = true;
         if constexpr (Bounded) fluid = ((p.links[n] >> kTypeShift) & 3U) == 0U;
-        if (!fluid) continue;
-        Real fin[Q];
-        PairSlots<Q> s;
-        load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
-        Real rho = Real(0);
-        static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
-        const double rho_d = Storage<T>::kShifted ? 1.0 + static_cast<double>(rho) : static_cast<double>(rho);
-        sum += rho_d * (which == 0 ? a.c_in : a.c_out);
+        if (fluid) {
+            Real fin[Q];
+            PairSlots<Q> s;
+            load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+            Real rho = Real(0);
+            static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
+            const double rho_d = Storage<T>::kShifted ? 1.0 + static_cast<double>(rho) : static_cast<double>(rho);
+            v = rho_d * (which == 0 ? a.c_in : a.c_out);
+        }
     }
-    sm[threadIdx.x] = sum;
+    sm[threadIdx.x] = v;
     __syncthreads();
-    for (uint32_t w = 512; w > 0; w >>= 1) {
+    for (uint32_t w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *acc += weight * sm[0];
+    if (threadIdx.x == 0) a.scratch[blockIdx.x] = sm[0];
 }
 
 __global__ __launch_bounds__(256) void fill_zero_f64(double* __restrict__ v, uint64_t n) {
@@ -377,6 +379,7 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
     static dim3 grid(const Geometry& g) { return dim3(g.tiles, 1, 1); }
 
     static unsigned list_blocks(const AdjointArgs& a) { return (a.drag_count + 255U) / 256U; }
+    static unsigned plane_blocks(const Geometry& g) { return (2U * g.ny * g.nz + 255U) / 256U; }
 
     // The objective's dj/df source added onto lam at parity odd (the drag part; the pressure-drop
     // part of adjoint_step is fused into the transposed step itself).
@@ -426,12 +429,13 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
             check(hipGetLastError(), "drag objective launch");
             return;
         }
+        const unsigned blocks = plane_blocks(st.geo);
         per_parity(odd, bounded, [&](auto O, auto B) {
-            hipLaunchKernelGGL((plane_objective<Q, T, decltype(O)::value, decltype(B)::value>), dim3(1), dim3(1024),
-                               0, st.stream, f, st.geo, *static_cast<const Physics<Real>*>(st.physics), a, weight,
-                               acc);
+            hipLaunchKernelGGL((plane_partial<Q, T, decltype(O)::value, decltype(B)::value>), dim3(blocks), dim3(256),
+                               0, st.stream, f, st.geo, *static_cast<const Physics<Real>*>(st.physics), a);
         });
-        check(hipGetLastError(), "plane_objective launch");
+        hipLaunchKernelGGL(sum_partials, dim3(1), dim3(1024), 0, st.stream, a.scratch, blocks, weight, acc);
+        check(hipGetLastError(), "pressure-drop objective launch");
     }
     void zero(const LaunchState& st, double* v, uint64_t n) const override {
         hipLaunchKernelGGL(fill_zero_f64, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st.stream, v,

@@ -22,7 +22,7 @@ struct AdjointArgs {
     uint32_t drag_count;
     int drag_axis;
     double c_drag;
-    double* scratch;      // ceil(drag_count / 256) doubles for the drag reduction
+    double* scratch;      // per-block partial sums: objective_scratch_doubles() of them
     int add_objective;    // adjoint_step: add dj/df_t for this step's state
     double* grad_gamma;   // per cell, accumulated
     double* grad_peak;    // per cell (inlet cells), accumulated
@@ -42,5 +42,12 @@ struct AdjointOps {
 };
 
 const AdjointOps* adjoint_ops(int q, int storage);
+
+// Doubles of AdjointArgs::scratch the objective reductions need on a lattice (drag_count: cells of
+// the drag list, 0 if none).
+inline uint64_t objective_scratch_doubles(const Geometry& g, uint32_t drag_count) {
+    const uint64_t plane = (2ULL * g.ny * g.nz + 255U) / 256U, drag = (drag_count + 255ULL) / 256U;
+    return plane > drag ? plane : drag;
+}
 
 }  // namespace cfd_hip

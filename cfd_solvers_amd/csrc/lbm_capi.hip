@@ -415,14 +415,13 @@ struct cfd_lbm_solver {
 
     // Drag objective: the fluid cells next to the registered body and, per cell, the populations
     // that arrive from a body cell (bit d-1 for population d).
-    DeviceBuffer<unsigned char> drag_cells, drag_links, drag_scratch;
+    DeviceBuffer<unsigned char> drag_cells, drag_links, objective_scratch;
     uint32_t drag_count = 0;
     int drag_axis = -1;  // -1: no body registered
 
     void clear_drag_body() {
         drag_cells.reset();
         drag_links.reset();
-        drag_scratch.reset();
         drag_count = 0;
         drag_axis = -1;
     }
@@ -473,7 +472,6 @@ struct cfd_lbm_solver {
         clear_drag_body();
         drag_cells.allocate(list.size() * 4);
         drag_links.allocate(bits.size() * 4);
-        drag_scratch.allocate((list.size() + 255) / 256 * sizeof(double));
         h2d(drag_cells.ptr, list.data(), list.size() * 4);
         h2d(drag_links.ptr, bits.data(), bits.size() * 4);
         sync();
@@ -490,14 +488,18 @@ struct cfd_lbm_solver {
         args.drag_count = drag_count;
         args.drag_axis = drag_axis;
         args.c_drag = 1.0 / static_cast<double>(window);
-        args.scratch = reinterpret_cast<double*>(drag_scratch.ptr);
         return args;
     }
 
-    cfd_hip::AdjointArgs objective_args(int kind, int x_in, int x_out, uint64_t window) const {
-        if (kind == cfd_hip::kPressureDrop) return pressure_args(x_in, x_out, window);
-        if (kind == cfd_hip::kDrag) return drag_args(window);
-        throw std::invalid_argument("objective must be 0 (pressure drop) or 1 (drag)");
+    cfd_hip::AdjointArgs objective_args(int kind, int x_in, int x_out, uint64_t window) {
+        cfd_hip::AdjointArgs args{};
+        if (kind == cfd_hip::kPressureDrop) args = pressure_args(x_in, x_out, window);
+        else if (kind == cfd_hip::kDrag) args = drag_args(window);
+        else throw std::invalid_argument("objective must be 0 (pressure drop) or 1 (drag)");
+        const size_t need = cfd_hip::objective_scratch_doubles(geo, drag_count) * sizeof(double);
+        if (objective_scratch.bytes < need) objective_scratch.allocate(need);
+        args.scratch = reinterpret_cast<double*>(objective_scratch.ptr);
+        return args;
     }
 
     // j of the current state (FP64).

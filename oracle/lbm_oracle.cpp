@@ -896,7 +896,7 @@ public:
                 coords(n, x, y, z);
                 double f[Q], lo[Q], mu[Q];
                 for (int d = 0; d < Q; ++d) {
-                    f[d] = static_cast<double>(fs[static_cast<size_t>(d) * N_ + n]);
+                    f[d] = static_cast<double>(to_full(fs[static_cast<size_t>(d) * N_ + n], d));
                     lo[d] = lam_out[static_cast<size_t>(d) * N_ + n];
                 }
                 double dgamma = 0.0, dpeak = 0.0;
@@ -912,11 +912,12 @@ public:
     }
 
 private:
+    // Loaded populations in the state representation (deviations for shifted storage).
     void logical_state(Real* out) const {
         for (size_t n = 0; n < N_; ++n) {
             Real fin[Q];
             load(n, fin);
-            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = to_full(fin[d], d);
+            for (int d = 0; d < Q; ++d) out[static_cast<size_t>(d) * N_ + n] = fin[d];
         }
     }
 
@@ -937,7 +938,8 @@ private:
         double s = 0.0;
         for (size_t n = 0; n < N_; ++n)
             for (int d = 1; d < Q; ++d)
-                if (body_link(n, d)) s += -2.0 * c_axis(d) * static_cast<double>(fin[static_cast<size_t>(d) * N_ + n]);
+                if (body_link(n, d))
+                    s += -2.0 * c_axis(d) * static_cast<double>(to_full(fin[static_cast<size_t>(d) * N_ + n], d));
         return s;
     }
 
@@ -950,11 +952,12 @@ private:
             if (x != x_in && x != x_out) continue;
             Real rho = Real(0);
             for (int d = 0; d < Q; ++d) rho += fin[static_cast<size_t>(d) * N_ + n];
+            const double r = kShifted ? 1.0 + static_cast<double>(rho) : static_cast<double>(rho);
             if (x == x_in) {
-                s_in += static_cast<double>(rho);
+                s_in += r;
                 ++c_in;
             } else {
-                s_out += static_cast<double>(rho);
+                s_out += r;
                 ++c_out;
             }
         }

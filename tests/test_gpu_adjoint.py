@@ -260,37 +260,72 @@ def test_fp16_pressure_adjoint_matches_oracle(hip, oracle_mod):
     assert du == pytest.approx(duo, rel=2e-4)
 
 
-def test_torch_autograd_function(hip):
-    """LbmObjective: J.backward() returns the adjoint gradients, and a directional central
-    difference of J along a random design perturbation matches <dJ/dgamma, v> (FP64 storage)."""
-    import torch
+TORCH_CHILD = r"""
+import json, sys
+import numpy as np
+import torch  # first: the library then shares torch's HIP runtime
 
-    from cfd_solvers_amd.autograd import LbmAdjointProblem, lbm_objective
+torch.ones(1, device="cuda:0").sum().item()
+sys.path.insert(0, sys.argv[1])
+import cfd_solvers_amd as hip
+from cfd_solvers_amd.autograd import LbmAdjointProblem, lbm_objective
+
+n = (16, 8, 8)
+f = np.zeros((8, 8, 16), np.uint8)
+f[:, 0, :] = f[:, -1, :] = 0x80
+f[0] = f[-1] = 0x80
+f[1:-1, 1:-1, 0] = 0x01
+f[1:-1, 1:-1, -1] = 0x02
+flags = f.reshape(-1)
+rng = np.random.default_rng(4)
+blend = np.where(flags == 0, 0.05 * rng.uniform(0.3, 0.7, flags.size), 0.0).astype(np.float32)
+prob = LbmAdjointProblem(flags=flags, shape=n, storage="fp64", steps=40, window=8, checkpoint_every=0,
+                         snapshots=4, x_in=1, x_out=14)
+dev = torch.device("cuda:0")
+gamma = torch.tensor(blend, device=dev, requires_grad=True)
+u_peak = torch.tensor(0.05, dtype=torch.float32, device=dev, requires_grad=True)
+J = lbm_objective(gamma, u_peak, prob)
+J.backward()
+s = hip.EsotericPullHipSolver(19, *n, 0.6, storage="fp64")
+s.set_flags(flags)
+s.set_inlet_outlet(0.05, 1.0)
+s.set_porosity(blend)
+s.initialize_uniform()
+ref_J, ref_grad, ref_du = s.adjoint_run(40, 8, 1, 1, 14)
+v = np.where(flags == 0, np.random.default_rng(12).standard_normal(flags.size), 0.0)
+gp = (blend + 1e-3 * 0.05 * v).astype(np.float32)
+gm = (blend - 1e-3 * 0.05 * v).astype(np.float32)
+with torch.no_grad():
+    Jp = lbm_objective(torch.tensor(gp, device=dev), u_peak.detach(), prob).item()
+    Jm = lbm_objective(torch.tensor(gm, device=dev), u_peak.detach(), prob).item()
+print(json.dumps({
+    "J": J.item(), "ref_J": ref_J, "grad_device": str(gamma.grad.device),
+    "grad_err": float((gamma.grad.double().cpu().numpy() - ref_grad.reshape(-1).astype(np.float32)).__abs__().max()),
+    "grad_scale": float(np.abs(ref_grad).max()), "du": u_peak.grad.item(), "ref_du": ref_du,
+    "dJ": Jp - Jm, "directional": float(np.dot(ref_grad.reshape(-1), gp.astype(np.float64) - gm.astype(np.float64))),
+}))
+"""
+
+
+def test_torch_autograd_function():
+    """LbmObjective on CUDA tensors: J.backward() returns the adjoint gradients, and a directional
+    central difference of J along a random design perturbation matches <dJ/dgamma, v> (FP64
+    storage).  Runs in a child process that imports torch first, as a torch user would: the
+    library then shares torch's HIP runtime (this process has already started the library's own)."""
+    import json
+    import subprocess
+    import sys
+    from pathlib import Path
 
     report("torch_autograd")
-    n = (16, 8, 8)
-    flags = duct(*n)
-    blend = 0.05 * design(flags)
-    prob = LbmAdjointProblem(flags=flags, shape=n, storage="fp64", steps=40, window=8, checkpoint_every=0,
-                             snapshots=4, x_in=1, x_out=n[0] - 2)
-    dev = torch.device("cuda:0")
-    gamma = torch.tensor(blend.astype(np.float32), device=dev, requires_grad=True)
-    u_peak = torch.tensor(0.05, dtype=torch.float32, device=dev, requires_grad=True)
-    J = lbm_objective(gamma, u_peak, prob)
-    J.backward()
-    ref_J, ref_grad, ref_du = gpu_duct(hip, n, blend, storage="fp64").adjoint_run(40, 8, 1, 1, n[0] - 2)
-    assert J.item() == ref_J
-    assert torch.allclose(gamma.grad.double().cpu(), torch.from_numpy(ref_grad.reshape(-1)).float().double(),
-                          rtol=1e-6, atol=0)
-    assert u_peak.grad.item() == pytest.approx(ref_du, rel=1e-6)
-    rng = np.random.default_rng(12)
-    v = np.where(flags == 0, rng.standard_normal(flags.size), 0.0)
-    h = 1e-3
-    gp = (blend + h * 0.05 * v).clip(0, 1).astype(np.float32)
-    gm = (blend - h * 0.05 * v).clip(0, 1).astype(np.float32)
-    with torch.no_grad():
-        Jp = lbm_objective(torch.tensor(gp, device=dev), u_peak.detach(), prob).item()
-        Jm = lbm_objective(torch.tensor(gm, device=dev), u_peak.detach(), prob).item()
-    directional = float(np.dot(ref_grad.reshape(-1), gp.astype(np.float64) - gm.astype(np.float64)))
-    assert Jp - Jm == pytest.approx(directional, rel=1e-5)
-    prob.close()
+    root = Path(__file__).resolve().parents[1]
+    out = subprocess.run([sys.executable, "-c", TORCH_CHILD, str(root)], capture_output=True, text=True,
+                         timeout=240, cwd=root)
+    assert out.returncode == 0, out.stderr[-3000:]
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    report("torch_autograd_result", **r)
+    assert r["J"] == r["ref_J"]
+    assert r["grad_device"] == "cuda:0"
+    assert r["grad_err"] <= 1e-6 * r["grad_scale"]
+    assert r["du"] == pytest.approx(r["ref_du"], rel=1e-6)
+    assert r["dJ"] == pytest.approx(r["directional"], rel=1e-5)
