@@ -560,11 +560,11 @@ struct cfd_lbm_solver {
             throw std::invalid_argument("checkpoints do not fit in device memory (" + std::to_string(need >> 20) +
                                         " MiB needed, " + std::to_string(free_b >> 20) +
                                         " MiB free): raise the checkpoint interval or lower the snapshot count");
-        std::vector<std::unique_ptr<DeviceBuffer<unsigned char>>> pool(buffers);
-        for (auto& b : pool) {
-            b = std::make_unique<DeviceBuffer<unsigned char>>();
-            b->allocate(state_bytes);
-        }
+        // One allocation for all checkpoint states (state_bytes is a multiple of 1 KiB).
+        DeviceBuffer<unsigned char> pool_mem;
+        pool_mem.allocate(buffers * state_bytes);
+        std::vector<void*> pool(buffers);
+        for (size_t k = 0; k < buffers; ++k) pool[k] = pool_mem.ptr + k * state_bytes;
         DeviceBuffer<unsigned char> lam, gg, gp, acc;
         lam.allocate(static_cast<size_t>(q) * geo.stride * real);
         gg.allocate(cells * sizeof(double));
@@ -584,10 +584,10 @@ struct cfd_lbm_solver {
         std::vector<std::pair<uint64_t, void*>> shots;
         std::vector<void*> free_list;
         if (every > 0) {
-            for (uint64_t c = 0; c * every < steps; ++c) shots.emplace_back(c * every, pool[c]->ptr);
+            for (uint64_t c = 0; c * every < steps; ++c) shots.emplace_back(c * every, pool[c]);
         } else {
-            shots.emplace_back(0, pool[0]->ptr);
-            for (size_t k = 1; k < buffers; ++k) free_list.push_back(pool[k]->ptr);
+            shots.emplace_back(0, pool[0]);
+            for (size_t k = 1; k < buffers; ++k) free_list.push_back(pool[k]);
             // The first descent of the binomial plan: its advances are the forward sweep itself.
             std::vector<void*> fl = free_list;
             for (uint64_t a0 = 0; steps - a0 > 1;) {
@@ -602,16 +602,20 @@ struct cfd_lbm_solver {
         hip_check(hipEventCreate(&e1), "hipEventCreate");
         hip_check(hipEventCreate(&e2), "hipEventCreate");
         hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-        size_t next = 0;
+        // Forward sweep.  A step whose result is a checkpoint writes it out of place into the
+        // checkpoint's buffer, and the step after it reads from there: storing costs no extra
+        // traffic.  Only the start state is copied.  The sweep ends in f (no checkpoint is at T).
+        hip_check(hipMemcpyAsync(shots[0].second, f.ptr, state_bytes, hipMemcpyDeviceToDevice, stream),
+                  "hipMemcpyAsync start state");
+        const void* cur = f.ptr;
+        size_t next = 1;
         for (uint64_t k = 0; k < steps; ++k) {
-            if (next < shots.size() && shots[next].first == k) {
-                hip_check(hipMemcpyAsync(shots[next].second, f.ptr, state_bytes, hipMemcpyDeviceToDevice, stream),
-                          "hipMemcpyAsync snapshot");
-                ++next;
-            }
-            ops->step(st, par(k), b, false, f.ptr, f.ptr);
+            void* dst = f.ptr;
+            if (next < shots.size() && shots[next].first == k + 1) dst = shots[next++].second;
+            ops->step(st, par(k), b, false, cur, dst);
+            cur = dst;
             ++forward_steps;
-            if (k + 1 >= steps - window + 1) adj->objective(st, par(k + 1), b, f.ptr, args, 1.0, reinterpret_cast<double*>(acc.ptr));
+            if (k + 1 >= steps - window + 1) adj->objective(st, par(k + 1), b, cur, args, 1.0, reinterpret_cast<double*>(acc.ptr));
         }
         t = t0 + steps;
         hip_check(hipEventRecord(e1, stream), "hipEventRecord");
@@ -634,7 +638,7 @@ struct cfd_lbm_solver {
                 std::vector<const void*> states(e - s0);
                 states[0] = shots[c].second;
                 for (uint64_t k = 1; k < e - s0; ++k) {
-                    void* dst = pool[segments + k - 1]->ptr;
+                    void* dst = pool[segments + k - 1];
                     advance(states[k - 1], dst, s0 + k - 1, s0 + k);
                     states[k] = dst;
                 }
