@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
     if constexpr (Storage<T>::kShifted)  // the transposed collision takes the populations themselves
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            f[d] = full_population<true, Q, d>(f[d]);
+            f[d] = full_population<Storage<T>, Q, d>(f[d]);
         });
     // lambda_{t+1} at the slots this cell wrote (its store slots).
     const uint32_t off_n = n * static_cast<uint32_t>(sizeof(Real));
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void drag_partial(const void* __restrict__ f, 
             const int cd = a.drag_axis == 0 ? L::cx[d] : (a.drag_axis == 1 ? L::cy[d] : L::cz[d]);
             if ((bits >> (d - 1)) & 1U)
                 v += -2.0 * static_cast<double>(cd) *
-                     static_cast<double>(full_population<Storage<T>::kShifted, Q, d>(fin[d]));
+                     static_cast<double>(full_population<Storage<T>, Q, d>(fin[d]));
         });
     }
     sm[threadIdx.x] = v;
@@ -351,7 +351,9 @@ __global__ __launch_bounds__(256) void plane_partial(const void* __restrict__ f,
             load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
             Real rho = Real(0);
             static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
-            const double rho_d = Storage<T>::kShifted ? 1.0 + static_cast<double>(rho) : static_cast<double>(rho);
+            const double rho_d = Storage<T>::kShifted
+                                     ? 1.0 + static_cast<double>(rho * static_cast<Real>(Storage<T>::kInvScale))
+                                     : static_cast<double>(rho);
             v = rho_d * (which == 0 ? a.c_in : a.c_out);
         }
     }

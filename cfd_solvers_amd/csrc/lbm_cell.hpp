@@ -72,6 +72,12 @@ __device__ __forceinline__ void load_cell(const void* f, const Geometry& g, cons
     });
 }
 
+// The moving-wall momentum term in the state representation (scaled for shifted storage).
+template <class S, class Real> __device__ __forceinline__ Real wall_term(Real v) {
+    if constexpr (S::kShifted) return v * static_cast<Real>(S::kScale);
+    else return v;
+}
+
 template <int Q, class T, bool Odd, bool Bounded>
 __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Physics<typename Storage<T>::Real>& p,
                                            uint32_t n, uint32_t x, uint32_t y, uint32_t z,
@@ -97,14 +103,14 @@ __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Phy
                     const Real cu = static_cast<Real>(L::cx[i]) * p.wall_u[ka][0] +
                                     static_cast<Real>(L::cy[i]) * p.wall_u[ka][1] +
                                     static_cast<Real>(L::cz[i]) * p.wall_u[ka][2];
-                    fout[i] = fout[i] - Real(6) * static_cast<Real>(L::w[i]) * rho_wall * cu;
+                    fout[i] = fout[i] - wall_term<S>(Real(6) * static_cast<Real>(L::w[i]) * rho_wall * cu);
                 }
                 const uint32_t kb = (vel[(i - 1) / 10] >> (3 * ((i - 1) % 10))) & 7U;
                 if (kb != 0U) {
                     const Real cu = static_cast<Real>(L::cx[i + 1]) * p.wall_u[kb][0] +
                                     static_cast<Real>(L::cy[i + 1]) * p.wall_u[kb][1] +
                                     static_cast<Real>(L::cz[i + 1]) * p.wall_u[kb][2];
-                    fout[i + 1] = fout[i + 1] - Real(6) * static_cast<Real>(L::w[i + 1]) * rho_wall * cu;
+                    fout[i + 1] = fout[i + 1] - wall_term<S>(Real(6) * static_cast<Real>(L::w[i + 1]) * rho_wall * cu);
                 }
             });
         }

@@ -92,17 +92,24 @@ template <class T> struct Storage;
 template <> struct Storage<float> {
     using Real = float;
     static constexpr bool kShifted = false;
+    static constexpr float kScale = 1.0F, kInvScale = 1.0F;
     template <int Q, int D> __device__ __forceinline__ static float decode(float v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static float encode(float v) { return v; }
 };
+// The state of an fp16s lattice is kept scaled by 4096 in registers as well (a power of two: every
+// operation on the scaled values is exact scaling of the same operation on the deviations), so
+// decoding and encoding are plain conversions.
 template <> struct Storage<_Float16> {
     using Real = float;
     static constexpr bool kShifted = true;
-    template <int Q, int D> __device__ __forceinline__ static float decode(_Float16 h) {
-        return static_cast<float>(h) * kHalfInvScale;
-    }
+    static constexpr float kScale = kHalfScale, kInvScale = kHalfInvScale;
+    template <int Q, int D> __device__ __forceinline__ static float decode(_Float16 h) { return static_cast<float>(h); }
+    // The barrier keeps the compiler from fusing the producing multiply or fma into the conversion
+    // (v_fma_mix*_f16 rounds once, straight to binary16): the stored value is the binary16 rounding
+    // of the FP32 result, on every path.
     template <int Q, int D> __device__ __forceinline__ static _Float16 encode(float v) {
-        return static_cast<_Float16>(v * kHalfScale);
+        asm volatile("" : "+v"(v));
+        return static_cast<_Float16>(v);
     }
 };
 // fp32s: FP32 storage of the deviation f - w_d, collision on deviations (the accuracy of the
@@ -113,6 +120,7 @@ struct F32Dev {
 template <> struct Storage<F32Dev> {
     using Real = float;
     static constexpr bool kShifted = true;
+    static constexpr float kScale = 1.0F, kInvScale = 1.0F;
     template <int Q, int D> __device__ __forceinline__ static float decode(F32Dev h) { return h.v; }
     template <int Q, int D> __device__ __forceinline__ static F32Dev encode(float v) { return F32Dev{v}; }
 };
@@ -120,24 +128,27 @@ template <> struct Storage<F32Dev> {
 template <> struct Storage<uint32_t> {
     using Real = uint32_t;
     static constexpr bool kShifted = false;
+    static constexpr float kScale = 1.0F, kInvScale = 1.0F;
     template <int Q, int D> __device__ __forceinline__ static uint32_t decode(uint32_t v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static uint32_t encode(uint32_t v) { return v; }
 };
 template <> struct Storage<double> {
     using Real = double;
     static constexpr bool kShifted = false;
+    static constexpr double kScale = 1.0, kInvScale = 1.0;
     template <int Q, int D> __device__ __forceinline__ static double decode(double v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static double encode(double v) { return v; }
 };
 
-// Population d of the state representation -> the population itself (adds w_d when shifted).
-template <bool Shifted, int Q, int D, class Real> __device__ __forceinline__ Real full_population(Real v) {
-    if constexpr (Shifted) return v + static_cast<Real>(Lattice<Q>::w[D]);
+// Population d of the state representation -> the population itself (deviation / scale + w_d
+// when shifted).
+template <class S, int Q, int D, class Real> __device__ __forceinline__ Real full_population(Real v) {
+    if constexpr (S::kShifted) return v * static_cast<Real>(S::kInvScale) + static_cast<Real>(Lattice<Q>::w[D]);
     else return v;
 }
 // The population itself -> the state representation.
-template <bool Shifted, int Q, int D, class Real> __device__ __forceinline__ Real state_population(Real v) {
-    if constexpr (Shifted) return v - static_cast<Real>(Lattice<Q>::w[D]);
+template <class S, int Q, int D, class Real> __device__ __forceinline__ Real state_population(Real v) {
+    if constexpr (S::kShifted) return (v - static_cast<Real>(Lattice<Q>::w[D])) * static_cast<Real>(S::kScale);
     else return v;
 }
 
@@ -248,18 +259,34 @@ __device__ __forceinline__ Real equilibrium(Real rho, Real ux, Real uy, Real uz,
     return static_cast<Real>(Lattice<Q>::w[D]) * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
 }
 
-// f_eq - w_d from rho' = rho - 1 and rho: w_d (rho' + rho (cu + cu^2/2 - uu)), no cancellation.
-template <int Q, int D, class Real>
+// Shifted arithmetic (fp16s, fp32s; no reference counterpart).  With cu = 3 c.u and
+// uu = 3/2 |u|^2:
+//   f_eq - w_d = w_d fma(rho, fma(cu/2, cu, -uu) + cu, rho')      rho' = rho - 1
+// scaled by the storage scale K (exact).  Opposite directions share fma(cu/2, cu, -uu).
+template <class S, int Q, int D, class Real> __device__ __forceinline__ Real scaled_weight() {
+    return static_cast<Real>(Lattice<Q>::w[D]) * static_cast<Real>(S::kScale);
+}
+template <class S, int Q, int D, class Real>
 __device__ __forceinline__ Real equilibrium_dev(Real rp, Real rho, Real ux, Real uy, Real uz, Real uu) {
     const Real cu = Real(3) * cdot<Q, D>(ux, uy, uz);
-    return static_cast<Real>(Lattice<Q>::w[D]) * (rp + rho * (cu + Real(0.5) * cu * cu - uu));
+    const Real x = __builtin_fmaf(Real(0.5) * cu, cu, -uu) + cu;
+    return scaled_weight<S, Q, D, Real>() * __builtin_fmaf(rho, x, rp);
+}
+// 3/2 |u|^2 as the shifted arithmetic forms it.
+template <class Real> __device__ __forceinline__ Real shifted_uu(Real ux, Real uy, Real uz) {
+    return Real(1.5) * __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
 }
 
-// Equilibrium in the state representation (rp = rho - 1, used when shifted).
-template <bool Shifted, int Q, int D, class Real>
+// Equilibrium in the state representation of storage S (rp = rho - 1, used when shifted).
+template <class S, int Q, int D, class Real>
 __device__ __forceinline__ Real equilibrium_state(Real rp, Real rho, Real ux, Real uy, Real uz, Real uu) {
-    if constexpr (Shifted) return equilibrium_dev<Q, D>(rp, rho, ux, uy, uz, uu);
+    if constexpr (S::kShifted) return equilibrium_dev<S, Q, D>(rp, rho, ux, uy, uz, uu);
     else return equilibrium<Q, D>(rho, ux, uy, uz, uu);
+}
+// 3/2 |u|^2 in the arithmetic of storage S.
+template <class S, class Real> __device__ __forceinline__ Real kinetic_term(Real ux, Real uy, Real uz) {
+    if constexpr (S::kShifted) return shifted_uu(ux, uy, uz);
+    else return Real(1.5) * (ux * ux + uy * uy + uz * uz);
 }
 
 template <int Q, int D, class Real>
@@ -328,11 +355,11 @@ __device__ __forceinline__ Real inlet_velocity(const Physics<Real>& p, const Geo
 // Density of a velocity-inlet cell (Zou-He, from the populations arriving from inside) as
 // rho' = rho - 1 and rho.  Unshifted: rho = (S0 + 2 S-) / (1 - u) as in src/lbm/d2q9.cpp:152-168.
 // Shifted (S0 + 2 S- of the weights is 1): rho' = (S0' + 2 S-' + u) / (1 - u).
-template <bool Shifted, int Q, class Real>
+template <class S, int Q, class Real>
 __device__ __forceinline__ void inlet_density(const Real (&f)[Q], Real uin, Real& rp, Real& rho) {
     const Real s = sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, -1>(f);
-    if constexpr (Shifted) {
-        rp = (s + uin) / (Real(1) - uin);
+    if constexpr (S::kShifted) {
+        rp = (s * static_cast<Real>(S::kInvScale) + uin) / (Real(1) - uin);
         rho = Real(1) + rp;
     } else {
         rho = s / (Real(1) - uin);
@@ -341,21 +368,40 @@ __device__ __forceinline__ void inlet_density(const Real (&f)[Q], Real uin, Real
 }
 // Normal velocity of a pressure-outlet cell: -1 + (S0 + 2 S+) / rho_out (shifted: the weights'
 // part of the sum is 1, so u = (1 - rho_out + S0' + 2 S+') / rho_out).
-template <bool Shifted, int Q, class Real>
+template <class S, int Q, class Real>
 __device__ __forceinline__ Real outlet_velocity(const Physics<Real>& p, const Real (&f)[Q]) {
     const Real s = sum_cx<Q, 0>(f) + Real(2) * sum_cx<Q, 1>(f);
-    if constexpr (Shifted) return ((Real(1) - p.rho_out) + s) / p.rho_out;
+    if constexpr (S::kShifted) return ((Real(1) - p.rho_out) + s * static_cast<Real>(S::kInvScale)) / p.rho_out;
     else return Real(-1) + s / p.rho_out;
 }
 
-// Density (and rho' = rho - 1) and velocity m / rho of a fluid cell from its state.
-template <bool Shifted, int Q, class Real>
+// m / rho from c = 1 / rho with one residual correction: u0 = m c, u = u0 + (m - u0 rho) c, the
+// residual exact by fma.  A bare m * (1 / rho) is biased by its double rounding (a 2e-6 drift of the
+// mean velocity in a forced porous channel); the corrected quotient is not.
+template <class Real> __device__ __forceinline__ Real corrected_quotient(Real m, Real rho, Real c) {
+    const Real u0 = m * c;
+    return __builtin_fmaf(__builtin_fmaf(-u0, rho, m), c, u0);
+}
+
+// Density (and rho' = rho - 1) and velocity m / rho of a fluid cell from its state.  The shifted
+// arithmetic divides once (corrected_quotient); scaled states are unscaled first (exact).
+template <class S, int Q, class Real>
 __device__ __forceinline__ void fluid_moments(const Real (&f)[Q], Real& rp, Real& rho, Real& ux, Real& uy, Real& uz) {
     Real mx, my, mz, s;
     moments<Q>(f, s, mx, my, mz);
-    if constexpr (Shifted) {
-        rp = s;
-        rho = Real(1) + s;
+    if constexpr (S::kShifted) {
+        const Real k = static_cast<Real>(S::kInvScale);
+        rp = s * k;
+        rho = Real(1) + rp;
+        if (rho > Real(0)) {
+            const Real c = Real(1) / rho;
+            ux = corrected_quotient(mx * k, rho, c);
+            uy = corrected_quotient(my * k, rho, c);
+            uz = corrected_quotient(mz * k, rho, c);
+        } else {
+            ux = uy = uz = Real(0);
+        }
+        return;
     } else {
         rho = s;
         rp = s - Real(1);
@@ -370,26 +416,26 @@ __device__ __forceinline__ void fluid_moments(const Real (&f)[Q], Real& rp, Real
 }
 
 // Macroscopic state a non-wall cell reports (esoteric_pull.hpp:180-209 for fluid cells).
-template <int Q, bool Shifted, class Real>
+template <int Q, class S, class Real>
 __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
                                            uint32_t y, uint32_t z, const Real (&f)[Q], Real& rho, Real& ux, Real& uy,
                                            Real& uz) {
     if (flag == kInlet) {
         const Real uin = inlet_velocity(p, g, y, z, Lattice<Q>::dims == 3);
         Real rp;
-        inlet_density<Shifted, Q>(f, uin, rp, rho);
+        inlet_density<S, Q>(f, uin, rp, rho);
         ux = uin;
         uy = uz = Real(0);
         return;
     }
     if (flag == kOutlet) {
         rho = p.rho_out;
-        ux = outlet_velocity<Shifted, Q>(p, f);
+        ux = outlet_velocity<S, Q>(p, f);
         uy = uz = Real(0);
         return;
     }
     Real rp;
-    fluid_moments<Shifted, Q>(f, rp, rho, ux, uy, uz);
+    fluid_moments<S, Q>(f, rp, rho, ux, uy, uz);
     Real ax = p.ax, ay = p.ay, az = p.az;
     if (p.lax) {
         ax = ax + p.lax[n];
@@ -401,34 +447,60 @@ __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometr
     uz = uz + Real(0.5) * az;
 }
 
-// Collision of one non-wall cell in registers, on the state representation (populations, or
-// deviations when Shifted).  Returns the density used for moving-wall momentum corrections
-// (src/lbm/d2q9.cpp:217-223 uses the fluid cell's density).
+// BGK relaxation of the state f (populations, or scaled deviations for shifted storage S) towards
+// the equilibrium of (rho, u): the reference's f - omega (f - f_eq) for unshifted storage, and
+// fma(-omega, f' - f_eq', f') with the pair-shared equilibrium for shifted storage.
+template <int Q, class S, class Real>
+__device__ __forceinline__ void relax(Real (&f)[Q], Real rp, Real rho, Real ux, Real uy, Real uz, Real uu,
+                                      Real omega) {
+    if constexpr (S::kShifted) {
+        f[0] = __builtin_fmaf(-omega, f[0] - scaled_weight<S, Q, 0, Real>() * __builtin_fmaf(rho, -uu, rp), f[0]);
+        static_for<(Q - 1) / 2>([&](auto P) {
+            constexpr int i = 2 * decltype(P)::value + 1;
+            const Real cu = Real(3) * cdot<Q, i>(ux, uy, uz);
+            const Real common = __builtin_fmaf(Real(0.5) * cu, cu, -uu);
+            const Real wk = scaled_weight<S, Q, i, Real>();  // w_i = w_(i+1)
+            const Real feq_a = wk * __builtin_fmaf(rho, common + cu, rp);
+            const Real feq_b = wk * __builtin_fmaf(rho, common - cu, rp);
+            f[i] = __builtin_fmaf(-omega, f[i] - feq_a, f[i]);
+            f[i + 1] = __builtin_fmaf(-omega, f[i + 1] - feq_b, f[i + 1]);
+        });
+    } else {
+        static_for<Q>([&](auto DD) {
+            constexpr int d = decltype(DD)::value;
+            f[d] = f[d] - omega * (f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu));
+        });
+    }
+}
+
+// Collision of one non-wall cell in registers, on the state representation of storage S
+// (populations, or scaled deviations when shifted).  Returns the density used for moving-wall
+// momentum corrections (src/lbm/d2q9.cpp:217-223 uses the fluid cell's density).
 // Forced selects the Guo-forcing variant at compile time: the unforced variant carries no force
 // temporaries and fits twice the waves per SIMD.  With Forced, a zero acceleration still takes the
 // unforced arithmetic (the reference's guo_force returns 0 then), so both variants agree bitwise.
-template <int Q, bool Forced, bool Shifted, class Real>
+template <int Q, bool Forced, class S, class Real>
 __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
                                         uint32_t y, uint32_t z, Real (&f)[Q]) {
     if (flag == kInlet || flag == kOutlet) {
         Real rp, rho, ux;
         if (flag == kInlet) {
             ux = inlet_velocity(p, g, y, z, Lattice<Q>::dims == 3);
-            inlet_density<Shifted, Q>(f, ux, rp, rho);
+            inlet_density<S, Q>(f, ux, rp, rho);
         } else {
             rho = p.rho_out;
             rp = rho - Real(1);
-            ux = outlet_velocity<Shifted, Q>(p, f);
+            ux = outlet_velocity<S, Q>(p, f);
         }
-        const Real uu = Real(1.5) * (ux * ux);
+        const Real uu = kinetic_term<S>(ux, Real(0), Real(0));
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            f[d] = equilibrium_state<Shifted, Q, d>(rp, rho, ux, Real(0), Real(0), uu);
+            f[d] = equilibrium_state<S, Q, d>(rp, rho, ux, Real(0), Real(0), uu);
         });
         return rho;
     }
     Real rp, rho, ux, uy, uz;
-    fluid_moments<Shifted, Q>(f, rp, rho, ux, uy, uz);
+    fluid_moments<S, Q>(f, rp, rho, ux, uy, uz);
     Real ax = Real(0), ay = Real(0), az = Real(0);
     if constexpr (Forced) {
         ax = p.ax;
@@ -447,25 +519,24 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
         uy = uy + Real(0.5) * ay;
         uz = uz + Real(0.5) * az;
     }
-    const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
+    const Real uu = kinetic_term<S>(ux, uy, uz);
     const Real omega = p.omega;
     Real fin_copy[Q];
     if (p.gamma) {
         static_for<Q>([&](auto DD) { fin_copy[decltype(DD)::value] = f[decltype(DD)::value]; });
     }
     if (!Forced || (ax == Real(0) && ay == Real(0) && az == Real(0))) {
-        static_for<Q>([&](auto DD) {
-            constexpr int d = decltype(DD)::value;
-            f[d] = f[d] - omega * (f[d] - equilibrium_state<Shifted, Q, d>(rp, rho, ux, uy, uz, uu));
-        });
+        relax<Q, S>(f, rp, rho, ux, uy, uz, uu, omega);
     } else {
         const Real fx = rho * ax, fy = rho * ay, fz = rho * az;
         const Real uf = ux * fx + uy * fy + uz * fz;
         const Real wfac = Real(1) - Real(0.5) * omega;
+        relax<Q, S>(f, rp, rho, ux, uy, uz, uu, omega);
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            const Real post = f[d] - omega * (f[d] - equilibrium_state<Shifted, Q, d>(rp, rho, ux, uy, uz, uu));
-            f[d] = post + guo<Q, d>(ux, uy, uz, fx, fy, fz, uf, wfac);
+            Real src = guo<Q, d>(ux, uy, uz, fx, fy, fz, uf, wfac);
+            if constexpr (S::kShifted) src = src * static_cast<Real>(S::kScale);
+            f[d] = f[d] + src;
         });
     }
     if (p.gamma) {
@@ -477,7 +548,8 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
         Real mixed[Q];
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            mixed[d] = keep * f[d] + gm * fin_copy[opposite(d)];
+            if constexpr (S::kShifted) mixed[d] = __builtin_fmaf(keep, f[d], gm * fin_copy[opposite(d)]);
+            else mixed[d] = keep * f[d] + gm * fin_copy[opposite(d)];
         });
         static_for<Q>([&](auto DD) { f[decltype(DD)::value] = mixed[decltype(DD)::value]; });
     }

@@ -27,7 +27,7 @@ __device__ __forceinline__ void update_cell(const void* src, void* dst, const Ge
     Real fin[Q];
     PairSlots<Q> s;
     load_cell<Q, T, Odd, Bounded>(src, g, p.links, h, n, fin, s);
-    const Real rho = collide<Q, Forced, Storage<T>::kShifted>(p, g, flag, n, y, z, fin);
+    const Real rho = collide<Q, Forced, Storage<T>>(p, g, flag, n, y, z, fin);
     store_cell<Q, T, Odd, Bounded>(dst, g, p, n, x, y, z, rho, fin, s);
 }
 
@@ -174,16 +174,15 @@ __global__ __launch_bounds__(256) void initialize(void* __restrict__ f, const Ge
         uz = a.fields[3U * cells + n];
     }
     if (flags && is_wall(flags[n])) ux = uy = uz = Real(0);
-    const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
-    constexpr bool Sh = S::kShifted;
+    const Real uu = kinetic_term<S>(ux, uy, uz);
     const Real rp = rho - Real(1);
-    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium_state<Sh, Q, 0>(rp, rho, ux, uy, uz, uu));
+    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium_state<S, Q, 0>(rp, rho, ux, uy, uz, uu));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
-        plane<T>(f, g, i)[up] = S::template encode<Q, i>(equilibrium_state<Sh, Q, i>(rp, rho, ux, uy, uz, uu));
+        plane<T>(f, g, i)[up] = S::template encode<Q, i>(equilibrium_state<S, Q, i>(rp, rho, ux, uy, uz, uu));
         plane<T>(f, g, i + 1)[n] =
-            S::template encode<Q, i + 1>(equilibrium_state<Sh, Q, i + 1>(rp, rho, ux, uy, uz, uu));
+            S::template encode<Q, i + 1>(equilibrium_state<S, Q, i + 1>(rp, rho, ux, uy, uz, uu));
     });
 }
 
@@ -198,13 +197,12 @@ __global__ __launch_bounds__(256) void upload_logical(void* __restrict__ f, cons
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
     const uint32_t n = h.cell(0, 0, 0);
-    constexpr bool Sh = S::kShifted;
-    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(state_population<Sh, Q, 0>(in[n]));
+    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(state_population<S, Q, 0>(in[n]));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
-        const auto va = state_population<Sh, Q, i>(in[static_cast<uint64_t>(i) * cells + n]);
-        const auto vb = state_population<Sh, Q, i + 1>(in[static_cast<uint64_t>(i + 1) * cells + n]);
+        const auto va = state_population<S, Q, i>(in[static_cast<uint64_t>(i) * cells + n]);
+        const auto vb = state_population<S, Q, i + 1>(in[static_cast<uint64_t>(i + 1) * cells + n]);
         if constexpr (Odd) {
             plane<T>(f, g, i + 1)[up] = S::template encode<Q, i + 1>(va);
             plane<T>(f, g, i)[n] = S::template encode<Q, i>(vb);
@@ -267,7 +265,7 @@ __global__ __launch_bounds__(256) void freeze_wall_density(const void* __restric
     load_cell<Q, T, Odd, false>(f, g, nullptr, h, n, fin, s);
     Real rho = Real(0);
     static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
-    if constexpr (Storage<T>::kShifted) rho = Real(1) + rho;
+    if constexpr (Storage<T>::kShifted) rho = Real(1) + rho * static_cast<Real>(Storage<T>::kInvScale);
     solid_rho[n] = rho;
 }
 
@@ -289,13 +287,12 @@ __global__ __launch_bounds__(256) void thaw_walls(void* __restrict__ f, const Ge
     const Real rho = solid_rho[n];
     const Real rp = rho - Real(1);
     const Real zero = Real(0);
-    constexpr bool Sh = S::kShifted;
-    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium_state<Sh, Q, 0>(rp, rho, zero, zero, zero, zero));
+    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium_state<S, Q, 0>(rp, rho, zero, zero, zero, zero));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
-        const Real va = equilibrium_state<Sh, Q, i>(rp, rho, zero, zero, zero, zero);
-        const Real vb = equilibrium_state<Sh, Q, i + 1>(rp, rho, zero, zero, zero, zero);
+        const Real va = equilibrium_state<S, Q, i>(rp, rho, zero, zero, zero, zero);
+        const Real vb = equilibrium_state<S, Q, i + 1>(rp, rho, zero, zero, zero, zero);
         if constexpr (Odd) {
             plane<T>(f, g, i + 1)[up] = S::template encode<Q, i + 1>(va);
             plane<T>(f, g, i)[n] = S::template encode<Q, i>(vb);
@@ -332,7 +329,7 @@ __global__ __launch_bounds__(256) void macroscopic(const void* __restrict__ f, c
     PairSlots<Q> s;
     load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
     Real rho, ux, uy, uz;
-    cell_state<Q, Storage<T>::kShifted>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
+    cell_state<Q, Storage<T>>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
     out[n] = rho;
     out[cells + n] = ux;
     out[2U * cells + n] = uy;
@@ -363,7 +360,7 @@ __global__ __launch_bounds__(256) void download_logical(const void* __restrict__
     load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
     static_for<Q>([&](auto DD) {
         constexpr int d = decltype(DD)::value;
-        out[static_cast<uint64_t>(d) * p.cells + n] = full_population<Storage<T>::kShifted, Q, d>(fin[d]);
+        out[static_cast<uint64_t>(d) * p.cells + n] = full_population<Storage<T>, Q, d>(fin[d]);
     });
 }
 
@@ -397,9 +394,10 @@ __global__ __launch_bounds__(256) void diagnostics_partial(const void* __restric
             Real sum = Real(0);
             static_for<Q>([&](auto DD) { sum += fin[decltype(DD)::value]; });
             // Shifted storage: the sum of deviations is rho - 1 (added in FP64).
-            mass = Storage<T>::kShifted ? 1.0 + static_cast<double>(sum) : static_cast<double>(sum);
+            mass = Storage<T>::kShifted ? 1.0 + static_cast<double>(sum * static_cast<Real>(Storage<T>::kInvScale))
+                                        : static_cast<double>(sum);
             Real rho, ux, uy, uz;
-            cell_state<Q, Storage<T>::kShifted>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
+            cell_state<Q, Storage<T>>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
             ke = static_cast<double>(rho) * (static_cast<double>(ux) * ux + static_cast<double>(uy) * uy +
                                              static_cast<double>(uz) * uz);
             vmax = static_cast<double>(sqrt(ux * ux + uy * uy + uz * uz));

@@ -532,12 +532,17 @@ private:
         const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
         return w(d) * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
     }
-    // f_eq - w_d = w_d (rho' + rho (cu + cu^2/2 - uu)), rho' = rho - 1.
+    // Shifted arithmetic (the GPU's FP16S/FP32S kernels, operation for operation; no reference
+    // counterpart): uu = 3/2 fma(uz, uz, fma(uy, uy, ux ux)), cu = 3 c.u and
+    //   f_eq - w_d = w_d fma(rho, fma(cu/2, cu, -uu) + cu, rho'),   rho' = rho - 1.
+    static Real shifted_uu(Real ux, Real uy, Real uz) {
+        return Real(1.5) * std::fma(uz, uz, std::fma(uy, uy, ux * ux));
+    }
     static Real feq_dev(int d, Real rp, Real rho, Real ux, Real uy, Real uz) {
         const Real cu = Real(3) * (static_cast<Real>(L::cx[d]) * ux + static_cast<Real>(L::cy[d]) * uy +
                                    static_cast<Real>(L::cz[d]) * uz);
-        const Real uu = Real(1.5) * (ux * ux + uy * uy + uz * uz);
-        return w(d) * (rp + rho * (cu + Real(0.5) * cu * cu - uu));
+        const Real uu = shifted_uu(ux, uy, uz);
+        return w(d) * std::fma(rho, std::fma(Real(0.5) * cu, cu, -uu) + cu, rp);
     }
     static Real feq_state(int d, Real rp, Real rho, Real ux, Real uy, Real uz) {
         return kShifted ? feq_dev(d, rp, rho, ux, uy, uz) : feq(d, rho, ux, uy, uz);
@@ -600,10 +605,22 @@ private:
         if (kShifted) {
             rp = s;
             rho = Real(1) + s;
-        } else {
-            rho = s;
-            rp = s - Real(1);
+            if (rho > Real(0)) {  // one division and a residual correction: u0 = m c, u = fma(fma(-u0, rho, m), c, u0)
+                const Real c = Real(1) / rho;
+                const auto q = [&](Real m) {
+                    const Real u0 = m * c;
+                    return std::fma(std::fma(-u0, rho, m), c, u0);
+                };
+                ux = q(ux);
+                uy = q(uy);
+                uz = q(uz);
+            } else {
+                ux = uy = uz = Real(0);
+            }
+            return;
         }
+        rho = s;
+        rp = s - Real(1);
         if (rho > Real(0)) {
             ux /= rho;
             uy /= rho;
@@ -684,16 +701,34 @@ private:
         uy += Real(0.5) * ay;
         uz += Real(0.5) * az;
         const Real omega = Real(1) / tau_;
-        for (int d = 0; d < Q; ++d) {
-            Real p = fin[d];
-            p -= omega * (p - feq_state(d, rp, rho, ux, uy, uz));
-            p += guo(d, rho, ux, uy, uz, ax, ay, az, omega);
-            fout[d] = p;
+        if (kShifted) {
+            // Relaxation fma(-omega, f' - f_eq', f'); opposite directions share fma(cu/2, cu, -uu).
+            const Real uu = shifted_uu(ux, uy, uz);
+            fout[0] = std::fma(-omega, fin[0] - w(0) * std::fma(rho, -uu, rp), fin[0]);
+            for (int i = 1; i < Q; i += 2) {
+                const Real cu = Real(3) * (static_cast<Real>(L::cx[i]) * ux + static_cast<Real>(L::cy[i]) * uy +
+                                           static_cast<Real>(L::cz[i]) * uz);
+                const Real common = std::fma(Real(0.5) * cu, cu, -uu);
+                const Real feq_a = w(i) * std::fma(rho, common + cu, rp);
+                const Real feq_b = w(i) * std::fma(rho, common - cu, rp);
+                fout[i] = std::fma(-omega, fin[i] - feq_a, fin[i]);
+                fout[i + 1] = std::fma(-omega, fin[i + 1] - feq_b, fin[i + 1]);
+            }
+            for (int d = 0; d < Q; ++d) fout[d] += guo(d, rho, ux, uy, uz, ax, ay, az, omega);
+        } else {
+            for (int d = 0; d < Q; ++d) {
+                Real p = fin[d];
+                p -= omega * (p - feq(d, rho, ux, uy, uz));
+                p += guo(d, rho, ux, uy, uz, ax, ay, az, omega);
+                fout[d] = p;
+            }
         }
         if (!gamma_.empty()) {
             const Real g = gamma_[n];
             Real mixed[Q];
-            for (int d = 0; d < Q; ++d) mixed[d] = (Real(1) - g) * fout[d] + g * fin[opposite(d)];
+            for (int d = 0; d < Q; ++d)
+                mixed[d] = kShifted ? std::fma(Real(1) - g, fout[d], g * fin[opposite(d)])
+                                    : (Real(1) - g) * fout[d] + g * fin[opposite(d)];
             for (int d = 0; d < Q; ++d) fout[d] = mixed[d];
         }
         return rho;
