@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     uint8_t flag = kFluid;
     if constexpr (Bounded) {
         const uint32_t type = (p.links[n] >> kTypeShift) & 3U;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
             f[d] = full_population<Storage<T>, Q, d>(f[d]);
         });
     // lambda_{t+1} at the slots this cell wrote (its store slots).
-    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(Real));
+    const uint32_t off_n = h.cell(0, 0, 0) * static_cast<uint32_t>(sizeof(Real));
     const uint32_t rb = static_cast<uint32_t>(sizeof(Real));
     const uint32_t plane_bytes = static_cast<uint32_t>(g.stride * sizeof(Real));
     Real lam[Q];
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void plane_source(void* __restrict__ lam_buf, 
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0), np = h.cell(0, 0, 0);
     bool fluid = true;
     uint32_t desc = 0;
     if constexpr (Bounded) {
@@ -239,14 +239,14 @@ __global__ __launch_bounds__(256) void plane_source(void* __restrict__ lam_buf, 
     const Real c = plane_weight<Real>(a, x, fluid);
     if (c == Real(0)) return;
     Real* lam = static_cast<Real*>(lam_buf);
-    lam[n] += c;
+    lam[np] += c;
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
         const bool wup = Bounded && ((desc >> (i - 1)) & 1U);
         const bool wdn = Bounded && ((desc >> i) & 1U);
         lam[static_cast<uint64_t>(dir_a<Odd>(i, wup)) * g.stride + up] += c;
-        lam[static_cast<uint64_t>(dir_b<Odd>(i, wdn)) * g.stride + n] += c;
+        lam[static_cast<uint64_t>(dir_b<Odd>(i, wdn)) * g.stride + np] += c;
     });
 }
 
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void drag_source(void* __restrict__ lam_buf, c
             lam[static_cast<uint64_t>(dir_a<Odd>(i, true)) * g.stride + up] += Real(-2) * ci * c;
         }
         if ((bits >> i) & 1U)  // population i+1 has c = -c_i
-            lam[static_cast<uint64_t>(dir_b<Odd>(i, true)) * g.stride + n] += Real(2) * ci * c;
+            lam[static_cast<uint64_t>(dir_b<Odd>(i, true)) * g.stride + h.cell(0, 0, 0)] += Real(2) * ci * c;
     });
 }
 
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void plane_partial(const void* __restrict__ f,
         const uint32_t y = r % g.ny, z = r / g.ny;
         const uint32_t x = static_cast<uint32_t>(which == 0 ? a.x_in : a.x_out);
         const Neighbourhood h = neighbourhood(g, x, y, z);
-        const uint32_t n = h.cell(0, 0, 0);
+        const uint32_t n = h.lin(0, 0, 0);
         bool fluid = true;
         if constexpr (Bounded) fluid = ((p.links[n] >> kTypeShift) & 3U) == 0U;
         if (fluid) {

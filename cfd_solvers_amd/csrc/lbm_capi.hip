@@ -64,6 +64,7 @@ template <class T> struct DeviceBuffer {
 
 struct cfd_lbm_solver {
     int q = 19, storage = 0, device = 0;
+    uint64_t pop_elems = 0;  // elements of the population array (layout of Geometry)
     size_t nx = 0, ny = 0, nz = 0, cells = 0;
     float tau = 0.6F, ax = 0, ay = 0, az = 0;
     size_t elem = 4, real = 4;
@@ -135,15 +136,40 @@ struct cfd_lbm_solver {
         geo.xcd_remap = 0U;
         if (const char* v = std::getenv("CFD_LBM_XCD_REMAP"))
             geo.xcd_remap = (std::atoi(v) != 0 && geo.tiles % 8U == 0U && geo.tiles >= 64U) ? 1U : 0U;
-        // Planes start on 256-element boundaries and are staggered by 4352 elements so that they
-        // never sit a power of two apart (measured +2 % on 256^3); CFD_LBM_PLANE_PAD overrides.
-        uint64_t pad = 4352;
-        if (const char* v = std::getenv("CFD_LBM_PLANE_PAD")) pad = std::strtoull(v, nullptr, 10);
-        geo.stride = (static_cast<uint64_t>(cells) + pad + 255U) / 256U * 256U;
-        // Kernels address a population as plane base + 32-bit byte offset, reaching one plane further
-        // for bounce-back: two planes must fit in 4 GiB.
-        if (2U * geo.stride * elem > (uint64_t{1} << 32))
-            throw std::invalid_argument("lattice too large: two population planes must fit in 4 GiB");
+        // Population layout (Geometry in lbm_device.hpp).  Kernels address a population as a plane
+        // base plus a 32-bit byte offset, reaching one plane further for bounce-back; the adjoint
+        // keeps lambda in the same layout with `real`-sized elements.  Blocks of 2^k x-rows holding
+        // ~4096 cells each (measured +3.5 % on the in-place pattern at 256^3 FP32) when the whole
+        // array fits those offsets, plain planes otherwise; CFD_LBM_LAYOUT=planes forces plain
+        // planes, CFD_LBM_BLOCK_CELLS sets the block size.
+        const size_t widest = std::max(elem, real);
+        const uint64_t rows = static_cast<uint64_t>(ny) * nz;
+        uint64_t block_cells = 4096;
+        if (const char* v = std::getenv("CFD_LBM_BLOCK_CELLS")) block_cells = std::strtoull(v, nullptr, 10);
+        bool blocked = block_cells >= nx;
+        if (const char* v = std::getenv("CFD_LBM_LAYOUT")) blocked = blocked && std::string(v) != "planes";
+        uint32_t rs = 0;
+        while ((static_cast<uint64_t>(nx) << (rs + 1)) <= block_cells && (uint64_t{1} << (rs + 1)) <= rows) ++rs;
+        const uint64_t block = (uint64_t{1} << rs) * nx;
+        const uint64_t blocks = (rows + (uint64_t{1} << rs) - 1) >> rs;
+        blocked = blocked && (blocks * q + 1) * block * widest < (uint64_t{1} << 32);
+        if (blocked) {
+            geo.row_shift = rs;
+            geo.stride = block;
+            geo.block_elems = static_cast<uint32_t>(q * block);
+            pop_elems = blocks * q * block;
+        } else {
+            // Plain planes start on 256-element boundaries and are staggered by 4352 elements so that
+            // they never sit a power of two apart (measured +2 % on 256^3); CFD_LBM_PLANE_PAD overrides.
+            uint64_t pad = 4352;
+            if (const char* v = std::getenv("CFD_LBM_PLANE_PAD")) pad = std::strtoull(v, nullptr, 10);
+            geo.row_shift = 31;
+            geo.block_elems = 0;
+            geo.stride = (static_cast<uint64_t>(cells) + pad + 255U) / 256U * 256U;
+            pop_elems = static_cast<uint64_t>(q) * geo.stride;
+            if (2U * geo.stride * elem > (uint64_t{1} << 32))
+                throw std::invalid_argument("lattice too large: two population planes must fit in 4 GiB");
+        }
 
         ops = cfd_hip::kernel_ops(q, storage);
         int count = 0;
@@ -152,7 +178,7 @@ struct cfd_lbm_solver {
         hip_check(hipGetDevice(&device), "hipGetDevice");
         hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         try {
-            f.allocate(static_cast<size_t>(q) * geo.stride * elem);
+            f.allocate(pop_elems * elem);
             initialize_uniform(1.0F, 0.0F, 0.0F, 0.0F);
         } catch (...) {
             (void)hipStreamDestroy(stream);
@@ -170,6 +196,11 @@ struct cfd_lbm_solver {
 
     void use_device() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
     bool odd() const { return (t & 1U) != 0U; }
+    // 32-bit byte offsets cover an array of this layout with elements of `bytes` (plus one plane).
+    bool offsets_fit(size_t bytes) const {
+        const uint64_t span = geo.row_shift == 31 ? 2U * geo.stride : pop_elems + geo.stride;
+        return span * bytes <= (uint64_t{1} << 32);
+    }
     bool bounded() const { return static_cast<bool>(flags); }
     uint32_t seg_x() const { return static_cast<uint32_t>((nx + 63) / 64); }
     size_t vel_words() const { return static_cast<size_t>((q - 1 + 9) / 10); }
@@ -539,8 +570,10 @@ struct cfd_lbm_solver {
         adjoint_checks(steps, window);
         const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage);
         cfd_hip::AdjointArgs args = objective_args(kind, x_in, x_out, window);
-        const size_t state_bytes = static_cast<size_t>(q) * geo.stride * elem;
-        const size_t fixed = static_cast<size_t>(q) * geo.stride * real + 2 * cells * 8 + 64;
+        if (!offsets_fit(real))
+            throw std::invalid_argument("lattice too large for the adjoint: two FP32 lambda planes must fit in 4 GiB");
+        const size_t state_bytes = pop_elems * elem;
+        const size_t fixed = pop_elems * real + 2 * cells * 8 + 64;
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         size_t buffers = 0;  // state buffers besides f
@@ -566,7 +599,7 @@ struct cfd_lbm_solver {
         std::vector<void*> pool(buffers);
         for (size_t k = 0; k < buffers; ++k) pool[k] = pool_mem.ptr + k * state_bytes;
         DeviceBuffer<unsigned char> lam, gg, gp, acc;
-        lam.allocate(static_cast<size_t>(q) * geo.stride * real);
+        lam.allocate(pop_elems * real);
         gg.allocate(cells * sizeof(double));
         gp.allocate(cells * sizeof(double));
         acc.allocate(sizeof(double));
@@ -791,7 +824,7 @@ struct cfd_lbm_solver {
         use_device();
         const size_t n = static_cast<size_t>(q) * cells;
         DeviceBuffer<unsigned char> ids, loads;
-        ids.allocate(static_cast<size_t>(q) * geo.stride * sizeof(uint32_t));
+        ids.allocate(pop_elems * sizeof(uint32_t));
         loads.allocate(n * sizeof(int64_t));
         const LaunchState st = state();
         // Wall-velocity indices do not move slots; the probe sees plain walls.
@@ -811,7 +844,7 @@ struct cfd_lbm_solver {
         ops->index_map(st, (parity & 1) != 0, bounded(), probe_flags.ptr,
                        reinterpret_cast<const uint32_t*>(probe_links.ptr), reinterpret_cast<uint32_t*>(ids.ptr),
                        reinterpret_cast<int64_t*>(loads.ptr));
-        std::vector<uint32_t> slots(static_cast<size_t>(q) * geo.stride);
+        std::vector<uint32_t> slots(pop_elems);
         d2h(load, loads.ptr, n * sizeof(int64_t));
         d2h(slots.data(), ids.ptr, slots.size() * sizeof(uint32_t));
         sync();
@@ -819,7 +852,8 @@ struct cfd_lbm_solver {
         std::fill(store, store + n, int64_t{-1});
         for (size_t d = 0; d < static_cast<size_t>(q); ++d)
             for (size_t c = 0; c < cells; ++c) {
-                const uint32_t id = slots[d * geo.stride + c];
+                const size_t pc = cfd_hip::row_base(geo, static_cast<uint32_t>(c / nx)) + c % nx;
+                const uint32_t id = slots[d * geo.stride + pc];
                 if (id != 0xFFFFFFFFU) store[id] = static_cast<int64_t>(d * cells + c);
             }
     }

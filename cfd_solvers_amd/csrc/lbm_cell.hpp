@@ -40,13 +40,15 @@ template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t
 }
 
 // Loads the logical populations of a non-wall cell (bounce-back aware) and records the slots.
+// n is the cell's linear index (for its link descriptor); slots come from the neighbourhood.
 template <int Q, class T, bool Odd, bool Bounded>
 __device__ __forceinline__ void load_cell(const void* f, const Geometry& g, const uint32_t* links,
                                           const Neighbourhood& h, uint32_t n,
                                           typename Storage<T>::Real (&fin)[Q], PairSlots<Q>& s) {
     using L = Lattice<Q>;
     using S = Storage<T>;
-    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
+    const uint32_t off_n = h.cell(0, 0, 0) * static_cast<uint32_t>(sizeof(T));
+    s.self = off_n;
     const uint32_t plane_bytes = static_cast<uint32_t>(g.stride * sizeof(T));
     if constexpr (Bounded) s.desc = links[n];
     fin[0] = S::template decode<Q, 0>(load_at<T>(plane<T>(f, g, 0), off_n));
@@ -115,7 +117,7 @@ __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Phy
             });
         }
     }
-    const uint32_t off_n = n * static_cast<uint32_t>(sizeof(T));
+    const uint32_t off_n = s.self;
     store_at<T>(plane<T>(f, g, 0), off_n, S::template encode<Q, 0>(fout[0]));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;

@@ -157,13 +157,27 @@ template <class S, int Q, int D, class Real> __device__ __forceinline__ Real sta
 // numbered x-fastest.  The hardware deals consecutive blocks round-robin to the 8 XCDs, so the
 // chip streams through neighbouring rows together; xcd_remap (off by default: it measured slower
 // on MI355X) instead gives each XCD one contiguous slab of tiles.
+// Population layout.  The lattice's x-rows (row r = z*ny + y) are grouped into blocks of 2^row_shift
+// consecutive rows; a block holds the q direction planes of its cells one after another, `stride`
+// elements apart, and blocks are `block_elems` elements apart.  Slot (d, cell (r, x)) is element
+//   (r >> row_shift) * block_elems + d * stride + (r & (2^row_shift - 1)) * nx + x.
+// row_shift = 31 puts every row in one block: plain structure-of-arrays planes, the reference's
+// StaticSoA order (soa_field.hpp:34-36), with `stride` >= cells.  Blocks of ~4096 cells keep the 19
+// slot streams of a wavefront inside one 300 KB region, which the in-place pattern streams faster.
 struct Geometry {
     uint32_t nx, ny, nz;
     uint32_t bx, by;
     uint32_t tiles_x, tiles_y, tiles;
     uint32_t xcd_remap;
-    uint64_t stride;  // elements between consecutive direction planes
+    uint32_t row_shift;    // log2 rows per block; 31: one block (plain planes)
+    uint32_t block_elems;  // elements per block (q * stride); unused when row_shift = 31
+    uint64_t stride;       // elements between consecutive direction planes of a block
 };
+
+// Element of direction plane 0 holding the first cell of x-row r.
+__host__ __device__ __forceinline__ uint32_t row_base(const Geometry& g, uint32_t r) {
+    return (r >> g.row_shift) * g.block_elems + (r & ((1U << g.row_shift) - 1U)) * g.nx;
+}
 
 __device__ __forceinline__ bool cell_coords(const Geometry& g, uint32_t& x, uint32_t& y, uint32_t& z) {
     uint32_t b = blockIdx.x;
@@ -177,11 +191,16 @@ __device__ __forceinline__ bool cell_coords(const Geometry& g, uint32_t& x, uint
     return x < g.nx && y < g.ny;
 }
 
-// Periodic neighbourhood of (x, y, z): 3 wrapped x coordinates and 9 wrapped row bases.
+// Periodic neighbourhood of (x, y, z): 3 wrapped x coordinates and, for the 9 wrapped rows, their
+// plane-0 element (population slots) and their linear cell index (flags, descriptors, fields).
 struct Neighbourhood {
     uint32_t xs[3];
-    uint32_t rows[3][3];  // [dy + 1][dz + 1]
+    uint32_t rows[3][3];   // [dy + 1][dz + 1]: row_base
+    uint32_t lrows[3][3];  // [dy + 1][dz + 1]: row * nx
+    // Element of direction plane 0 for the neighbour (dx, dy, dz).
     __device__ __forceinline__ uint32_t cell(int dx, int dy, int dz) const { return rows[dy + 1][dz + 1] + xs[dx + 1]; }
+    // Linear cell index (z*ny + y)*nx + x of the neighbour.
+    __device__ __forceinline__ uint32_t lin(int dx, int dy, int dz) const { return lrows[dy + 1][dz + 1] + xs[dx + 1]; }
 };
 
 __device__ __forceinline__ Neighbourhood neighbourhood(const Geometry& g, uint32_t x, uint32_t y, uint32_t z) {
@@ -194,7 +213,11 @@ __device__ __forceinline__ Neighbourhood neighbourhood(const Geometry& g, uint32
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) h.rows[a][b] = (zs[b] * g.ny + ys[a]) * g.nx;
+        for (int b = 0; b < 3; ++b) {
+            const uint32_t r = zs[b] * g.ny + ys[a];
+            h.rows[a][b] = row_base(g, r);
+            h.lrows[a][b] = r * g.nx;
+        }
     return h;
 }
 
@@ -222,6 +245,7 @@ constexpr uint32_t kMovingBit = 31;
 template <int Q> struct PairSlots {
     static constexpr int kVelWords = (Q - 1 + 9) / 10;  // 3-bit wall-velocity index per link, 10 per word
     uint32_t up[Q];  // byte offset of cell up = n - c_i in a plane, index by i (odd)
+    uint32_t self;   // byte offset of the cell itself in a plane
     uint32_t desc;   // link descriptor (bounded lattices only)
     __device__ __forceinline__ bool is_wall_link(int link) const { return ((desc >> link) & 1U) != 0U; }
 };

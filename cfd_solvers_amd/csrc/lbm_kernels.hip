@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const vo
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     if constexpr (Bounded) {
         // Wavefronts whose whole neighbourhood is plain fluid take the periodic path: no link
         // loads, no bounce-back selects.  A wavefront covers one 64-cell row segment when
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void build_links(const uint8_t* __restrict__ f
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     const uint8_t own = flags[n];
     const uint32_t type = is_wall(own) ? kTypeWall : (own == kInlet ? 1U : (own == kOutlet ? 2U : 0U));
     constexpr int kW = PairSlots<Q>::kVelWords;
@@ -109,8 +109,8 @@ __global__ __launch_bounds__(256) void build_links(const uint8_t* __restrict__ f
     uint32_t vel[kW] = {};
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
-        const uint8_t fu = flags[h.cell(-L::cx[i], -L::cy[i], -L::cz[i])];
-        const uint8_t fd = flags[h.cell(L::cx[i], L::cy[i], L::cz[i])];
+        const uint8_t fu = flags[h.lin(-L::cx[i], -L::cy[i], -L::cz[i])];
+        const uint8_t fd = flags[h.lin(L::cx[i], L::cy[i], L::cz[i])];
         if (is_wall(fu)) {
             desc |= 1U << (i - 1);
             vel[(i - 1) / 10] |= static_cast<uint32_t>(fu & 7U) << (3 * ((i - 1) % 10));
@@ -150,7 +150,8 @@ __global__ __launch_bounds__(256) void initialize(void* __restrict__ f, const Ge
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);  // flags, fields, noise
+    const uint32_t np = h.cell(0, 0, 0);  // population slots
     Real rho = a.rho, ux = a.ux, uy = a.uy, uz = a.uz;
     if (a.mode == 1) {
         const Real* sx = a.tables;
@@ -176,12 +177,12 @@ __global__ __launch_bounds__(256) void initialize(void* __restrict__ f, const Ge
     if (flags && is_wall(flags[n])) ux = uy = uz = Real(0);
     const Real uu = kinetic_term<S>(ux, uy, uz);
     const Real rp = rho - Real(1);
-    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium_state<S, Q, 0>(rp, rho, ux, uy, uz, uu));
+    plane<T>(f, g, 0)[np] = S::template encode<Q, 0>(equilibrium_state<S, Q, 0>(rp, rho, ux, uy, uz, uu));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
         plane<T>(f, g, i)[up] = S::template encode<Q, i>(equilibrium_state<S, Q, i>(rp, rho, ux, uy, uz, uu));
-        plane<T>(f, g, i + 1)[n] =
+        plane<T>(f, g, i + 1)[np] =
             S::template encode<Q, i + 1>(equilibrium_state<S, Q, i + 1>(rp, rho, ux, uy, uz, uu));
     });
 }
@@ -196,8 +197,8 @@ __global__ __launch_bounds__(256) void upload_logical(void* __restrict__ f, cons
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
-    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(state_population<S, Q, 0>(in[n]));
+    const uint32_t n = h.lin(0, 0, 0), np = h.cell(0, 0, 0);
+    plane<T>(f, g, 0)[np] = S::template encode<Q, 0>(state_population<S, Q, 0>(in[n]));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
@@ -205,10 +206,10 @@ __global__ __launch_bounds__(256) void upload_logical(void* __restrict__ f, cons
         const auto vb = state_population<S, Q, i + 1>(in[static_cast<uint64_t>(i + 1) * cells + n]);
         if constexpr (Odd) {
             plane<T>(f, g, i + 1)[up] = S::template encode<Q, i + 1>(va);
-            plane<T>(f, g, i)[n] = S::template encode<Q, i>(vb);
+            plane<T>(f, g, i)[np] = S::template encode<Q, i>(vb);
         } else {
             plane<T>(f, g, i)[up] = S::template encode<Q, i>(va);
-            plane<T>(f, g, i + 1)[n] = S::template encode<Q, i + 1>(vb);
+            plane<T>(f, g, i + 1)[np] = S::template encode<Q, i + 1>(vb);
         }
     });
 }
@@ -226,18 +227,17 @@ __global__ __launch_bounds__(256) void bounce_back_slots(void* __restrict__ f, c
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
     const uint32_t n = h.cell(0, 0, 0);
-    if (is_wall(flags[n])) return;
+    if (is_wall(flags[h.lin(0, 0, 0)])) return;
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
-        const uint32_t dn = h.cell(L::cx[i], L::cy[i], L::cz[i]);
-        if (is_wall(flags[up])) {
+        if (is_wall(flags[h.lin(-L::cx[i], -L::cy[i], -L::cz[i])])) {
             T* normal = Odd ? plane<T>(f, g, i + 1) : plane<T>(f, g, i);
             T* other = Odd ? plane<T>(f, g, i) : plane<T>(f, g, i + 1);
             if (ToBounceBack) other[up] = normal[up];
             else normal[up] = other[up];
         }
-        if (is_wall(flags[dn])) {
+        if (is_wall(flags[h.lin(L::cx[i], L::cy[i], L::cz[i])])) {
             T* normal = Odd ? plane<T>(f, g, i) : plane<T>(f, g, i + 1);
             T* other = Odd ? plane<T>(f, g, i + 1) : plane<T>(f, g, i);
             if (ToBounceBack) other[n] = normal[n];
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void freeze_wall_density(const void* __restric
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     if (!is_wall(flags[n])) return;
     if (old_flags && is_wall(old_flags[n])) return;
     Real fin[Q];
@@ -282,12 +282,12 @@ __global__ __launch_bounds__(256) void thaw_walls(void* __restrict__ f, const Ge
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0), np = h.cell(0, 0, 0);
     if (!is_wall(old_flags[n]) || (flags && is_wall(flags[n]))) return;
     const Real rho = solid_rho[n];
     const Real rp = rho - Real(1);
     const Real zero = Real(0);
-    plane<T>(f, g, 0)[n] = S::template encode<Q, 0>(equilibrium_state<S, Q, 0>(rp, rho, zero, zero, zero, zero));
+    plane<T>(f, g, 0)[np] = S::template encode<Q, 0>(equilibrium_state<S, Q, 0>(rp, rho, zero, zero, zero, zero));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t up = h.cell(-L::cx[i], -L::cy[i], -L::cz[i]);
@@ -295,10 +295,10 @@ __global__ __launch_bounds__(256) void thaw_walls(void* __restrict__ f, const Ge
         const Real vb = equilibrium_state<S, Q, i + 1>(rp, rho, zero, zero, zero, zero);
         if constexpr (Odd) {
             plane<T>(f, g, i + 1)[up] = S::template encode<Q, i + 1>(va);
-            plane<T>(f, g, i)[n] = S::template encode<Q, i>(vb);
+            plane<T>(f, g, i)[np] = S::template encode<Q, i>(vb);
         } else {
             plane<T>(f, g, i)[up] = S::template encode<Q, i>(va);
-            plane<T>(f, g, i + 1)[n] = S::template encode<Q, i + 1>(vb);
+            plane<T>(f, g, i + 1)[np] = S::template encode<Q, i + 1>(vb);
         }
     });
 }
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void macroscopic(const void* __restrict__ f, c
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     const uint32_t cells = p.cells;
     uint8_t flag = kFluid;
     if constexpr (Bounded) {
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void download_logical(const void* __restrict__
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     if constexpr (Bounded) {
         if (is_wall(p.flags[n])) {
             const Real rho = p.solid_rho[n];
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void diagnostics_partial(const void* __restric
     uint32_t x, y, z;
     if (cell_coords(g, x, y, z)) {
         const Neighbourhood h = neighbourhood(g, x, y, z);
-        const uint32_t n = h.cell(0, 0, 0);
+        const uint32_t n = h.lin(0, 0, 0);
         uint8_t flag = kFluid;
         bool wall = false;
         if constexpr (Bounded) {
@@ -460,7 +460,8 @@ __global__ __launch_bounds__(256) void fill_ids(uint32_t* __restrict__ ids, cons
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const uint32_t n = (z * g.ny + y) * g.nx + x;
-    for (int d = 0; d < q; ++d) ids[static_cast<uint64_t>(d) * g.stride + n] = empty ? empty : d * cells + n;
+    const uint32_t np = row_base(g, z * g.ny + y) + x;
+    for (int d = 0; d < q; ++d) ids[static_cast<uint64_t>(d) * g.stride + np] = empty ? empty : d * cells + n;
 }
 
 template <int Q, bool Odd, bool Bounded>
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(256) void probe_load(const uint32_t* __restrict__ i
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     if (Bounded && is_wall(flags[n])) {
         for (int d = 0; d < Q; ++d) load[static_cast<uint64_t>(d) * cells + n] = -1;
         return;
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(256) void probe_store(uint32_t* __restrict__ ids, c
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
-    const uint32_t n = h.cell(0, 0, 0);
+    const uint32_t n = h.lin(0, 0, 0);
     if (Bounded && is_wall(p.flags[n])) return;
     uint32_t fin[Q];
     PairSlots<Q> s;

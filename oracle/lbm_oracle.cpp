@@ -500,6 +500,7 @@ private:
     std::vector<Store> F_;
     std::vector<uint8_t> flags_;
     std::vector<uint8_t> body_;  // drag body (wall cells), empty: none
+    std::vector<Real> scratch_cur_, scratch_nxt_;  // bounded-step scratch lattices
     int drag_axis_{0};
     std::vector<Real> gamma_, lax_, lay_, laz_;
     Real wall_u_[8][3];
@@ -753,7 +754,13 @@ private:
     // esoteric_pull.hpp:434-492 with the :483 repack corrected; moving walls per src/lbm/d2q9.cpp:217-223.
     void step_bounded() {
         const size_t total = static_cast<size_t>(Q) * N_;
-        std::vector<Real> cur(total), nxt(total);
+        // The reference's two scratch lattices, kept between steps (every entry is rewritten).
+        if (scratch_cur_.size() != total) {
+            scratch_cur_.assign(total, Real(0));
+            scratch_nxt_.assign(total, Real(0));
+        }
+        std::vector<Real>& cur = scratch_cur_;
+        std::vector<Real>& nxt = scratch_nxt_;
 #pragma omp parallel for schedule(static)
         for (int64_t nn = 0; nn < static_cast<int64_t>(N_); ++nn) {
             const size_t n = static_cast<size_t>(nn);
