@@ -473,21 +473,23 @@ __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometr
 
 // BGK relaxation of the state f (populations, or scaled deviations for shifted storage S) towards
 // the equilibrium of (rho, u): the reference's f - omega (f - f_eq) for unshifted storage, and
-// fma(-omega, f' - f_eq', f') with the pair-shared equilibrium for shifted storage.
+// fma(-omega, fma(-w_d, y_d, f'), f') with the pair-shared equilibrium terms for shifted storage.
 template <int Q, class S, class Real>
 __device__ __forceinline__ void relax(Real (&f)[Q], Real rp, Real rho, Real ux, Real uy, Real uz, Real uu,
                                       Real omega) {
     if constexpr (S::kShifted) {
-        f[0] = __builtin_fmaf(-omega, f[0] - scaled_weight<S, Q, 0, Real>() * __builtin_fmaf(rho, -uu, rp), f[0]);
+        // f' - f_eq' = fma(-w_d K, y_d, f') with y_d = fma(rho, x_d, rho'), one rounding.
+        const Real d0 = __builtin_fmaf(-scaled_weight<S, Q, 0, Real>(), __builtin_fmaf(rho, -uu, rp), f[0]);
+        f[0] = __builtin_fmaf(-omega, d0, f[0]);
         static_for<(Q - 1) / 2>([&](auto P) {
             constexpr int i = 2 * decltype(P)::value + 1;
             const Real cu = Real(3) * cdot<Q, i>(ux, uy, uz);
             const Real common = __builtin_fmaf(Real(0.5) * cu, cu, -uu);
             const Real wk = scaled_weight<S, Q, i, Real>();  // w_i = w_(i+1)
-            const Real feq_a = wk * __builtin_fmaf(rho, common + cu, rp);
-            const Real feq_b = wk * __builtin_fmaf(rho, common - cu, rp);
-            f[i] = __builtin_fmaf(-omega, f[i] - feq_a, f[i]);
-            f[i + 1] = __builtin_fmaf(-omega, f[i + 1] - feq_b, f[i + 1]);
+            const Real da = __builtin_fmaf(-wk, __builtin_fmaf(rho, common + cu, rp), f[i]);
+            const Real db = __builtin_fmaf(-wk, __builtin_fmaf(rho, common - cu, rp), f[i + 1]);
+            f[i] = __builtin_fmaf(-omega, da, f[i]);
+            f[i + 1] = __builtin_fmaf(-omega, db, f[i + 1]);
         });
     } else {
         static_for<Q>([&](auto DD) {

@@ -705,15 +705,16 @@ private:
         if (kShifted) {
             // Relaxation fma(-omega, f' - f_eq', f'); opposite directions share fma(cu/2, cu, -uu).
             const Real uu = shifted_uu(ux, uy, uz);
-            fout[0] = std::fma(-omega, fin[0] - w(0) * std::fma(rho, -uu, rp), fin[0]);
+            // f' - f_eq' = fma(-w_d, y_d, f') with y_d = fma(rho, x_d, rho'), one rounding.
+            fout[0] = std::fma(-omega, std::fma(-w(0), std::fma(rho, -uu, rp), fin[0]), fin[0]);
             for (int i = 1; i < Q; i += 2) {
                 const Real cu = Real(3) * (static_cast<Real>(L::cx[i]) * ux + static_cast<Real>(L::cy[i]) * uy +
                                            static_cast<Real>(L::cz[i]) * uz);
                 const Real common = std::fma(Real(0.5) * cu, cu, -uu);
-                const Real feq_a = w(i) * std::fma(rho, common + cu, rp);
-                const Real feq_b = w(i) * std::fma(rho, common - cu, rp);
-                fout[i] = std::fma(-omega, fin[i] - feq_a, fin[i]);
-                fout[i + 1] = std::fma(-omega, fin[i + 1] - feq_b, fin[i + 1]);
+                const Real da = std::fma(-w(i), std::fma(rho, common + cu, rp), fin[i]);
+                const Real db = std::fma(-w(i), std::fma(rho, common - cu, rp), fin[i + 1]);
+                fout[i] = std::fma(-omega, da, fin[i]);
+                fout[i + 1] = std::fma(-omega, db, fin[i + 1]);
             }
             for (int d = 0; d < Q; ++d) fout[d] += guo(d, rho, ux, uy, uz, ax, ay, az, omega);
         } else {
