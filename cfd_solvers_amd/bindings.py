@@ -67,6 +67,7 @@ _SIGNATURES = {
                             C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_double)],
     "cfd_lbm_adjoint_timing": [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "cfd_lbm_set_drag_body": [C.c_void_p, C.c_void_p, C.c_int],
+    "cfd_lbm_set_smagorinsky": [C.c_void_p, C.c_int],
     "cfd_lbm_objective_drag": [C.c_void_p, C.POINTER(C.c_double)],
     "cfd_lbm_adjoint_run_binomial": [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
                                      C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_double),
@@ -308,6 +309,10 @@ class EsotericPullHipSolver:
         self.lib.call("cfd_lbm_adjoint_run", self._h, steps, window, checkpoint_every, x_in, x_out, C.byref(J),
                       grad.ctypes.data if grad is not None else None, C.byref(du))
         return J.value, (grad.reshape(self.shape) if grad is not None else None), du.value
+
+    def set_smagorinsky(self, enabled=True):
+        """The reference's smagorinsky_les switch (esoteric_pull.hpp:72-97, 422-425)."""
+        self.lib.call("cfd_lbm_set_smagorinsky", self._h, int(enabled))
 
     def set_drag_body(self, body, axis=0):
         """Registers the wall cells of the drag body (nonzero entries of `body`, shape cells); None clears."""

@@ -74,6 +74,7 @@ struct cfd_lbm_solver {
     DeviceBuffer<unsigned char> f, flags, links, link_vel, interior, solid_rho, gamma, lacc, macro, partial, reduce_out;
     float wall_u[8][3] = {};
     float u_peak = 0.0F, rho_out = 1.0F;
+    bool les = false;  // Smagorinsky LES (esoteric_pull.hpp:72-97, InPlaceLbmConfig::smagorinsky_les)
     uint64_t t = 0;
     uint32_t step_lds = 0;
     std::vector<uint8_t> host_flags;  // copy of the flags (empty: periodic)
@@ -208,6 +209,8 @@ struct cfd_lbm_solver {
     template <class Real> void build_physics() {
         Physics<Real> p{};
         p.omega = Real(1) / static_cast<Real>(tau);
+        p.tau = static_cast<Real>(tau);
+        p.les = les ? 1U : 0U;
         p.ax = static_cast<Real>(ax);
         p.ay = static_cast<Real>(ay);
         p.az = static_cast<Real>(az);
@@ -552,6 +555,7 @@ struct cfd_lbm_solver {
     void adjoint_checks(uint64_t steps, uint64_t window) const {
         if (steps < 1 || window < 1 || window > steps) throw std::invalid_argument("need 1 <= window <= steps");
         if (ax != 0.0F || ay != 0.0F || az != 0.0F || lacc) throw std::invalid_argument("the adjoint supports unforced lattices only");
+        if (les) throw std::invalid_argument("the adjoint does not differentiate the Smagorinsky closure");
         for (uint8_t v : host_flags)
             if ((v & 0x80U) && (v & 7U) && (wall_u[v & 7U][0] != 0.0F || wall_u[v & 7U][1] != 0.0F || wall_u[v & 7U][2] != 0.0F))
                 throw std::invalid_argument("the adjoint supports walls at rest only");
@@ -719,6 +723,12 @@ struct cfd_lbm_solver {
         return J;
     }
 
+    void set_smagorinsky(bool enabled) {
+        if (enabled && (storage == 1 || storage == 3))
+            throw std::invalid_argument("Smagorinsky LES runs on FP32 or FP64 storage (the reference's arithmetic)");
+        les = enabled;
+    }
+
     void set_porosity(const float* g) {
         use_device();
         if (!g) {
@@ -747,7 +757,7 @@ struct cfd_lbm_solver {
     void step(uint64_t count) {
         use_device();
         const bool b = bounded();
-        const bool forced = ax != 0.0F || ay != 0.0F || az != 0.0F || static_cast<bool>(lacc);
+        const bool forced = ax != 0.0F || ay != 0.0F || az != 0.0F || static_cast<bool>(lacc) || les;
         LaunchState st = state();
         for (uint64_t k = 0; k < count; ++k) {
             ops->step(st, odd(), b, forced, f.ptr, f.ptr);
@@ -1081,6 +1091,10 @@ int cfd_lbm_objective_pressure_drop(cfd_lbm_solver* s, int x_in, int x_out, doub
         const double v = need(s).objective(cfd_hip::kPressureDrop, x_in, x_out);
         if (value) *value = v;
     });
+}
+
+int cfd_lbm_set_smagorinsky(cfd_lbm_solver* s, int enabled, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).set_smagorinsky(enabled != 0); });
 }
 
 int cfd_lbm_set_drag_body(cfd_lbm_solver* s, const uint8_t* body, int axis, char* error, size_t cap) {

@@ -340,6 +340,8 @@ __device__ __forceinline__ void moments(const Real (&f)[Q], Real& rho, Real& mx,
 
 template <class Real> struct Physics {
     Real omega;
+    Real tau;
+    uint32_t les;               // Smagorinsky relaxation time (general kernel variant only)
     Real ax, ay, az;
     Real u_peak, rho_out;
     Real wall_u[8][3];
@@ -471,6 +473,44 @@ __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometr
     uz = uz + Real(0.5) * az;
 }
 
+// Smagorinsky-Lilly relaxation time (esoteric_pull.hpp:72-97): Pi_ab = sum_d c_da c_db (f_d - f_eq_d),
+// Q = sum_ab Pi_ab^2 in row-major order, tau_local = (tau + sqrt(tau^2 + C sqrt(Q) / rho)) / 2 with
+// C = 16 sqrt(2) / (3 pi^2) evaluated in float.  The reference's additions of zero terms are
+// skipped (exact: they add +-0).  Unshifted storage only.
+template <int Q, class Real>
+__device__ __forceinline__ Real smagorinsky_tau(const Real (&f)[Q], Real rho, Real ux, Real uy, Real uz, Real uu,
+                                                Real tau) {
+    using L = Lattice<Q>;
+    if (!(rho > Real(0))) return tau;
+    Real pxx = Real(0), pxy = Real(0), pxz = Real(0), pyy = Real(0), pyz = Real(0), pzz = Real(0);
+    static_for<Q>([&](auto DD) {
+        constexpr int d = decltype(DD)::value;
+        constexpr int cx = L::cx[d], cy = L::cy[d], cz = L::cz[d];
+        const Real neq = f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu);
+        if constexpr (cx != 0) pxx = pxx + neq;
+        if constexpr (cy != 0) pyy = pyy + neq;
+        if constexpr (cz != 0) pzz = pzz + neq;
+        if constexpr (cx * cy > 0) pxy = pxy + neq;
+        if constexpr (cx * cy < 0) pxy = pxy - neq;
+        if constexpr (cx * cz > 0) pxz = pxz + neq;
+        if constexpr (cx * cz < 0) pxz = pxz - neq;
+        if constexpr (cy * cz > 0) pyz = pyz + neq;
+        if constexpr (cy * cz < 0) pyz = pyz - neq;
+    });
+    Real q = pxx * pxx;
+    q = q + pxy * pxy;
+    q = q + pxz * pxz;
+    q = q + pxy * pxy;
+    q = q + pyy * pyy;
+    q = q + pyz * pyz;
+    q = q + pxz * pxz;
+    q = q + pyz * pyz;
+    q = q + pzz * pzz;
+    constexpr float kPi = 3.14159265358979323846F;
+    const Real c = static_cast<Real>(16.0F * 1.41421356237309504880F / (3.0F * kPi * kPi));
+    return Real(0.5) * (tau + sqrt(tau * tau + c * sqrt(q) / rho));
+}
+
 // BGK relaxation of the state f (populations, or scaled deviations for shifted storage S) towards
 // the equilibrium of (rho, u): the reference's f - omega (f - f_eq) for unshifted storage, and
 // fma(-omega, fma(-w_d, y_d, f'), f') with the pair-shared equilibrium terms for shifted storage.
@@ -546,7 +586,9 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
         uz = uz + Real(0.5) * az;
     }
     const Real uu = kinetic_term<S>(ux, uy, uz);
-    const Real omega = p.omega;
+    Real omega = p.omega;
+    if constexpr (Forced && !S::kShifted)
+        if (p.les) omega = Real(1) / smagorinsky_tau<Q>(f, rho, ux, uy, uz, uu, p.tau);
     Real fin_copy[Q];
     if (p.gamma) {
         static_for<Q>([&](auto DD) { fin_copy[decltype(DD)::value] = f[decltype(DD)::value]; });

@@ -141,6 +141,10 @@ CFD_LBM_API int cfd_lbm_stream(cfd_lbm_solver* s, void** stream, char* error, si
 CFD_LBM_API int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int reset, char* error,
                                        size_t error_capacity);
 
+/* InPlaceLbmConfig::smagorinsky_les (esoteric_pull.hpp:28, closure at :72-97, used at :422-425):
+ * local relaxation time from the non-equilibrium momentum flux.  FP32 or FP64 storage. */
+CFD_LBM_API int cfd_lbm_set_smagorinsky(cfd_lbm_solver* s, int enabled, char* error, size_t error_capacity);
+
 /* Objective kinds of the adjoint entry points. */
 enum { CFD_LBM_OBJECTIVE_PRESSURE_DROP = 0, CFD_LBM_OBJECTIVE_DRAG = 1 };
 

@@ -74,11 +74,19 @@ public:
         config_.acceleration_x = config.acceleration_x;
         config_.acceleration_y = config.acceleration_y;
         config_.acceleration_z = config.acceleration_z;
-        if (config.smagorinsky_les) throw std::invalid_argument("the HIP backend has no Smagorinsky LES yet");
+        config_.smagorinsky_les = config.smagorinsky_les;
         call(cfd_lbm_create(q, config_.nx, config_.ny, config_.nz, config_.tau, config_.acceleration_x,
                             config_.acceleration_y, config_.acceleration_z, static_cast<int>(storage), &h_, err_,
                             sizeof(err_)));
         cells_ = config_.nx * config_.ny * config_.nz;
+        if (config_.smagorinsky_les) {
+            const int rc = cfd_lbm_set_smagorinsky(h_, 1, err_, sizeof(err_));
+            if (rc != 0) {
+                (void)cfd_lbm_destroy(h_, nullptr, 0);
+                h_ = nullptr;
+                throw std::invalid_argument(err_);
+            }
+        }
     }
 
     ~EsotericPullHipSolver() noexcept {

@@ -116,6 +116,10 @@ class Oracle:
     def set_wall_velocity(self, index, u):
         _check(self.L.orc_set_wall_velocity(self.h, index, *map(float, u)))
 
+    def set_smagorinsky(self, enabled=True):
+        self.L.orc_set_smagorinsky.argtypes = [C.c_void_p, C.c_int]
+        _check(self.L.orc_set_smagorinsky(self.h, int(enabled)))
+
     def set_gamma(self, gamma):
         if gamma is None:
             _check(self.L.orc_set_gamma(self.h, None))
@@ -229,6 +233,8 @@ def ref_lib(native=False):
         L.ref_last_error.restype = C.c_char_p
         L.ref_eso_create.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_float,
                                      C.c_float, C.c_float, C.POINTER(C.c_void_p)]
+        L.ref_eso_create_les.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, C.c_int, C.POINTER(C.c_void_p)]
         L.ref_eso_destroy.argtypes = [C.c_void_p]
         L.ref_eso_init_tg.argtypes = [C.c_void_p, C.c_float]
         L.ref_eso_init_uniform.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]
@@ -276,11 +282,11 @@ def ref_lib(native=False):
 class RefEsoteric:
     """The reference's EsotericPullSolver<D>, compiled from its sources."""
 
-    def __init__(self, q, nx, ny, nz, tau, accel=(0.0, 0.0, 0.0), native=False):
+    def __init__(self, q, nx, ny, nz, tau, accel=(0.0, 0.0, 0.0), native=False, smagorinsky=False):
         self.L = ref_lib(native)
         self.q = q
         h = C.c_void_p()
-        _check(self.L.ref_eso_create(q, nx, ny, nz, tau, *accel, C.byref(h)), self.L, "ref")
+        _check(self.L.ref_eso_create_les(q, nx, ny, nz, tau, *accel, int(smagorinsky), C.byref(h)), self.L, "ref")
         self.h = h
         self.cells = nx * ny * (nz if q != 9 else 1)
         self.shape = (nz if q != 9 else 1, ny, nx)

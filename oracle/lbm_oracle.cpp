@@ -199,6 +199,7 @@ public:
     virtual double adjoint_run(int kind, uint64_t steps, uint64_t window, int x_in, int x_out, double* grad_gamma,
                                double* grad_u_peak) = 0;
     virtual void set_drag_body(const uint8_t* body, int axis) = 0;
+    virtual void set_smagorinsky(bool enabled) = 0;
 };
 
 template <int Q, class Real, class Store> class Oracle final : public OracleBase {
@@ -264,6 +265,10 @@ public:
             lay_[n] = static_cast<Real>(ay[n]);
             laz_[n] = static_cast<Real>(az[n]);
         }
+    }
+    void set_smagorinsky(bool enabled) override {
+        if (enabled && kShifted) throw std::invalid_argument("Smagorinsky LES runs on FP32 or FP64 storage");
+        les_ = enabled;
     }
     void set_inlet_outlet(double u_peak, double rho_out) override {
         u_peak_ = static_cast<Real>(u_peak);
@@ -501,6 +506,7 @@ private:
     std::vector<uint8_t> flags_;
     std::vector<uint8_t> body_;  // drag body (wall cells), empty: none
     std::vector<Real> scratch_cur_, scratch_nxt_;  // bounded-step scratch lattices
+    bool les_{false};                              // Smagorinsky relaxation time
     int drag_axis_{0};
     std::vector<Real> gamma_, lax_, lay_, laz_;
     Real wall_u_[8][3];
@@ -547,6 +553,25 @@ private:
     }
     static Real feq_state(int d, Real rp, Real rho, Real ux, Real uy, Real uz) {
         return kShifted ? feq_dev(d, rp, rho, ux, uy, uz) : feq(d, rho, ux, uy, uz);
+    }
+    // esoteric_pull.hpp:72-97 (Smagorinsky-Lilly closure on the non-equilibrium momentum flux):
+    // Pi_ab = sum_d c_da c_db (f_d - f_eq_d), Q = sum_ab Pi_ab^2 (row-major), and
+    // tau_local = (tau + sqrt(tau^2 + C sqrt(Q) / rho)) / 2 with C = 16 sqrt(2) / (3 pi^2) in float.
+    Real smagorinsky_tau(const Real* fin, Real rho, Real ux, Real uy, Real uz) const {
+        if (!(rho > Real(0))) return tau_;
+        Real pi[3][3] = {};
+        for (int d = 0; d < Q; ++d) {
+            const Real neq = fin[d] - feq(d, rho, ux, uy, uz);
+            const Real e[3] = {static_cast<Real>(L::cx[d]), static_cast<Real>(L::cy[d]), static_cast<Real>(L::cz[d])};
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) pi[a][b] += e[a] * e[b] * neq;
+        }
+        Real q = Real(0);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) q += pi[a][b] * pi[a][b];
+        const float pi_f = 3.14159265358979323846F;
+        const Real c = static_cast<Real>(16.0F * 1.41421356237309504880F / (3.0F * pi_f * pi_f));
+        return Real(0.5) * (tau_ + std::sqrt(tau_ * tau_ + c * std::sqrt(q) / rho));
     }
     static Real guo(int d, Real rho, Real ux, Real uy, Real uz, Real ax, Real ay, Real az, Real omega) {
         if (ax == Real(0) && ay == Real(0) && az == Real(0)) return Real(0);
@@ -701,7 +726,7 @@ private:
         ux += Real(0.5) * ax;
         uy += Real(0.5) * ay;
         uz += Real(0.5) * az;
-        const Real omega = Real(1) / tau_;
+        const Real omega = Real(1) / (les_ ? smagorinsky_tau(fin, rho, ux, uy, uz) : tau_);
         if (kShifted) {
             // Relaxation fma(-omega, f' - f_eq', f'); opposite directions share fma(cu/2, cu, -uu).
             const Real uu = shifted_uu(ux, uy, uz);
@@ -1227,6 +1252,9 @@ ORC_API void orc_set_time_step(void* h, uint64_t t) { H(h)->set_time_step(t); }
 ORC_API size_t orc_cells(void* h) { return H(h)->cells(); }
 ORC_API int orc_objective(void* h, int kind, int x_in, int x_out, double* out) {
     return guarded([&] { *out = H(h)->objective(kind, x_in, x_out); });
+}
+ORC_API int orc_set_smagorinsky(void* h, int enabled) {
+    return guarded([&] { H(h)->set_smagorinsky(enabled != 0); });
 }
 ORC_API int orc_set_drag_body(void* h, const uint8_t* body, int axis) {
     return guarded([&] { H(h)->set_drag_body(body, axis); });

@@ -148,6 +148,17 @@ REF_API int ref_eso_create(int q, std::size_t nx, std::size_t ny, std::size_t nz
         else throw std::invalid_argument("q must be 9, 19 or 27");
     });
 }
+// The same with the reference's Smagorinsky-LES switch (InPlaceLbmConfig::smagorinsky_les).
+REF_API int ref_eso_create_les(int q, std::size_t nx, std::size_t ny, std::size_t nz, float tau, float ax, float ay,
+                               float az, int les, void** out) {
+    return guarded([&] {
+        const InPlaceLbmConfig c{nx, ny, nz, tau, ax, ay, az, les != 0};
+        if (q == 9) *out = static_cast<EsoBase*>(new Eso<D2Q9InPlaceDescriptor>(c));
+        else if (q == 19) *out = static_cast<EsoBase*>(new Eso<D3Q19Descriptor>(c));
+        else if (q == 27) *out = static_cast<EsoBase*>(new Eso<D3Q27Descriptor>(c));
+        else throw std::invalid_argument("q must be 9, 19 or 27");
+    });
+}
 REF_API void ref_eso_destroy(void* h) { delete static_cast<EsoBase*>(h); }
 REF_API int ref_eso_init_tg(void* h, float a) { return guarded([&] { static_cast<EsoBase*>(h)->init_tg(a); }); }
 REF_API int ref_eso_init_uniform(void* h, float r, float ux, float uy, float uz) {

@@ -406,3 +406,32 @@ def test_invalid_arguments_raise(hip):
         g.set_flags(np.full(g.cells, 0x05, dtype=np.uint8))
     with pytest.raises(RuntimeError, match="gamma"):
         g.set_porosity(np.full(g.cells, 1.5, dtype=np.float32))
+
+
+# ---------------------------------------------------------------------- Smagorinsky LES
+@pytest.mark.parametrize("q,storage,bounded", [(19, "fp32", False), (19, "fp32", True), (27, "fp32", False),
+                                               (9, "fp32", False), (19, "fp64", False)])
+def test_smagorinsky_parity(hip, oracle_mod, q, storage, bounded):
+    """The Smagorinsky closure (esoteric_pull.hpp:72-97) on the GPU, bit for bit against the oracle
+    (which is bit-exact against the reference's own LES build)."""
+    report("smagorinsky", q=q, storage=storage, bounded=bounded)
+    n = (32, 24, 1) if q == 9 else (20, 18, 16)
+    accel = (1e-5, 0.0, 0.0) if bounded else (0.0, 0.0, 0.0)
+    g, o = make_pair(hip, oracle_mod, q, n, 0.56, accel, storage=storage)
+    if bounded:
+        flags = channel_flags(*n, np.random.default_rng(7), 0.05)
+        g.set_flags(flags)
+        o.set_flags(flags)
+    g.set_smagorinsky(True)
+    o.set_smagorinsky(True)
+    g.initialize_taylor_green(0.05)
+    o.init_taylor_green(float(np.float32(0.05)))
+    g.step(24)
+    o.step(24)
+    assert_fields(gpu_fields(g, f64=storage == "fp64"), o.macroscopic())
+
+
+def test_smagorinsky_rejects_shifted_storage(hip):
+    g = hip.EsotericPullHipSolver(19, 8, 8, 8, 0.56, storage="fp16s")
+    with pytest.raises(RuntimeError, match="FP32 or FP64"):
+        g.set_smagorinsky(True)

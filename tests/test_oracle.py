@@ -513,3 +513,49 @@ def test_shifted_arithmetic_removes_the_fp32_low_velocity_bias():
     assert err["fp32"] > 5e-3
     assert err["fp32s"] < 1e-6
     assert err["fp16s"] < 2e-4
+
+
+# ---------------------------------------------------------------- Smagorinsky LES
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("q,n,accel", [(19, (20, 18, 16), (0.0, 0.0, 0.0)), (19, (20, 18, 16), (1e-5, 2e-6, 0.0)),
+                                       (27, (16, 14, 12), (0.0, 0.0, 0.0)), (9, (32, 24, 1), (0.0, 0.0, 0.0))])
+def test_smagorinsky_matches_live_reference(q, n, accel):
+    """The restated closure (esoteric_pull.hpp:72-97, 422-425) against the reference build with
+    smagorinsky_les set: raw storage bit for bit after 24 steps of the reference's own LES case."""
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    r = O.RefEsoteric(q, *n, 0.56, accel=accel, smagorinsky=True)
+    o = O.Oracle(q, *n, f32(0.56), tuple(f32(a) for a in accel), storage="fp32")
+    o.set_smagorinsky(True)
+    plain = O.Oracle(q, *n, f32(0.56), tuple(f32(a) for a in accel), storage="fp32")
+    for s in (r, o, plain):
+        s.init_taylor_green(0.05)
+        s.step(24)
+    assert np.array_equal(np.asarray(r.raw(), np.float32), o.raw().astype(np.float32))
+    assert not np.array_equal(o.raw(), plain.raw())  # the closure is active
+
+
+def test_smagorinsky_behaviour():
+    """The reference's LES checks (tests/test_v0141_lbm_les_qcriterion.cpp:26-62): mass is conserved
+    and Taylor-Green energy decays, faster than without the closure (it adds eddy viscosity); an
+    equilibrium state is left unchanged."""
+    def run(les):
+        o = O.Oracle(19, 20, 18, 16, float(np.float32(0.56)), storage="fp32")
+        o.set_smagorinsky(les)
+        o.init_taylor_green(0.05)
+        m0, e0 = o.mass(), o.kinetic_energy()
+        o.step(24)
+        return m0, e0, o.mass(), o.kinetic_energy()
+
+    m0, e0, m1, e1 = run(True)
+    _, _, _, e_plain = run(False)
+    assert abs(m1 - m0) / m0 < 5e-6
+    assert 0 < e1 < e_plain < e0
+    a = O.Oracle(19, 8, 8, 8, float(np.float32(0.56)), storage="fp32")
+    b = O.Oracle(19, 8, 8, 8, float(np.float32(0.56)), storage="fp32")
+    a.set_smagorinsky(True)
+    for s in (a, b):
+        s.init_uniform(1.0, 0.05, -0.02, 0.01)
+        s.step(5)
+    assert np.array_equal(a.raw(), b.raw())
+    with pytest.raises(RuntimeError, match="FP32 or FP64"):
+        O.Oracle(19, 8, 8, 8, 0.56, storage="fp16s").set_smagorinsky(True)
