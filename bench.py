@@ -1,14 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint[,config5]] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize[,config5]]
+                    [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
 u = (0.1, 0, 0), tau 0.55, uint8 flags) with FP32 population storage; further legs run the same
 geometry with FP16 and FP32 shifted-population storage, FP64 (the reference for the velocity
 error of the others) and D3Q27, and the adjoint leg runs the
-checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps).  The opt-in
+checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps), and the
+voxelize leg the GPU STL voxelisation of a 16128-triangle sphere on 512^3 cells.  The opt-in
 config5 leg runs forward + adjoint of a 768^3 FP16 duct past a sphere cluster (drag objective,
 binomial checkpointing) against the FP32-storage run.  Prints progress on stderr and ONE JSON
 line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
@@ -214,6 +216,37 @@ def config5_leg(hip, n=768, steps=1000, storages=("fp16s", "fp32")):
     return out
 
 
+def voxelize_leg(hip, n=512, reps=5):
+    """GPU STL voxelisation (cfd_lbm_voxelize_surface, src/lbm/voxelize.cpp:5): a closed
+    16128-triangle sphere (radius 0.37) on an n^3 grid over [0, 1]^3.  Kernel times are HIP events
+    on the call's stream (binning: count + scan + fill; lines: the per-line parity kernel); the
+    wall time includes the triangle upload and the mask download over PCIe.  The reference's CPU
+    figure for the same mesh is taken in cpu_baseline()."""
+    from cfd_solvers_amd import geometry as G
+
+    tris = G.sphere_surface((.5, .5, .5), .37, 64, 128)
+    hip.voxelize_surface(tris, (n, n, n))  # warmup
+    kernel, lines, bins, wall = [], [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m = hip.voxelize_surface(tris, (n, n, n))
+        wall.append((time.perf_counter() - t0) * 1e3)
+        b, l, pairs = hip.voxelize_timing()
+        bins.append(b)
+        lines.append(l)
+        kernel.append(b + l)
+    k_ms, l_ms, b_ms = (float(np.median(v)) for v in (kernel, lines, bins))
+    cells = n ** 3
+    res = {"workload": f"{len(tris)}-triangle sphere r=0.37 on {n}^3 cells in [0,1]^3", "cells": cells,
+           "triangles": len(tris), "solid_cells": int(m.sum()), "tile_pairs": int(pairs),
+           "bin_ms": b_ms, "lines_ms": l_ms, "kernel_ms": k_ms, "wall_ms": float(np.median(wall)),
+           "gcells_per_s": cells / (k_ms * 1e-3) / 1e9,
+           "mask_write_gbs": cells / (l_ms * 1e-3) / 1e9}
+    log(f"voxelize {n}^3, {len(tris)} triangles: bin {b_ms:.3f} ms + lines {l_ms:.3f} ms "
+        f"({res['gcells_per_s']:.1f} Gcells/s), wall {res['wall_ms']:.1f} ms incl. PCIe")
+    return res
+
+
 def copy_probe():
     import ctypes as C
 
@@ -228,7 +261,7 @@ def copy_probe():
     return {"bytes_per_copy": 2 * nbytes, "ms": ms.value, "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS}
 
 
-def cpu_baseline(target_seconds=15.0):
+def cpu_baseline(target_seconds=15.0, voxelize_mesh=None):
     """The reference's EsotericPullSolver<D3Q19> (oracle/_ref, built from the reference sources
     like its Release build) on the host cores: 256^3 periodic Taylor-Green, cfd-bench protocol
     (apps/cfd-bench.cpp:174-191) on a bounded number of steps.  Falls back to the CPU
@@ -265,9 +298,20 @@ def cpu_baseline(target_seconds=15.0):
     except OSError:
         pass
     log(f"cpu baseline ({kind}): {mlups:.1f} MLUPS on {threads} threads, {steps} steps in {dt:.1f} s")
-    return {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": kind,
-            "sample": f"D3Q19 256^3 periodic Taylor-Green (A 0.02, tau 0.6), 1 warmup + {steps} timed steps, "
-                      f"FP32, {threads} OpenMP threads, {cpu}"}
+    out = {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": kind,
+           "sample": f"D3Q19 256^3 periodic Taylor-Green (A 0.02, tau 0.6), 1 warmup + {steps} timed steps, "
+                     f"FP32, {threads} OpenMP threads, {cpu}"}
+    if voxelize_mesh is not None and oracle.ref_available(native=True):
+        # The reference's voxelize_surface (src/lbm/voxelize.cpp:5: every cell against every
+        # triangle, one thread) on a small grid with the voxelize leg's mesh.
+        vn = 40
+        t0 = time.perf_counter()
+        oracle.ref_voxelize(voxelize_mesh, (vn, vn, vn), native=True)
+        vdt = time.perf_counter() - t0
+        out["voxelize"] = {"value": vn ** 3 / vdt, "unit": "cells/s", "cores": 1, "kind": "reference",
+                           "sample": f"{vn}^3 cells in [0,1]^3, {len(voxelize_mesh)}-triangle sphere, {vdt:.1f} s"}
+        log(f"cpu baseline voxelize (reference): {vn ** 3 / vdt:.0f} cells/s on 1 thread ({vdt:.1f} s)")
+    return out
 
 
 def main():
@@ -276,7 +320,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint")
+    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize")
     ap.add_argument("--config5-steps", type=int, default=1000)
     ap.add_argument("--config5-n", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
@@ -300,6 +344,9 @@ def main():
             continue
         if name == "config5":
             legs[name] = config5_leg(hip, n=args.config5_n, steps=args.config5_steps)
+            continue
+        if name == "voxelize":
+            legs[name] = voxelize_leg(hip)
             continue
         lattice, storage = table[name]
         legs[name], vel = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
@@ -354,7 +401,16 @@ def main():
     if "fp16" in legs and "fp32" in legs:
         result["fp16_over_fp32"] = legs["fp16"]["mlups"] / legs["fp32"]["mlups"]
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline()
+        mesh = None
+        if "voxelize" in legs:
+            from cfd_solvers_amd import geometry as G
+
+            mesh = G.sphere_surface((.5, .5, .5), .37, 64, 128)
+        result["cpu_baseline"] = cpu_baseline(voxelize_mesh=mesh)
+        vox = result["cpu_baseline"].pop("voxelize", None)
+        if vox is not None:
+            legs["voxelize"]["cpu_reference"] = vox
+            legs["voxelize"]["gpu_over_cpu_reference"] = legs["voxelize"]["gcells_per_s"] * 1e9 / vox["value"]
     print(json.dumps(result), flush=True)
 
 

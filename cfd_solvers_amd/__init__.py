@@ -6,10 +6,12 @@ reference's ``python/cfd_solvers`` bindings.  Build the library with
 ``python -m cfd_solvers_amd.build``.
 """
 from .bindings import (EsotericPullHipSolver, MacroscopicFields, SolverLibrary, load, checkpoint_plan, FLUID,
-                       VELOCITY_INLET, PRESSURE_OUTLET, WALL, LATTICES, STORAGE, OBJECTIVES)
+                       VELOCITY_INLET, PRESSURE_OUTLET, WALL, LATTICES, STORAGE, OBJECTIVES, voxelize_surface,
+                       voxelize_timing, transform_surface)
 
 __all__ = ["EsotericPullHipSolver", "MacroscopicFields", "SolverLibrary", "load", "checkpoint_plan", "FLUID",
-           "VELOCITY_INLET", "PRESSURE_OUTLET", "WALL", "LATTICES", "STORAGE", "OBJECTIVES"]
+           "VELOCITY_INLET", "PRESSURE_OUTLET", "WALL", "LATTICES", "STORAGE", "OBJECTIVES", "voxelize_surface",
+           "voxelize_timing", "transform_surface"]
 
 # torch.autograd.Function over the adjoint: `from cfd_solvers_amd.autograd import LbmObjective`
 # (imported on demand so that the ctypes layer does not need torch).

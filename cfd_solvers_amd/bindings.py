@@ -42,6 +42,7 @@ _SIGNATURES = {
     "cfd_lbm_upload_populations": [C.c_void_p, _f32p],
     "cfd_lbm_download_populations": [C.c_void_p, _f32p],
     "cfd_lbm_set_flags": [C.c_void_p, C.c_void_p],
+    "cfd_lbm_get_flags": [C.c_void_p, C.c_void_p],
     "cfd_lbm_set_wall_velocity": [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float],
     "cfd_lbm_set_porosity": [C.c_void_p, C.c_void_p],
     "cfd_lbm_set_inlet_outlet": [C.c_void_p, C.c_float, C.c_float],
@@ -74,6 +75,10 @@ _SIGNATURES = {
                                      C.POINTER(C.c_uint64)],
     "cfd_lbm_checkpoint_plan": [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_uint64)],
+    "cfd_lbm_voxelize_surface": [_f64p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _f64p, _f64p, C.c_void_p],
+    "cfd_lbm_voxelize_timing": [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+    "cfd_lbm_transform_surface": [_f64p, C.c_size_t, _f64p, _f64p, C.c_double, _f64p],
+    "cfd_lbm_set_solid_surface": [C.c_void_p, _f64p, C.c_size_t, _f64p, _f64p, C.c_int],
 }
 
 OBJECTIVES = {"pressure_drop": 0, "drag": 1}
@@ -185,12 +190,26 @@ class EsotericPullHipSolver:
             raise ValueError("LBM flag field size mismatch")
         self.lib.call("cfd_lbm_set_flags", self._h, f.ctypes.data)
 
+    def flags(self):
+        """The solver's flag field (uint8, shape (nz, ny, nx)); all FLUID when none are set."""
+        out = np.empty(self.cells, np.uint8)
+        self.lib.call("cfd_lbm_get_flags", self._h, out.ctypes.data)
+        return out.reshape(self.shape)
+
     def set_solid_mask(self, mask):
         m = np.asarray(mask).reshape(-1)
         self.set_flags(np.where(m != 0, WALL, FLUID).astype(np.uint8))
 
     def clear_solid_mask(self):
         self.set_flags(None)
+
+    def set_solid_surface(self, triangles, minimum=(0.0, 0.0, 0.0), maximum=(1.0, 1.0, 1.0), keep_flags=False):
+        """set_solid_mask(voxelize_surface(triangles, grid)) on the device (examples/real_lbm3d_model.cpp:
+        146-151): the grid is this lattice's nx, ny, nz over [minimum, maximum]; cells inside the
+        closed surface become walls at rest.  keep_flags keeps the other cells' current flags."""
+        t = _triangles(triangles)
+        self.lib.call("cfd_lbm_set_solid_surface", self._h, t, len(t) // 9, _vec3(minimum), _vec3(maximum),
+                      1 if keep_flags else 0)
 
     def set_wall_velocity(self, index, u):
         self.lib.call("cfd_lbm_set_wall_velocity", self._h, index, *map(float, u))
@@ -358,3 +377,46 @@ def checkpoint_plan(steps, snapshots, library=None):
     fw, r, walked = C.c_uint64(), C.c_uint64(), C.c_uint64()
     lib.call("cfd_lbm_checkpoint_plan", steps, snapshots, C.byref(fw), C.byref(r), C.byref(walked))
     return fw.value, r.value, walked.value
+
+
+def _triangles(triangles):
+    t = np.ascontiguousarray(triangles, dtype=np.float64)
+    if t.ndim == 0 or t.size % 9:
+        raise ValueError("triangles must hold 9 coordinates each (shape (count, 3, 3) or (count, 9))")
+    return t.reshape(-1)
+
+
+def _vec3(v):
+    a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    if a.size != 3:
+        raise ValueError("expected three coordinates")
+    return a
+
+
+def voxelize_surface(triangles, shape, minimum=(0.0, 0.0, 0.0), maximum=(1.0, 1.0, 1.0), library=None):
+    """voxelize_surface (src/lbm/voxelize.cpp:5) on the GPU: the ray-parity inside mask of a closed
+    triangle surface (triangles: (count, 3, 3) vertex coordinates) on the grid of shape (nx, ny, nz)
+    spanning [minimum, maximum]; uint8 array of shape (nz, ny, nx), equal to the reference's CPU
+    mask bit for bit."""
+    nx, ny, nz = (int(n) for n in shape)
+    t = _triangles(triangles)
+    mask = np.empty((nz, ny, nx), np.uint8)
+    load(library).call("cfd_lbm_voxelize_surface", t, len(t) // 9, nx, ny, nz, _vec3(minimum), _vec3(maximum),
+                       mask.ctypes.data)
+    return mask
+
+
+def voxelize_timing(library=None):
+    """(binning ms, line kernel ms, triangle-tile pairs) of this thread's last voxelisation."""
+    b, l, n = C.c_double(), C.c_double(), C.c_uint64()
+    load(library).call("cfd_lbm_voxelize_timing", C.byref(b), C.byref(l), C.byref(n))
+    return b.value, l.value, n.value
+
+
+def transform_surface(triangles, translation, axis, angle, library=None):
+    """transform_surface (src/lbm/voxelize.cpp:6): rotate about `axis` by `angle` radians, then
+    translate; returns a new (count, 3, 3) float64 array."""
+    t = _triangles(triangles)
+    out = np.empty_like(t)
+    load(library).call("cfd_lbm_transform_surface", t, len(t) // 9, _vec3(translation), _vec3(axis), float(angle), out)
+    return out.reshape(-1, 3, 3)

@@ -9,11 +9,13 @@
 #include "checkpoint_schedule.hpp"
 #include "lbm_adjoint.hpp"
 #include "lbm_kernels.hpp"
+#include "voxelize.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -21,6 +23,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -394,13 +397,21 @@ struct cfd_lbm_solver {
         }
         if (host_flags) host_flags_assign(host_flags);
         else this->host_flags.clear();
+        install_flags(fresh);
+    }
+
+    // Makes the device flags in `fresh` (empty: periodic, no flags) the solver's flags: walls that
+    // stop being walls are thawed, new walls frozen, links and the interior map rebuilt.  The host
+    // copy (host_flags) must already hold the same values.
+    void install_flags(DeviceBuffer<unsigned char>& fresh) {
+        const bool have = static_cast<bool>(fresh);
         clear_drag_body();  // the body's fluid neighbours depend on the flags
         LaunchState st = state();
         if (bounded()) {
             ops->bounce_back(st, odd(), false, flags.ptr);  // back to the plain layout
             ops->thaw(st, odd(), flags.ptr, fresh.ptr, solid_rho.ptr);
         }
-        if (!host_flags) {
+        if (!have) {
             sync();
             flags.reset();
             links.reset();
@@ -423,6 +434,29 @@ struct cfd_lbm_solver {
     }
 
     void host_flags_assign(const uint8_t* v) { this->host_flags.assign(v, v + cells); }
+
+    // set_solid_mask(voxelize_surface(surface, grid)) (examples/real_lbm3d_model.cpp:146-151) on the
+    // device: the mask is voxelised straight into the new flag array, solid cells become walls at
+    // rest, the others keep their flags (keep) or become fluid.
+    void set_solid_surface(const double* triangles, size_t count, const double* minimum, const double* maximum,
+                           bool keep) {
+        if (!minimum || !maximum) throw std::invalid_argument("voxelization needs the grid's minimum and maximum");
+        use_device();
+        cfd_hip::VoxelGridArgs g;
+        g.nx = nx;
+        g.ny = ny;
+        g.nz = nz;
+        for (int a = 0; a < 3; ++a) g.minimum[a] = minimum[a], g.maximum[a] = maximum[a];
+        DeviceBuffer<unsigned char> fresh;
+        fresh.allocate(cells);
+        cfd_hip::voxelize_surface(triangles, count, g, fresh.ptr, stream);
+        cfd_hip::solid_to_flags(fresh.ptr, keep && flags ? flags.ptr : nullptr, cells, stream);
+        std::vector<uint8_t> mirror(cells);
+        d2h(mirror.data(), fresh.ptr, cells);
+        sync();
+        host_flags.swap(mirror);
+        install_flags(fresh);
+    }
 
     // Pressure drop between fluid cells of planes x_in and x_out, as a mean over `window` states.
     cfd_hip::AdjointArgs pressure_args(int x_in, int x_out, uint64_t window) const {
@@ -950,6 +984,14 @@ int cfd_lbm_set_flags(cfd_lbm_solver* s, const uint8_t* flags, char* error, size
     return guarded(error, cap, [&] { need(s).set_flags(flags); });
 }
 
+int cfd_lbm_get_flags(cfd_lbm_solver* s, uint8_t* flags, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (!flags) throw std::invalid_argument("get_flags requires cells bytes");
+        if (need(s).host_flags.empty()) std::memset(flags, 0, s->cells);
+        else std::memcpy(flags, s->host_flags.data(), s->cells);
+    });
+}
+
 int cfd_lbm_set_wall_velocity(cfd_lbm_solver* s, int index, float ux, float uy, float uz, char* error, size_t cap) {
     return guarded(error, cap, [&] {
         if (index < 1 || index > 7) throw std::invalid_argument("wall velocity index must be 1..7");
@@ -1145,6 +1187,49 @@ int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_
         if (forward_ms) *forward_ms = need(s).last_forward_ms;
         if (reverse_ms) *reverse_ms = s->last_reverse_ms;
     });
+}
+
+int cfd_lbm_voxelize_surface(const double* triangles, size_t count, size_t nx, size_t ny, size_t nz,
+                             const double* minimum, const double* maximum, uint8_t* mask, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (!minimum || !maximum || !mask) throw std::invalid_argument("voxelization needs minimum, maximum and a mask");
+        cfd_hip::VoxelGridArgs g;
+        g.nx = nx;
+        g.ny = ny;
+        g.nz = nz;
+        for (int a = 0; a < 3; ++a) g.minimum[a] = minimum[a], g.maximum[a] = maximum[a];
+        cfd_hip::validate_voxelization(triangles, count, g);
+        if (nx > SIZE_MAX / ny || nx * ny > SIZE_MAX / nz) throw std::invalid_argument("voxelization grid too large");
+        const size_t cells = nx * ny * nz;
+        hipStream_t stream = nullptr;
+        hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+        std::unique_ptr<std::remove_pointer<hipStream_t>::type, void (*)(hipStream_t)> own(
+            stream, [](hipStream_t st) { (void)hipStreamDestroy(st); });
+        DeviceBuffer<unsigned char> d_mask;
+        d_mask.allocate(std::max<size_t>(cells, 1));
+        cfd_hip::voxelize_surface(triangles, count, g, d_mask.ptr, stream);
+        hip_check(hipMemcpyAsync(mask, d_mask.ptr, cells, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    });
+}
+
+int cfd_lbm_voxelize_timing(double* bin_ms, double* lines_ms, uint64_t* pairs, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        const cfd_hip::VoxelizeTiming t = cfd_hip::last_voxelize_timing();
+        if (bin_ms) *bin_ms = t.bin_ms;
+        if (lines_ms) *lines_ms = t.lines_ms;
+        if (pairs) *pairs = t.pairs;
+    });
+}
+
+int cfd_lbm_transform_surface(const double* triangles, size_t count, const double* translation, const double* axis,
+                              double angle, double* out, char* error, size_t cap) {
+    return guarded(error, cap, [&] { cfd_hip::transform_surface(triangles, count, translation, axis, angle, out); });
+}
+
+int cfd_lbm_set_solid_surface(cfd_lbm_solver* s, const double* triangles, size_t count, const double* minimum,
+                              const double* maximum, int keep_flags, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).set_solid_surface(triangles, count, minimum, maximum, keep_flags != 0); });
 }
 
 int cfd_lbm_index_map(cfd_lbm_solver* s, int parity, int64_t* load, int64_t* store, char* error, size_t cap) {

@@ -11,8 +11,14 @@
 * ``porosity_halo``: a partial-bounce-back blend that ramps from ``gamma_max`` at a sphere's surface
   to zero ``width`` cells out (the halo of a porous coating around the solid cluster).
 
-The reference builds wall masks from STL voxelisation (src/lbm/voxelize.cpp); these generators
-stand in for it with reproducible random packings.
+Triangle surfaces (float64 arrays of shape (count, 3, 3): triangle, vertex a/b/c, x/y/z) feed the
+GPU voxeliser (``voxelize_surface`` / ``EsotericPullHipSolver.set_solid_surface``, the reference's
+src/lbm/voxelize.cpp):
+
+* ``read_ascii_stl``: the reference's ASCII STL reader (src/io/mesh_io.cpp:18): every ``vertex``
+  keyword followed by three numbers, grouped in threes.
+* ``box_surface``, ``sphere_surface``, ``torus_surface``, ``prism_surface``: closed, consistently
+  oriented meshes of a box, a UV sphere, a torus and a triangle extruded along x.
 """
 from __future__ import annotations
 
@@ -114,3 +120,87 @@ def porosity_halo(shape, centres, radii, width, gamma_max):
         sub = g[z0:z1 + 1, y0:y1 + 1, x0:x1 + 1]
         np.maximum(sub, val, out=sub)
     return g
+
+
+def read_ascii_stl(path):
+    """Triangles of an ASCII STL file as a (count, 3, 3) float64 array (src/io/mesh_io.cpp:18):
+    the three numbers after each ``vertex`` keyword, grouped in threes; a trailing incomplete
+    triangle is an error."""
+    try:
+        text = open(path, "r", encoding="utf-8", errors="replace").read()
+    except OSError as e:
+        raise RuntimeError("cannot open STL") from e
+    words = text.split()
+    verts = []
+    i = 0
+    while i < len(words):
+        if words[i] == "vertex":
+            if i + 3 >= len(words):
+                raise RuntimeError("malformed ASCII STL")
+            verts.append([float(w) for w in words[i + 1:i + 4]])
+            i += 4
+        else:
+            i += 1
+    if len(verts) % 3:
+        raise RuntimeError("malformed ASCII STL")
+    return np.asarray(verts, np.float64).reshape(-1, 3, 3)
+
+
+def box_surface(lo, hi):
+    """12 outward-oriented triangles of the axis-aligned box [lo, hi]."""
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    p = np.array([[x0, y0, z0], [x1, y0, z0], [x0, y1, z0], [x1, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x0, y1, z1], [x1, y1, z1]], np.float64)
+    faces = [(0, 3, 1), (0, 2, 3), (4, 5, 7), (4, 7, 6), (0, 1, 5), (0, 5, 4),
+             (2, 6, 7), (2, 7, 3), (0, 4, 6), (0, 6, 2), (1, 3, 7), (1, 7, 5)]
+    return p[np.array(faces)]
+
+
+def prism_surface(x0, x1, triangle_yz):
+    """8 triangles of the prism x0 <= x <= x1 over the (y, z) triangle ((ay, az), (by, bz), (cy, cz))."""
+    (ay, az), (by, bz), (cy, cz) = triangle_yz
+    a0, b0, c0 = [x0, ay, az], [x0, by, bz], [x0, cy, cz]
+    a1, b1, c1 = [x1, ay, az], [x1, by, bz], [x1, cy, cz]
+    return np.array([(a0, c0, b0), (a1, b1, c1), (a0, b0, b1), (a0, b1, a1), (b0, c0, c1), (b0, c1, b1),
+                     (c0, a0, a1), (c0, a1, c1)], np.float64)
+
+
+def sphere_surface(centre, radius, n_theta=16, n_phi=32):
+    """Closed UV sphere: n_phi segments around z, n_theta bands from pole to pole."""
+    c = np.asarray(centre, np.float64)
+    th = np.linspace(0.0, np.pi, n_theta + 1)
+    ph = np.linspace(0.0, 2.0 * np.pi, n_phi + 1)[:-1]
+    ring = lambda t: np.stack([np.sin(t) * np.cos(ph), np.sin(t) * np.sin(ph), np.full_like(ph, np.cos(t))], -1)
+    tris = []
+    top, bottom = c + radius * np.array([0.0, 0.0, 1.0]), c + radius * np.array([0.0, 0.0, -1.0])
+    for b in range(n_theta):
+        r0 = c + radius * ring(th[b])
+        r1 = c + radius * ring(th[b + 1])
+        for k in range(n_phi):
+            k1 = (k + 1) % n_phi
+            if b == 0:
+                tris.append([top, r1[k], r1[k1]])
+            elif b == n_theta - 1:
+                tris.append([r0[k], bottom, r0[k1]])
+            else:
+                tris.append([r0[k], r1[k], r1[k1]])
+                tris.append([r0[k], r1[k1], r0[k1]])
+    return np.asarray(tris, np.float64)
+
+
+def torus_surface(centre, major, minor, n_major=32, n_minor=16):
+    """Closed torus around the z axis through `centre`."""
+    c = np.asarray(centre, np.float64)
+    u = np.linspace(0.0, 2.0 * np.pi, n_major + 1)[:-1]
+    v = np.linspace(0.0, 2.0 * np.pi, n_minor + 1)[:-1]
+    U, V = np.meshgrid(u, v, indexing="ij")
+    pts = np.stack([(major + minor * np.cos(V)) * np.cos(U), (major + minor * np.cos(V)) * np.sin(U),
+                    minor * np.sin(V)], -1) + c
+    tris = []
+    for a in range(n_major):
+        a1 = (a + 1) % n_major
+        for b in range(n_minor):
+            b1 = (b + 1) % n_minor
+            tris.append([pts[a, b], pts[a1, b], pts[a1, b1]])
+            tris.append([pts[a, b], pts[a1, b1], pts[a, b1]])
+    return np.asarray(tris, np.float64)

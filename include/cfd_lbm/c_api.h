@@ -86,6 +86,9 @@ CFD_LBM_API int cfd_lbm_download_populations(cfd_lbm_solver* s, float* logical, 
 /* set_solid_mask / clear_solid_mask -- esoteric_pull.hpp:163-175, generalised to uint8 flags
  * (cells values; NULL clears).  Mask value 1 in the reference maps to CFD_LBM_WALL. */
 CFD_LBM_API int cfd_lbm_set_flags(cfd_lbm_solver* s, const uint8_t* flags, char* error, size_t error_capacity);
+/* The solver's flags (cells bytes; all CFD_LBM_FLUID when none are set) -- the flag form of
+ * solid_mask() (esoteric_pull.hpp:177-178).  Host copy, no device transfer. */
+CFD_LBM_API int cfd_lbm_get_flags(cfd_lbm_solver* s, uint8_t* flags, char* error, size_t error_capacity);
 /* Wall velocity for flag CFD_LBM_WALL|index (index 1..7); src/lbm/d2q9.cpp:109-115,217-223. */
 CFD_LBM_API int cfd_lbm_set_wall_velocity(cfd_lbm_solver* s, int index, float ux, float uy, float uz, char* error,
                                           size_t error_capacity);
@@ -191,6 +194,35 @@ CFD_LBM_API int cfd_lbm_checkpoint_plan(uint64_t steps, uint64_t snapshots, uint
 /* GPU time of the last adjoint run's forward phase and reverse (recompute + transposed) phase. */
 CFD_LBM_API int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_ms, char* error,
                                        size_t error_capacity);
+
+/* voxelize_surface -- src/lbm/voxelize.cpp:5 (include/cfd/solvers/lbm/voxelize.hpp:9), run on the GPU
+ * as the reference's device variant does (src/sycl/voxelize_sycl.cpp:13-68).  triangles: count x 9
+ * doubles (a.x a.y a.z b.x b.y b.z c.x c.y c.z, the layout of cfd::io::Triangle3, host memory);
+ * minimum, maximum: 3 doubles each (VoxelGrid3D); mask: nx*ny*nz bytes (host), cell (i,j,k) at
+ * (k*ny + j)*nx + i, 1 where the +x ray from the cell centre crosses the surface an odd number
+ * of times.  Double precision in the reference's CPU operation order: the mask equals
+ * voxelize_surface's bit for bit.  Errors: "invalid voxelization inputs" as voxelize.cpp:5, and
+ * non-finite vertex coordinates. */
+CFD_LBM_API int cfd_lbm_voxelize_surface(const double* triangles, size_t count, size_t nx, size_t ny, size_t nz,
+                                         const double* minimum, const double* maximum, uint8_t* mask, char* error,
+                                         size_t error_capacity);
+/* GPU time of the calling thread's last voxelisation: binning of triangles to line tiles, the
+ * line kernel, and the number of (triangle, tile) pairs binned. */
+CFD_LBM_API int cfd_lbm_voxelize_timing(double* bin_ms, double* lines_ms, uint64_t* pairs, char* error,
+                                        size_t error_capacity);
+/* transform_surface -- src/lbm/voxelize.cpp:6 (voxelize.hpp:10): each vertex rotated about `axis`
+ * through the origin by `angle` radians (Rodrigues; an axis shorter than 1e-30 leaves it), then
+ * translated.  Host arithmetic in the reference's operation order; out may equal triangles. */
+CFD_LBM_API int cfd_lbm_transform_surface(const double* triangles, size_t count, const double* translation,
+                                          const double* axis, double angle, double* out, char* error,
+                                          size_t error_capacity);
+/* set_solid_mask(voxelize_surface(surface, grid)) as examples/real_lbm3d_model.cpp:146-151 does,
+ * device-resident: the solver's nx, ny, nz with the grid's minimum and maximum; solid cells become
+ * CFD_LBM_WALL (at rest).  keep_flags != 0 keeps the current flags of the other cells (inlets,
+ * outlets, channel walls); 0 makes them fluid, as set_solid_mask does. */
+CFD_LBM_API int cfd_lbm_set_solid_surface(cfd_lbm_solver* s, const double* triangles, size_t count,
+                                          const double* minimum, const double* maximum, int keep_flags, char* error,
+                                          size_t error_capacity);
 
 /* Debug: the slots (d'*cells + n') read (load) and written (store) for every (d, n) by the
  * step kernel at the given parity, computed by the kernel's own index code (q*cells each).

@@ -59,6 +59,36 @@ struct HipTransferStats {
     }
 };
 
+// Triangles of any surface type with the shape of cfd::io::TriangleSurface (a `triangles` range of
+// {a, b, c} points with x, y, z; include/cfd/io/mesh_io.hpp:9-10) as 9 doubles each.
+template <class Surface> std::vector<double> packed_triangles(const Surface& surface) {
+    std::vector<double> out;
+    out.reserve(surface.triangles.size() * 9);
+    for (const auto& t : surface.triangles)
+        for (const auto* p : {&t.a, &t.b, &t.c}) {
+            out.push_back(static_cast<double>(p->x));
+            out.push_back(static_cast<double>(p->y));
+            out.push_back(static_cast<double>(p->z));
+        }
+    return out;
+}
+
+// voxelize_surface (src/lbm/voxelize.cpp:5) on the GPU.  Grid is any struct with the fields of
+// VoxelGrid3D (nx, ny, nz, minimum, maximum; include/cfd/solvers/lbm/voxelize.hpp:8).  The mask is
+// the reference's bit for bit.
+template <class Surface, class Grid>
+[[nodiscard]] std::vector<unsigned char> voxelize_surface_hip(const Surface& surface, const Grid& grid) {
+    const std::vector<double> t = packed_triangles(surface);
+    const double lo[3] = {grid.minimum.x, grid.minimum.y, grid.minimum.z};
+    const double hi[3] = {grid.maximum.x, grid.maximum.y, grid.maximum.z};
+    std::vector<unsigned char> mask(grid.nx * grid.ny * grid.nz);
+    char err[512] = {};
+    if (cfd_lbm_voxelize_surface(t.data(), t.size() / 9, grid.nx, grid.ny, grid.nz, lo, hi, mask.data(), err,
+                                 sizeof(err)) != 0)
+        throw std::invalid_argument(err);
+    return mask;
+}
+
 template <class Descriptor> class EsotericPullHipSolver {
 public:
     static constexpr int q = Descriptor::q;
@@ -124,6 +154,21 @@ public:
         mask_.resize(cells_);
         for (std::size_t i = 0; i < cells_; ++i) mask_[i] = (flags_[i] & CFD_LBM_WALL) != 0U ? 1U : 0U;
         call(cfd_lbm_set_flags(h_, flags_.data(), err_, sizeof(err_)));
+    }
+    // set_solid_mask(voxelize_surface(surface, grid)) (examples/real_lbm3d_model.cpp:146-151) on the
+    // device; the grid's cell counts are this solver's, minimum/maximum any points with x, y, z.
+    // keep_flags keeps the current flags of the cells outside the surface.
+    template <class Surface, class Point>
+    void set_solid_surface(const Surface& surface, const Point& minimum, const Point& maximum,
+                           bool keep_flags = false) {
+        const std::vector<double> t = packed_triangles(surface);
+        const double lo[3] = {minimum.x, minimum.y, minimum.z};
+        const double hi[3] = {maximum.x, maximum.y, maximum.z};
+        call(cfd_lbm_set_solid_surface(h_, t.data(), t.size() / 9, lo, hi, keep_flags ? 1 : 0, err_, sizeof(err_)));
+        flags_.resize(cells_);
+        call(cfd_lbm_get_flags(h_, flags_.data(), err_, sizeof(err_)));
+        mask_.resize(cells_);
+        for (std::size_t i = 0; i < cells_; ++i) mask_[i] = (flags_[i] & CFD_LBM_WALL) != 0U ? 1U : 0U;
     }
     void clear_solid_mask() {
         flags_.clear();

@@ -78,6 +78,8 @@ def lib():
         L.orc_f16_to_f32.restype = C.c_float
         L.orc_uniform01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_uniform01.restype = C.c_double
+        L.orc_voxelize.argtypes = [_dp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp, _u8p]
+        L.orc_transform_surface.argtypes = [_dp, C.c_size_t, _dp, _dp, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -215,6 +217,49 @@ def uniform01(seed, cell, comp):
     return lib().orc_uniform01(seed, cell, comp)
 
 
+def _tri_args(triangles):
+    t = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1)
+    assert t.size % 9 == 0
+    return t, t.size // 9
+
+
+def _vox(fn, L, prefix, triangles, shape, lo, hi):
+    nx, ny, nz = shape
+    t, n = _tri_args(triangles)
+    mask = np.zeros(nx * ny * nz, np.uint8)
+    _check(fn(t, n, nx, ny, nz, np.asarray(lo, np.float64), np.asarray(hi, np.float64), mask), L, prefix)
+    return mask.reshape(nz, ny, nx)
+
+
+def _xform(fn, L, prefix, triangles, translation, axis, angle):
+    t, n = _tri_args(triangles)
+    out = np.empty_like(t)
+    _check(fn(t, n, np.asarray(translation, np.float64), np.asarray(axis, np.float64), float(angle), out), L, prefix)
+    return out.reshape(-1, 3, 3)
+
+
+def voxelize(triangles, shape, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    """Ray-parity mask (nz, ny, nx) of voxelize.cpp:5, restated (every cell against every triangle)."""
+    L = lib()
+    return _vox(L.orc_voxelize, L, "orc", triangles, shape, lo, hi)
+
+
+def transform_surface(triangles, translation, axis, angle):
+    L = lib()
+    return _xform(L.orc_transform_surface, L, "orc", triangles, translation, axis, angle)
+
+
+def ref_voxelize(triangles, shape, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), native=False):
+    """The reference's own cfd::lbm::voxelize_surface, compiled from src/lbm/voxelize.cpp."""
+    L = ref_lib(native)
+    return _vox(L.ref_voxelize, L, "ref", triangles, shape, lo, hi)
+
+
+def ref_transform_surface(triangles, translation, axis, angle):
+    L = ref_lib()
+    return _xform(L.ref_transform_surface, L, "ref", triangles, translation, axis, angle)
+
+
 # ---------------------------------------------------------------- compiled reference (oracle/_ref)
 _ref = {}
 
@@ -275,6 +320,8 @@ def ref_lib(native=False):
         L.ref_d2q9_fields.argtypes = [C.c_void_p, _fp, _fp, _fp]
         L.ref_d2q9_mass.argtypes = [C.c_void_p]
         L.ref_d2q9_mass.restype = C.c_double
+        L.ref_voxelize.argtypes = [_dp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp, _u8p]
+        L.ref_transform_surface.argtypes = [_dp, C.c_size_t, _dp, _dp, C.c_double, _dp]
         _ref[key] = L
     return _ref[key]
 

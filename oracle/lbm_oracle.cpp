@@ -17,6 +17,8 @@
 //   * moments / mass / logical pops .... esoteric_pull.hpp:180-267
 //   * moving-wall correction ........... src/lbm/d2q9.cpp:217-223 (+6 w rho_w c.u_w, rho_w = max(rho,1e-8))
 //   * local body acceleration .......... esoteric_pull.hpp:144-159,414-420
+//   * STL ray-parity voxelisation ...... src/lbm/voxelize.cpp:4-5 (every cell against every
+//                                        triangle, double), transform_surface voxelize.cpp:4,6
 //
 // Extensions the reference does not have (spec restated here, pinned by closed forms in tests/):
 //   * seeded noise on the Taylor-Green initial velocity (SplitMix64 counter hash),
@@ -1179,6 +1181,79 @@ template <int Q> OracleBase* make_q(int storage, size_t nx, size_t ny, size_t nz
     }
 }
 
+// ---------------------------------------------------------------- voxelisation (voxelize.cpp:4-6)
+// A cell is solid when the ray from its centre (nudged by 1e-10 dx in y and 2e-10 dx in z) along
+// +x crosses the surface an odd number of times.  Brute force over all cells and triangles, as the
+// reference does, in double with its operation order (-ffp-contract=off).
+struct V3 {
+    double x, y, z;
+};
+inline V3 vsub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 vcross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// Moller-Trumbore along d = (1,0,0) with the thresholds of voxelize.cpp:4.
+inline bool ray_x_hits(V3 o, const double* t) {
+    const V3 a{t[0], t[1], t[2]}, b{t[3], t[4], t[5]}, c{t[6], t[7], t[8]};
+    const V3 d{1, 0, 0};
+    const V3 e1 = vsub(b, a), e2 = vsub(c, a);
+    const V3 h = vcross(d, e2);
+    const double det = vdot(e1, h);
+    if (std::fabs(det) < 1e-12) return false;
+    const double inv = 1 / det;
+    const V3 s = vsub(o, a);
+    const double u = inv * vdot(s, h);
+    if (u < 0 || u > 1) return false;
+    const V3 q = vcross(s, e1);
+    const double v = inv * vdot(d, q);
+    if (v < 0 || u + v > 1) return false;
+    return inv * vdot(e2, q) > 1e-10;
+}
+
+inline void voxelize(const double* tris, size_t count, size_t nx, size_t ny, size_t nz, const double* lo,
+                     const double* hi, uint8_t* mask) {
+    if (count == 0 || nx == 0 || ny == 0 || nz == 0 || !(hi[0] > lo[0]) || !(hi[1] > lo[1]) || !(hi[2] > lo[2]))
+        throw std::invalid_argument("invalid voxelization inputs");
+    const double dx = (hi[0] - lo[0]) / static_cast<double>(nx);
+    const double dy = (hi[1] - lo[1]) / static_cast<double>(ny);
+    const double dz = (hi[2] - lo[2]) / static_cast<double>(nz);
+#pragma omp parallel for schedule(dynamic)
+    for (long long kk = 0; kk < static_cast<long long>(nz); ++kk) {
+        const size_t k = static_cast<size_t>(kk);
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                const V3 o{lo[0] + (static_cast<double>(i) + .5) * dx, lo[1] + (static_cast<double>(j) + .5) * dy + 1e-10 * dx,
+                           lo[2] + (static_cast<double>(k) + .5) * dz + 2e-10 * dx};
+                unsigned parity = 0;
+                for (size_t t = 0; t < count; ++t) parity ^= ray_x_hits(o, tris + 9 * t) ? 1U : 0U;
+                mask[(k * ny + j) * nx + i] = static_cast<uint8_t>(parity);
+            }
+    }
+}
+
+// Rodrigues rotation about a unit axis through the origin, then a translation (voxelize.cpp:4,6).
+inline void transform(const double* in, size_t count, const double* tr, const double* axis, double angle,
+                      double* out) {
+    const V3 ax0{axis[0], axis[1], axis[2]};
+    const double n = std::sqrt(vdot(ax0, ax0));
+    for (size_t v = 0; v < 3 * count; ++v) {
+        const V3 p{in[3 * v], in[3 * v + 1], in[3 * v + 2]};
+        V3 r = p;
+        if (!(n < 1e-30)) {
+            const double m = 1 / n;
+            const V3 k{ax0.x * m, ax0.y * m, ax0.z * m};
+            const double c = std::cos(angle), sn = std::sin(angle);
+            const V3 kp = vcross(k, p);
+            const double along = vdot(k, p) * (1 - c);
+            r = {(p.x * c + kp.x * sn) + k.x * along, (p.y * c + kp.y * sn) + k.y * along,
+                 (p.z * c + kp.z * sn) + k.z * along};
+        }
+        out[3 * v] = r.x + tr[0];
+        out[3 * v + 1] = r.y + tr[1];
+        out[3 * v + 2] = r.z + tr[2];
+    }
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------- C ABI for the tests (ctypes)
@@ -1267,4 +1342,12 @@ ORC_API uint16_t orc_f32_to_f16(float f) { return orc::f32_to_f16_rne(f); }
 ORC_API float orc_f16_to_f32(uint16_t h) { return orc::f16_to_f32(h); }
 ORC_API double orc_uniform01(uint64_t seed, uint64_t cell, uint64_t comp) {
     return static_cast<double>(orc::uniform01<float>(seed, cell, comp));
+}
+ORC_API int orc_voxelize(const double* tris, size_t count, size_t nx, size_t ny, size_t nz, const double* lo,
+                         const double* hi, uint8_t* mask) {
+    return guarded([&] { orc::voxelize(tris, count, nx, ny, nz, lo, hi, mask); });
+}
+ORC_API int orc_transform_surface(const double* tris, size_t count, const double* tr, const double* axis,
+                                  double angle, double* out) {
+    return guarded([&] { orc::transform(tris, count, tr, axis, angle, out); });
 }

@@ -5,11 +5,13 @@
 //   cfd::lbm::OneStepPullSolver<D>    include/cfd/solvers/lbm/one_step_pull.hpp
 //   cfd::lbm::PrecisionPullSolver<D,double> include/cfd/solvers/lbm/precision_pull.hpp
 //   cfd::lbm::D2Q9Solver              src/lbm/d2q9.cpp
+//   cfd::lbm::voxelize_surface, transform_surface   src/lbm/voxelize.cpp:5-6
 // through ctypes.  Nothing here re-implements the reference; it only forwards.
 #include "cfd/solvers/lbm/d2q9.hpp"
 #include "cfd/solvers/lbm/esoteric_pull.hpp"
 #include "cfd/solvers/lbm/one_step_pull.hpp"
 #include "cfd/solvers/lbm/precision_pull.hpp"
+#include "cfd/solvers/lbm/voxelize.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -244,3 +246,38 @@ REF_API int ref_d2q9_fields(void* h, float* r, float* ux, float* uy) {
     });
 }
 REF_API double ref_d2q9_mass(void* h) { return static_cast<D2Q9Solver*>(h)->mass(); }
+
+namespace {
+cfd::io::TriangleSurface to_surface(const double* tris, std::size_t count) {
+    cfd::io::TriangleSurface s;
+    s.triangles.resize(count);
+    for (std::size_t t = 0; t < count; ++t) {
+        const double* v = tris + 9 * t;
+        s.triangles[t].a = {v[0], v[1], v[2]};
+        s.triangles[t].b = {v[3], v[4], v[5]};
+        s.triangles[t].c = {v[6], v[7], v[8]};
+    }
+    return s;
+}
+}  // namespace
+
+REF_API int ref_voxelize(const double* tris, std::size_t count, std::size_t nx, std::size_t ny, std::size_t nz,
+                         const double* lo, const double* hi, std::uint8_t* mask) {
+    return guarded([&] {
+        const VoxelGrid3D g{nx, ny, nz, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+        const auto m = voxelize_surface(to_surface(tris, count), g);
+        std::memcpy(mask, m.data(), m.size());
+    });
+}
+REF_API int ref_transform_surface(const double* tris, std::size_t count, const double* tr, const double* axis,
+                                  double angle, double* out) {
+    return guarded([&] {
+        const auto s = transform_surface(to_surface(tris, count), {tr[0], tr[1], tr[2]}, {axis[0], axis[1], axis[2]},
+                                         angle);
+        for (std::size_t t = 0; t < count; ++t) {
+            const auto& x = s.triangles[t];
+            const double v[9] = {x.a.x, x.a.y, x.a.z, x.b.x, x.b.y, x.b.z, x.c.x, x.c.y, x.c.z};
+            std::memcpy(out + 9 * t, v, sizeof(v));
+        }
+    });
+}
