@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize[,config5]]
-                    [--no-cpu]
+    python bench.py [--steps K] [--warmup W]
+                    [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident[,config5]] [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
@@ -10,7 +10,8 @@ u = (0.1, 0, 0), tau 0.55, uint8 flags) with FP32 population storage; further le
 geometry with FP16 and FP32 shifted-population storage, FP64 (the reference for the velocity
 error of the others) and D3Q27, and the adjoint leg runs the
 checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps), and the
-voxelize leg the GPU STL voxelisation of a 16128-triangle sphere on 512^3 cells.  The opt-in
+voxelize leg the GPU STL voxelisation of a 16128-triangle sphere on 512^3 cells, and the resident
+leg the device-resident coupling modules (thermal, fill, particles, Q-criterion) at 256^3.  The opt-in
 config5 leg runs forward + adjoint of a 768^3 FP16 duct past a sphere cluster (drag objective,
 binomial checkpointing) against the FP32-storage run.  Prints progress on stderr and ONE JSON
 line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
@@ -247,6 +248,58 @@ def voxelize_leg(hip, n=512, reps=5):
     return res
 
 
+def resident_leg(hip, n=256, reps=50, particles=1 << 20):
+    """Device-resident coupling modules on an n^3 D3Q19 Taylor-Green flow (FP32 views): D3Q7 thermal
+    step, Boussinesq coupling, fill advection, Q-criterion and a one-way particle push.  Each figure
+    is the wall time of `reps` back-to-back launches on the flow's stream between two
+    synchronisations (launch gaps included), so it bounds the kernel time from above.  Algorithmic
+    bytes per cell: thermal 7 reads + 7 writes + 3 velocity reads = 68 B; Boussinesq 60 B (the
+    temperature plane's refresh, 7 reads + 1 write, then T read + acceleration read/write); advection fill read + write + 3 velocities = 20 B;
+    Q-criterion 3 velocities + 1 write = 16 B; per particle 8 state reads + 6 writes = 56 B plus
+    the 8-cell velocity gather."""
+    flow = hip.EsotericPullHipSolver("d3q19", n, n, n, 0.6)
+    flow.initialize_taylor_green(0.02)
+    flow.step(2)
+    macro = flow.refresh_device_macroscopic()
+    force = flow.device_local_acceleration(True)
+    thermal = hip.ResidentThermalD3Q7(flow, 0.02)
+    thermal.initialize_uniform(300.0)
+    surface = hip.ResidentFreeSurface(flow)
+    surface.initialize_half_space_x(n / 2 + 0.5)
+    diag = hip.ResidentFlowDiagnostics(flow)
+    rng = np.random.default_rng(0)
+    pts = np.column_stack([rng.uniform(0, n, (particles, 3)), np.zeros((particles, 3)),
+                           np.ones(particles), np.full(particles, 0.5)])
+    parts = hip.ResidentParticles(flow, particles)
+    parts.upload(pts)
+    cells = n ** 3
+
+    def timed(fn, count):
+        fn()
+        flow.wait()
+        t0 = time.perf_counter()
+        for _ in range(count):
+            fn()
+        flow.wait()
+        return (time.perf_counter() - t0) * 1e3 / count
+
+    out = {"workload": f"D3Q19 {n}^3 Taylor-Green flow, FP32 views; {particles} particles", "cells": cells}
+    for name, fn, nbytes in (
+            ("thermal_step", lambda: thermal.step(macro, 1), 68 * cells),
+            ("boussinesq", lambda: thermal.add_boussinesq_acceleration(force, 300.0, 1e-4, (0, -1, 0)), 60 * cells),
+            ("fill_advect", lambda: surface.advect(macro, 0.1), 20 * cells),
+            ("q_criterion", lambda: diag.q_criterion(macro), 16 * cells),
+            ("particles_one_way", lambda: parts.advance(macro, force, 0.05, False), 56 * particles)):
+        ms = timed(fn, reps)
+        out[name] = {"ms": ms, "algorithmic_bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_of_peak": nbytes / (ms * 1e-3) / 1e9 / PEAK_GBS}
+        log(f"resident {name}: {ms * 1e3:.1f} us per launch, {out[name]['gbs']:.0f} GB/s algorithmic")
+    for m in (parts, diag, surface, thermal):
+        m.close()
+    flow.close()
+    return out
+
+
 def copy_probe():
     import ctypes as C
 
@@ -320,7 +373,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize")
+    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident")
     ap.add_argument("--config5-steps", type=int, default=1000)
     ap.add_argument("--config5-n", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
@@ -347,6 +400,9 @@ def main():
             continue
         if name == "voxelize":
             legs[name] = voxelize_leg(hip)
+            continue
+        if name == "resident":
+            legs[name] = resident_leg(hip)
             continue
         lattice, storage = table[name]
         legs[name], vel = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)

@@ -7,11 +7,13 @@ reference's ``python/cfd_solvers`` bindings.  Build the library with
 """
 from .bindings import (EsotericPullHipSolver, MacroscopicFields, SolverLibrary, load, checkpoint_plan, FLUID,
                        VELOCITY_INLET, PRESSURE_OUTLET, WALL, LATTICES, STORAGE, OBJECTIVES, voxelize_surface,
-                       voxelize_timing, transform_surface)
+                       voxelize_timing, transform_surface, ResidentThermalD3Q7, ResidentFreeSurface,
+                       ResidentParticles, ResidentFlowDiagnostics)
 
 __all__ = ["EsotericPullHipSolver", "MacroscopicFields", "SolverLibrary", "load", "checkpoint_plan", "FLUID",
            "VELOCITY_INLET", "PRESSURE_OUTLET", "WALL", "LATTICES", "STORAGE", "OBJECTIVES", "voxelize_surface",
-           "voxelize_timing", "transform_surface"]
+           "voxelize_timing", "transform_surface", "ResidentThermalD3Q7", "ResidentFreeSurface",
+           "ResidentParticles", "ResidentFlowDiagnostics"]
 
 # torch.autograd.Function over the adjoint: `from cfd_solvers_amd.autograd import LbmObjective`
 # (imported on demand so that the ctypes layer does not need torch).

@@ -42,7 +42,6 @@ _SIGNATURES = {
     "cfd_lbm_upload_populations": [C.c_void_p, _f32p],
     "cfd_lbm_download_populations": [C.c_void_p, _f32p],
     "cfd_lbm_set_flags": [C.c_void_p, C.c_void_p],
-    "cfd_lbm_get_flags": [C.c_void_p, C.c_void_p],
     "cfd_lbm_set_wall_velocity": [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float],
     "cfd_lbm_set_porosity": [C.c_void_p, C.c_void_p],
     "cfd_lbm_set_inlet_outlet": [C.c_void_p, C.c_float, C.c_float],
@@ -79,6 +78,44 @@ _SIGNATURES = {
     "cfd_lbm_voxelize_timing": [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
     "cfd_lbm_transform_surface": [_f64p, C.c_size_t, _f64p, _f64p, C.c_double, _f64p],
     "cfd_lbm_set_solid_surface": [C.c_void_p, _f64p, C.c_size_t, _f64p, _f64p, C.c_int],
+    "cfd_lbm_device_local_acceleration": [C.c_void_p, C.c_int] + [C.POINTER(C.c_void_p)] * 3,
+    "cfd_lbm_device_download": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t],
+    "cfd_lbm_device_upload": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t],
+    "cfd_lbm_thermal_create": [C.c_void_p, C.c_float, C.POINTER(C.c_void_p)],
+    "cfd_lbm_thermal_destroy": [C.c_void_p],
+    "cfd_lbm_thermal_initialize_uniform": [C.c_void_p, C.c_float],
+    "cfd_lbm_thermal_initialize_fields": [C.c_void_p, _f32p],
+    "cfd_lbm_thermal_step": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64],
+    "cfd_lbm_thermal_refresh_device_scalar": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "cfd_lbm_thermal_download_scalar": [C.c_void_p, _f32p],
+    "cfd_lbm_thermal_add_boussinesq": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
+                                       C.c_double, C.c_double, C.c_double],
+    "cfd_lbm_thermal_total_scalar": [C.c_void_p, C.POINTER(C.c_double)],
+    "cfd_lbm_thermal_resident_bytes": [C.c_void_p, C.POINTER(C.c_size_t)],
+    "cfd_lbm_thermal_transfer_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.c_int],
+    "cfd_lbm_surface_create": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "cfd_lbm_surface_destroy": [C.c_void_p],
+    "cfd_lbm_surface_initialize_uniform": [C.c_void_p, C.c_float],
+    "cfd_lbm_surface_initialize_half_space_x": [C.c_void_p, C.c_float],
+    "cfd_lbm_surface_upload_fill": [C.c_void_p, _f32p],
+    "cfd_lbm_surface_download_fill": [C.c_void_p, _f32p],
+    "cfd_lbm_surface_advect": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float],
+    "cfd_lbm_surface_fill_device": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "cfd_lbm_surface_refresh_geometry": [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4,
+    "cfd_lbm_surface_add_surface_tension": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float],
+    "cfd_lbm_surface_volume": [C.c_void_p, C.POINTER(C.c_double)],
+    "cfd_lbm_surface_transfer_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.c_int],
+    "cfd_lbm_particles_create": [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)],
+    "cfd_lbm_particles_destroy": [C.c_void_p],
+    "cfd_lbm_particles_upload": [C.c_void_p, _f64p, C.c_size_t],
+    "cfd_lbm_particles_download": [C.c_void_p, _f64p, C.c_size_t],
+    "cfd_lbm_particles_advance": [C.c_void_p] + [C.c_void_p] * 6 + [C.c_float, C.c_int, C.c_double, C.c_double,
+                                                                    C.c_double],
+    "cfd_lbm_particles_transfer_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.c_int],
+    "cfd_lbm_flow_diagnostics_create": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "cfd_lbm_flow_diagnostics_destroy": [C.c_void_p],
+    "cfd_lbm_q_criterion": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],
+    "cfd_lbm_get_flags": [C.c_void_p, C.c_void_p],
 }
 
 OBJECTIVES = {"pressure_drop": 0, "drag": 1}
@@ -138,6 +175,7 @@ class EsotericPullHipSolver:
         self.q = LATTICES[lattice] if isinstance(lattice, str) else int(lattice)
         self.storage = storage
         nz = 1 if self.q == 9 else nz
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.shape = (nz, ny, nx)
         h = C.c_void_p()
         self.lib.call("cfd_lbm_create", self.q, nx, ny, nz, tau, *map(float, acceleration),
@@ -262,6 +300,24 @@ class EsotericPullHipSolver:
         ptrs = [C.c_void_p() for _ in range(4)]
         self.lib.call("cfd_lbm_refresh_device_macroscopic", self._h, *(C.byref(p) for p in ptrs))
         return tuple(p.value for p in ptrs)
+
+    def device_local_acceleration(self, clear=False):
+        """(ax, ay, az) device pointers of the per-cell acceleration planes
+        (esoteric_pull_sycl.hpp:349-358); allocated zeroed on first use."""
+        ptrs = [C.c_void_p() for _ in range(3)]
+        self.lib.call("cfd_lbm_device_local_acceleration", self._h, int(clear), *(C.byref(p) for p in ptrs))
+        return tuple(p.value for p in ptrs)
+
+    def device_download(self, pointer, dtype=np.float32, count=None):
+        """Copy of `count` values (default: one per cell) at a device pointer of this solver."""
+        count = self.cells if count is None else int(count)
+        out = np.empty(count, dtype)
+        self.lib.call("cfd_lbm_device_download", self._h, pointer, out.ctypes.data, out.nbytes)
+        return out
+
+    def device_upload(self, pointer, values):
+        v = np.ascontiguousarray(values)
+        self.lib.call("cfd_lbm_device_upload", self._h, pointer, v.ctypes.data, v.nbytes)
 
     def total_mass(self) -> float:
         m = C.c_double()
@@ -420,3 +476,186 @@ def transform_surface(triangles, translation, axis, angle, library=None):
     out = np.empty_like(t)
     load(library).call("cfd_lbm_transform_surface", t, len(t) // 9, _vec3(translation), _vec3(axis), float(angle), out)
     return out.reshape(-1, 3, 3)
+
+
+def _stats(lib, name, handle, reset):
+    v = (C.c_uint64 * 4)()
+    lib.call(name, handle, v, int(reset))
+    return {"host_to_device_bytes": v[0], "device_to_host_bytes": v[1], "device_to_device_bytes": v[2],
+            "synchronization_points": v[3], "host_transfer_bytes": v[0] + v[1]}
+
+
+class _Module:
+    """A device-resident module on a flow solver's device and stream
+    (include/cfd/solvers/lbm/resident_multiphysics_sycl.hpp).  Field arguments are device pointers
+    (ints), e.g. from ``flow.refresh_device_macroscopic()`` and ``flow.device_local_acceleration()``."""
+
+    _prefix = ""
+
+    def __init__(self, flow, *args):
+        self.flow = flow
+        self.lib = flow.lib
+        h = C.c_void_p()
+        self.lib.call(f"cfd_lbm_{self._prefix}_create", flow._h, *args, C.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.call(f"cfd_lbm_{self._prefix}_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class ResidentThermalD3Q7(_Module):
+    """ResidentThermalD3Q7Sycl (resident_multiphysics_sycl.hpp:54-198)."""
+
+    _prefix = "thermal"
+
+    def __init__(self, flow, diffusivity=0.01):
+        super().__init__(flow, float(diffusivity))
+
+    def initialize_uniform(self, temperature):
+        self.lib.call("cfd_lbm_thermal_initialize_uniform", self._h, float(temperature))
+
+    def initialize_fields(self, temperature):
+        t = np.ascontiguousarray(temperature, np.float32).reshape(-1)
+        if t.size != self.flow.cells:
+            raise ValueError("thermal field size mismatch")
+        self.lib.call("cfd_lbm_thermal_initialize_fields", self._h, t)
+
+    def step(self, macroscopic, count=1):
+        """macroscopic: (rho, ux, uy, uz) device pointers from flow.refresh_device_macroscopic()."""
+        _, ux, uy, uz = macroscopic
+        self.lib.call("cfd_lbm_thermal_step", self._h, ux, uy, uz, int(count))
+
+    def refresh_device_scalar(self):
+        p = C.c_void_p()
+        self.lib.call("cfd_lbm_thermal_refresh_device_scalar", self._h, C.byref(p))
+        return p.value
+
+    def scalar(self):
+        out = np.empty(self.flow.cells, np.float32)
+        self.lib.call("cfd_lbm_thermal_download_scalar", self._h, out)
+        return out.reshape(self.flow.shape)
+
+    def add_boussinesq_acceleration(self, force, reference_temperature, beta, gravity):
+        ax, ay, az = force
+        self.lib.call("cfd_lbm_thermal_add_boussinesq", self._h, ax, ay, az, float(reference_temperature), float(beta),
+                      *map(float, gravity))
+
+    def total_scalar(self):
+        v = C.c_double()
+        self.lib.call("cfd_lbm_thermal_total_scalar", self._h, C.byref(v))
+        return v.value
+
+    def resident_bytes(self):
+        v = C.c_size_t()
+        self.lib.call("cfd_lbm_thermal_resident_bytes", self._h, C.byref(v))
+        return v.value
+
+    def transfer_stats(self, reset=False):
+        return _stats(self.lib, "cfd_lbm_thermal_transfer_stats", self._h, reset)
+
+
+class ResidentFreeSurface(_Module):
+    """ResidentFreeSurfaceSycl3D (resident_multiphysics_sycl.hpp:200-324)."""
+
+    _prefix = "surface"
+
+    def initialize_uniform(self, fill):
+        self.lib.call("cfd_lbm_surface_initialize_uniform", self._h, float(fill))
+
+    def initialize_half_space_x(self, interface_x):
+        self.lib.call("cfd_lbm_surface_initialize_half_space_x", self._h, float(interface_x))
+
+    def upload_fill(self, fill):
+        f = np.ascontiguousarray(fill, np.float32).reshape(-1)
+        if f.size != self.flow.cells:
+            raise ValueError("fill field size mismatch")
+        self.lib.call("cfd_lbm_surface_upload_fill", self._h, f)
+
+    def fill(self):
+        out = np.empty(self.flow.cells, np.float32)
+        self.lib.call("cfd_lbm_surface_download_fill", self._h, out)
+        return out.reshape(self.flow.shape)
+
+    def advect(self, macroscopic, dt):
+        _, ux, uy, uz = macroscopic
+        self.lib.call("cfd_lbm_surface_advect", self._h, ux, uy, uz, float(dt))
+
+    def fill_device(self):
+        p = C.c_void_p()
+        self.lib.call("cfd_lbm_surface_fill_device", self._h, C.byref(p))
+        return p.value
+
+    def refresh_geometry(self):
+        """(normal_x, normal_y, normal_z, curvature) device pointers."""
+        ptrs = [C.c_void_p() for _ in range(4)]
+        self.lib.call("cfd_lbm_surface_refresh_geometry", self._h, *(C.byref(p) for p in ptrs))
+        return tuple(p.value for p in ptrs)
+
+    def add_surface_tension_acceleration(self, force, surface_tension, density=1.0):
+        ax, ay, az = force
+        self.lib.call("cfd_lbm_surface_add_surface_tension", self._h, ax, ay, az, float(surface_tension),
+                      float(density))
+
+    def volume(self):
+        v = C.c_double()
+        self.lib.call("cfd_lbm_surface_volume", self._h, C.byref(v))
+        return v.value
+
+    def transfer_stats(self, reset=False):
+        return _stats(self.lib, "cfd_lbm_surface_transfer_stats", self._h, reset)
+
+
+class ResidentParticles(_Module):
+    """ResidentParticlesSycl3D (resident_multiphysics_sycl.hpp:326-380); particles as (count, 8)
+    float64 rows: position xyz, velocity xyz, mass, response time (ImmersedBoundaryParticle)."""
+
+    _prefix = "particles"
+
+    def __init__(self, flow, count):
+        self.count = int(count)
+        super().__init__(flow, self.count)
+
+    def upload(self, particles):
+        p = np.ascontiguousarray(particles, np.float64).reshape(-1, 8)
+        self.lib.call("cfd_lbm_particles_upload", self._h, p.reshape(-1), len(p))
+
+    def download(self):
+        out = np.empty((self.count, 8), np.float64)
+        self.lib.call("cfd_lbm_particles_download", self._h, out.reshape(-1), self.count)
+        return out
+
+    def advance(self, macroscopic, force, dt, two_way, gravity=(0.0, 0.0, 0.0)):
+        _, ux, uy, uz = macroscopic
+        ax, ay, az = force
+        self.lib.call("cfd_lbm_particles_advance", self._h, ux, uy, uz, ax, ay, az, float(dt), int(bool(two_way)),
+                      *map(float, gravity))
+
+    def transfer_stats(self, reset=False):
+        return _stats(self.lib, "cfd_lbm_particles_transfer_stats", self._h, reset)
+
+
+class ResidentFlowDiagnostics(_Module):
+    """ResidentFlowDiagnosticsSycl3D (resident_multiphysics_sycl.hpp:382-408)."""
+
+    _prefix = "flow_diagnostics"
+
+    def q_criterion(self, macroscopic):
+        """Device pointer to the Q-criterion field (cells floats)."""
+        _, ux, uy, uz = macroscopic
+        p = C.c_void_p()
+        self.lib.call("cfd_lbm_q_criterion", self._h, ux, uy, uz, C.byref(p))
+        return p.value

@@ -22,7 +22,7 @@ LIB = PKG / "libcfd_lbm_hip.so"
 PROBE = PKG / "libhbm_probe.so"
 APPS = {"cfd-bench": PKG / "apps" / "cfd_bench.cpp", "cfd-solve": PKG / "apps" / "cfd_solve.cpp"}
 
-HIP_SOURCES = ["lbm_kernels.hip", "lbm_adjoint.hip", "voxelize.hip", "lbm_capi.hip"]
+HIP_SOURCES = ["lbm_kernels.hip", "lbm_adjoint.hip", "voxelize.hip", "multiphysics.hip", "lbm_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -9,6 +9,7 @@
 #include "checkpoint_schedule.hpp"
 #include "lbm_adjoint.hpp"
 #include "lbm_kernels.hpp"
+#include "solver_view.hpp"
 #include "voxelize.hpp"
 
 #include <hip/hip_runtime.h>
@@ -63,6 +64,14 @@ template <class T> struct DeviceBuffer {
     explicit operator bool() const { return ptr != nullptr; }
 };
 
+}  // namespace
+
+namespace {
+// Zero fill by a kernel rather than hipMemsetAsync, so that callers may capture it in a graph.
+__global__ void __launch_bounds__(256) zero_words_kernel(uint32_t* p, uint64_t words) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < words) p[i] = 0U;
+}
 }  // namespace
 
 struct cfd_lbm_solver {
@@ -788,6 +797,24 @@ struct cfd_lbm_solver {
         upload_real(lacc, parts, 3);
     }
 
+    // device_local_acceleration (esoteric_pull_sycl.hpp:349-358): the per-cell acceleration planes
+    // (ax, ay, az; cells values each) on the device, allocated zeroed on first use and zeroed again
+    // on request.  From then on every step uses the forced kernel.
+    unsigned char* device_local_acceleration(bool clear) {
+        use_device();
+        const bool fresh = !lacc;
+        if (fresh) lacc.allocate(3 * cells * real);
+        if (fresh || clear) zero_words(lacc.ptr, 3 * cells * real / 4);
+        return lacc.ptr;
+    }
+
+    void zero_words(unsigned char* p, uint64_t words) {
+        const uint64_t blocks = (words + 255) / 256;
+        if (blocks > 0x7FFFFFFFull) throw std::invalid_argument("buffer too large to clear");
+        if (blocks) zero_words_kernel<<<static_cast<uint32_t>(blocks), 256, 0, stream>>>(reinterpret_cast<uint32_t*>(p), words);
+        hip_check(hipGetLastError(), "zero_words");
+    }
+
     void step(uint64_t count) {
         use_device();
         const bool b = bounded();
@@ -908,6 +935,21 @@ struct cfd_lbm_solver {
                reduce_out.bytes;
     }
 };
+
+namespace cfd_hip {
+FlowInfo flow_info(cfd_lbm_solver* s) {
+    if (!s) throw std::invalid_argument("null solver handle");
+    FlowInfo f;
+    f.device = s->device;
+    f.stream = s->stream;
+    f.nx = s->nx;
+    f.ny = s->ny;
+    f.nz = s->nz;
+    f.cells = s->cells;
+    f.real = s->real;
+    return f;
+}
+}  // namespace cfd_hip
 
 namespace {
 
@@ -1043,6 +1085,17 @@ int cfd_lbm_download_macroscopic(cfd_lbm_solver* s, float* rho, float* ux, float
 int cfd_lbm_download_macroscopic_f64(cfd_lbm_solver* s, double* rho, double* ux, double* uy, double* uz, char* error,
                                      size_t cap) {
     return guarded(error, cap, [&] { need(s).download_macroscopic<double>(rho, ux, uy, uz); });
+}
+
+int cfd_lbm_device_local_acceleration(cfd_lbm_solver* s, int clear, void** ax, void** ay, void** az, char* error,
+                                      size_t cap) {
+    return guarded(error, cap, [&] {
+        unsigned char* base = need(s).device_local_acceleration(clear != 0);
+        const size_t plane = s->cells * s->real;
+        if (ax) *ax = base;
+        if (ay) *ay = base + plane;
+        if (az) *az = base + 2 * plane;
+    });
 }
 
 int cfd_lbm_refresh_device_macroscopic(cfd_lbm_solver* s, void** rho, void** ux, void** uy, void** uz, char* error,

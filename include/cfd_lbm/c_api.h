@@ -224,6 +224,121 @@ CFD_LBM_API int cfd_lbm_set_solid_surface(cfd_lbm_solver* s, const double* trian
                                           const double* minimum, const double* maximum, int keep_flags, char* error,
                                           size_t error_capacity);
 
+/* ---------------------------------------------------------------------------------------------
+ * Device-resident coupling (include/cfd/solvers/lbm/resident_multiphysics_sycl.hpp:54-408).
+ * Each module lives on the flow solver's device and stream, on the flow's nx x ny x nz lattice,
+ * periodic in x, y and z, and exchanges fields with the flow only through device pointers: the
+ * macroscopic views of cfd_lbm_refresh_device_macroscopic and the acceleration planes of
+ * cfd_lbm_device_local_acceleration.  The views must be float (FP32, FP16S or FP32S storage).
+ * Destroy modules before their flow solver.  Field pointers passed in are device pointers of
+ * cells floats each. */
+typedef struct cfd_lbm_thermal cfd_lbm_thermal;
+typedef struct cfd_lbm_surface cfd_lbm_surface;
+typedef struct cfd_lbm_particles cfd_lbm_particles;
+typedef struct cfd_lbm_flow_diagnostics cfd_lbm_flow_diagnostics;
+
+/* device_local_acceleration(clear) -- esoteric_pull_sycl.hpp:349-358: the solver's per-cell
+ * acceleration planes on the device (allocated zeroed on first use; clear != 0 zeroes them); every
+ * later step applies them (Guo forcing).  Views are `real`-typed (float except FP64 storage). */
+CFD_LBM_API int cfd_lbm_device_local_acceleration(cfd_lbm_solver* s, int clear, void** ax, void** ay, void** az,
+                                                  char* error, size_t error_capacity);
+/* Copies between a device view and host memory on the solver's stream (synchronous).  No reference
+ * counterpart: SYCL's shared views are readable through queue.memcpy. */
+CFD_LBM_API int cfd_lbm_device_download(cfd_lbm_solver* s, const void* device, void* host, size_t bytes, char* error,
+                                        size_t error_capacity);
+CFD_LBM_API int cfd_lbm_device_upload(cfd_lbm_solver* s, void* device, const void* host, size_t bytes, char* error,
+                                      size_t error_capacity);
+
+/* ResidentThermalD3Q7Sycl (resident_multiphysics_sycl.hpp:54-198): D3Q7 passive scalar, tau =
+ * 0.5 + 4 diffusivity, equilibrium w T (1 + 4 c.u), two lattices. */
+CFD_LBM_API int cfd_lbm_thermal_create(cfd_lbm_solver* flow, float diffusivity, cfd_lbm_thermal** out, char* error,
+                                       size_t error_capacity);
+CFD_LBM_API int cfd_lbm_thermal_destroy(cfd_lbm_thermal* t, char* error, size_t error_capacity);
+/* initialize_uniform :82-91 */
+CFD_LBM_API int cfd_lbm_thermal_initialize_uniform(cfd_lbm_thermal* t, float temperature, char* error,
+                                                   size_t error_capacity);
+/* Equilibrium at rest of a host temperature field (cells values); no reference counterpart. */
+CFD_LBM_API int cfd_lbm_thermal_initialize_fields(cfd_lbm_thermal* t, const float* temperature, char* error,
+                                                  size_t error_capacity);
+/* step(flow view, count) :93-125 */
+CFD_LBM_API int cfd_lbm_thermal_step(cfd_lbm_thermal* t, const float* ux, const float* uy, const float* uz,
+                                     uint64_t count, char* error, size_t error_capacity);
+/* refresh_device_scalar :127-136 (device pointer to cells floats, valid until the next refresh) */
+CFD_LBM_API int cfd_lbm_thermal_refresh_device_scalar(cfd_lbm_thermal* t, void** scalar, char* error,
+                                                      size_t error_capacity);
+/* The temperature field on the host (cells floats). */
+CFD_LBM_API int cfd_lbm_thermal_download_scalar(cfd_lbm_thermal* t, float* temperature, char* error,
+                                                size_t error_capacity);
+/* add_boussinesq_acceleration :140-155: a += -beta (T - t_ref) g */
+CFD_LBM_API int cfd_lbm_thermal_add_boussinesq(cfd_lbm_thermal* t, float* ax, float* ay, float* az, float t_ref,
+                                               float beta, double gx, double gy, double gz, char* error,
+                                               size_t error_capacity);
+/* total_scalar :157-167, accumulated in FP64 (the reference sums in float) */
+CFD_LBM_API int cfd_lbm_thermal_total_scalar(cfd_lbm_thermal* t, double* total, char* error, size_t error_capacity);
+/* resident_bytes :169-172, transfer_stats :173-174 ({h2d, d2h, d2d bytes, synchronisations}) */
+CFD_LBM_API int cfd_lbm_thermal_resident_bytes(cfd_lbm_thermal* t, size_t* bytes, char* error, size_t error_capacity);
+CFD_LBM_API int cfd_lbm_thermal_transfer_stats(cfd_lbm_thermal* t, uint64_t* stats4, int reset, char* error,
+                                               size_t error_capacity);
+
+/* ResidentFreeSurfaceSycl3D (resident_multiphysics_sycl.hpp:200-324): fill fraction in [0,1]. */
+CFD_LBM_API int cfd_lbm_surface_create(cfd_lbm_solver* flow, cfd_lbm_surface** out, char* error,
+                                       size_t error_capacity);
+CFD_LBM_API int cfd_lbm_surface_destroy(cfd_lbm_surface* s, char* error, size_t error_capacity);
+/* initialize_uniform :220-224, initialize_half_space_x :225-236 */
+CFD_LBM_API int cfd_lbm_surface_initialize_uniform(cfd_lbm_surface* s, float fill, char* error,
+                                                   size_t error_capacity);
+CFD_LBM_API int cfd_lbm_surface_initialize_half_space_x(cfd_lbm_surface* s, float interface_x, char* error,
+                                                        size_t error_capacity);
+/* Host copies of the fill field (cells floats); no reference counterpart. */
+CFD_LBM_API int cfd_lbm_surface_upload_fill(cfd_lbm_surface* s, const float* fill, char* error,
+                                            size_t error_capacity);
+CFD_LBM_API int cfd_lbm_surface_download_fill(cfd_lbm_surface* s, float* fill, char* error, size_t error_capacity);
+/* advect(flow view, dt) :238-263: first-order upwind fluxes, clamped to [0,1] */
+CFD_LBM_API int cfd_lbm_surface_advect(cfd_lbm_surface* s, const float* ux, const float* uy, const float* uz,
+                                       float dt, char* error, size_t error_capacity);
+/* fill_device :265-266 */
+CFD_LBM_API int cfd_lbm_surface_fill_device(cfd_lbm_surface* s, void** fill, char* error, size_t error_capacity);
+/* refresh_geometry :268-297: unit normals grad(phi)/|grad(phi)| and curvature -div(n) (device
+ * pointers to cells floats each; any may be NULL) */
+CFD_LBM_API int cfd_lbm_surface_refresh_geometry(cfd_lbm_surface* s, void** normal_x, void** normal_y,
+                                                 void** normal_z, void** curvature, char* error,
+                                                 size_t error_capacity);
+/* add_surface_tension_acceleration :299-317: a += (sigma kappa / density) grad(phi) */
+CFD_LBM_API int cfd_lbm_surface_add_surface_tension(cfd_lbm_surface* s, float* ax, float* ay, float* az,
+                                                    float surface_tension, float density, char* error,
+                                                    size_t error_capacity);
+/* volume :319-323, accumulated in FP64 */
+CFD_LBM_API int cfd_lbm_surface_volume(cfd_lbm_surface* s, double* volume, char* error, size_t error_capacity);
+CFD_LBM_API int cfd_lbm_surface_transfer_stats(cfd_lbm_surface* s, uint64_t* stats4, int reset, char* error,
+                                               size_t error_capacity);
+
+/* ResidentParticlesSycl3D (resident_multiphysics_sycl.hpp:326-380).  Particles cross the ABI as 8
+ * doubles each in the layout of ImmersedBoundaryParticle (include/cfd/solvers/lbm/particles.hpp:
+ * 13-18): position x y z, velocity x y z, mass, response time; they are stored as float. */
+CFD_LBM_API int cfd_lbm_particles_create(cfd_lbm_solver* flow, size_t count, cfd_lbm_particles** out, char* error,
+                                         size_t error_capacity);
+CFD_LBM_API int cfd_lbm_particles_destroy(cfd_lbm_particles* p, char* error, size_t error_capacity);
+/* upload :337-344, download :368-376 */
+CFD_LBM_API int cfd_lbm_particles_upload(cfd_lbm_particles* p, const double* particles, size_t count, char* error,
+                                         size_t error_capacity);
+CFD_LBM_API int cfd_lbm_particles_download(cfd_lbm_particles* p, double* particles, size_t count, char* error,
+                                           size_t error_capacity);
+/* advance(flow view, force view, dt, two_way, gravity) :346-366: trilinear fluid velocity, Stokes
+ * drag, explicit Euler, periodic positions; two_way spreads -drag to the 8 cells (atomically). */
+CFD_LBM_API int cfd_lbm_particles_advance(cfd_lbm_particles* p, const float* ux, const float* uy, const float* uz,
+                                          float* ax, float* ay, float* az, float dt, int two_way, double gx,
+                                          double gy, double gz, char* error, size_t error_capacity);
+CFD_LBM_API int cfd_lbm_particles_transfer_stats(cfd_lbm_particles* p, uint64_t* stats4, int reset, char* error,
+                                                 size_t error_capacity);
+
+/* ResidentFlowDiagnosticsSycl3D (resident_multiphysics_sycl.hpp:382-408): Q-criterion
+ * (|Omega|^2 - |S|^2)/2 by central differences; q receives a device pointer (cells floats). */
+CFD_LBM_API int cfd_lbm_flow_diagnostics_create(cfd_lbm_solver* flow, cfd_lbm_flow_diagnostics** out, char* error,
+                                                size_t error_capacity);
+CFD_LBM_API int cfd_lbm_flow_diagnostics_destroy(cfd_lbm_flow_diagnostics* d, char* error, size_t error_capacity);
+CFD_LBM_API int cfd_lbm_q_criterion(cfd_lbm_flow_diagnostics* d, const float* ux, const float* uy, const float* uz,
+                                    void** q, char* error, size_t error_capacity);
+
 /* Debug: the slots (d'*cells + n') read (load) and written (store) for every (d, n) by the
  * step kernel at the given parity, computed by the kernel's own index code (q*cells each).
  * Wall cells report -1.  Periodic lattices follow esoteric_pull.hpp:349-382. */

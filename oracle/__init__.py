@@ -79,6 +79,18 @@ def lib():
         L.orc_uniform01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_uniform01.restype = C.c_double
         L.orc_voxelize.argtypes = [_dp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp, _u8p]
+        S3 = [C.c_size_t] * 3
+        L.orc_thermal_fill.argtypes = [_fp, C.c_void_p, C.c_float, C.c_size_t]
+        L.orc_thermal_step.argtypes = [_fp, _fp, _fp, _fp, _fp] + S3 + [C.c_float]
+        L.orc_thermal_scalar.argtypes = [_fp, _fp, C.c_size_t]
+        L.orc_boussinesq.argtypes = [_fp, _fp, _fp, _fp, C.c_size_t] + [C.c_float] * 5
+        L.orc_vof_half_space.argtypes = [_fp] + S3 + [C.c_float]
+        L.orc_vof_advect.argtypes = [_fp, _fp, _fp, _fp, _fp] + S3 + [C.c_float]
+        L.orc_vof_geometry.argtypes = [_fp, _fp] + S3
+        L.orc_vof_tension.argtypes = [_fp, _fp, _fp, _fp, _fp] + S3 + [C.c_float, C.c_float]
+        L.orc_particles_advance.argtypes = [_fp, C.c_size_t, _fp, _fp, _fp, _fp, _fp, _fp] + S3 + [
+            C.c_float, C.c_int, C.c_float, C.c_float, C.c_float]
+        L.orc_q_criterion.argtypes = [_fp, _fp, _fp, _fp] + S3
         L.orc_transform_surface.argtypes = [_dp, C.c_size_t, _dp, _dp, C.c_double, _dp]
         _lib = L
     return _lib
@@ -258,6 +270,87 @@ def ref_voxelize(triangles, shape, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), nativ
 def ref_transform_surface(triangles, translation, axis, angle):
     L = ref_lib()
     return _xform(L.ref_transform_surface, L, "ref", triangles, translation, axis, angle)
+
+
+class Resident:
+    """The resident coupling modules (resident_multiphysics_sycl.hpp:54-408) restated on float numpy
+    arrays of shape (nz, ny, nx), periodic.  Velocity and acceleration fields are (3, nz, ny, nx)."""
+
+    def __init__(self, nx, ny, nz):
+        self.L = lib()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (nz, ny, nx)
+        self.cells = nx * ny * nz
+
+    def _f(self, a, n=1):
+        a = np.ascontiguousarray(a, np.float32).reshape(-1)
+        assert a.size == n * self.cells
+        return a
+
+    # D3Q7 thermal
+    def thermal_equilibrium(self, temperature):
+        g = np.empty(7 * self.cells, np.float32)
+        t = self._f(temperature)
+        self.L.orc_thermal_fill(g, t.ctypes.data, 0.0, self.cells)
+        return g
+
+    def thermal_step(self, g, u, omega, steps=1):
+        ux, uy, uz = (self._f(c) for c in u)
+        a, b = np.ascontiguousarray(g, np.float32).copy(), np.empty(7 * self.cells, np.float32)
+        for _ in range(steps):
+            self.L.orc_thermal_step(a, b, ux, uy, uz, self.nx, self.ny, self.nz, float(omega))
+            a, b = b, a
+        return a
+
+    def thermal_scalar(self, g):
+        t = np.empty(self.cells, np.float32)
+        self.L.orc_thermal_scalar(np.ascontiguousarray(g, np.float32), t, self.cells)
+        return t.reshape(self.shape)
+
+    def boussinesq(self, temperature, accel, t_ref, beta, gravity):
+        a = [self._f(c).copy() for c in accel]
+        self.L.orc_boussinesq(self._f(temperature), *a, self.cells, float(t_ref), float(beta), *map(float, gravity))
+        return np.stack([c.reshape(self.shape) for c in a])
+
+    # volume of fluid
+    def half_space(self, x):
+        phi = np.empty(self.cells, np.float32)
+        self.L.orc_vof_half_space(phi, self.nx, self.ny, self.nz, float(x))
+        return phi.reshape(self.shape)
+
+    def advect(self, phi, u, dt):
+        out = np.empty(self.cells, np.float32)
+        ux, uy, uz = (self._f(c) for c in u)
+        self.L.orc_vof_advect(self._f(phi), out, ux, uy, uz, self.nx, self.ny, self.nz, float(dt))
+        return out.reshape(self.shape)
+
+    def geometry(self, phi):
+        geo = np.empty(4 * self.cells, np.float32)
+        self.L.orc_vof_geometry(self._f(phi), geo, self.nx, self.ny, self.nz)
+        return geo.reshape((4,) + self.shape)
+
+    def surface_tension(self, phi, accel, sigma, density=1.0):
+        curvature = self.geometry(phi)[3]
+        a = [self._f(c).copy() for c in accel]
+        self.L.orc_vof_tension(self._f(phi), self._f(curvature), *a, self.nx, self.ny, self.nz, float(sigma),
+                               float(density))
+        return np.stack([c.reshape(self.shape) for c in a])
+
+    # particles: rows (x, y, z, vx, vy, vz, mass, response time)
+    def particles_advance(self, particles, u, accel, dt, two_way, gravity=(0.0, 0.0, 0.0)):
+        p = np.ascontiguousarray(particles, np.float64).reshape(-1, 8)
+        s = np.ascontiguousarray(p.T.astype(np.float32)).reshape(-1)
+        ux, uy, uz = (self._f(c) for c in u)
+        a = [self._f(c).copy() for c in accel]
+        self.L.orc_particles_advance(s, len(p), ux, uy, uz, *a, self.nx, self.ny, self.nz, float(dt), int(two_way),
+                                     *map(float, gravity))
+        return s.reshape(8, -1).T.astype(np.float64), np.stack([c.reshape(self.shape) for c in a])
+
+    def q_criterion(self, u):
+        q = np.empty(self.cells, np.float32)
+        ux, uy, uz = (self._f(c) for c in u)
+        self.L.orc_q_criterion(ux, uy, uz, q, self.nx, self.ny, self.nz)
+        return q.reshape(self.shape)
 
 
 # ---------------------------------------------------------------- compiled reference (oracle/_ref)

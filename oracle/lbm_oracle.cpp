@@ -19,6 +19,9 @@
 //   * local body acceleration .......... esoteric_pull.hpp:144-159,414-420
 //   * STL ray-parity voxelisation ...... src/lbm/voxelize.cpp:4-5 (every cell against every
 //                                        triangle, double), transform_surface voxelize.cpp:4,6
+//   * resident coupling modules ........ include/cfd/solvers/lbm/resident_multiphysics_sycl.hpp:
+//                                        D3Q7 thermal + Boussinesq :54-198, volume of fluid :200-324,
+//                                        particles :326-380, Q-criterion :382-408 (float, periodic)
 //
 // Extensions the reference does not have (spec restated here, pinned by closed forms in tests/):
 //   * seeded noise on the Taylor-Green initial velocity (SplitMix64 counter hash),
@@ -1254,6 +1257,266 @@ inline void transform(const double* in, size_t count, const double* tr, const do
     }
 }
 
+// ---------------------------------------------------------------- resident coupling modules
+// Periodic lattices of nx*ny*nz cells, cell n = (z*ny + y)*nx + x, float arithmetic in the
+// reference's order (resident_multiphysics_sycl.hpp:21-48 for the index helpers).
+namespace mp {
+struct Grid {
+    size_t nx, ny, nz;
+    size_t cells() const { return nx * ny * nz; }
+    size_t at(size_t x, size_t y, size_t z) const { return (z * ny + y) * nx + x; }
+    static size_t shift(size_t v, int delta, size_t n) {
+        if (delta > 0) return v + 1 == n ? 0 : v + 1;
+        if (delta < 0) return v == 0 ? n - 1 : v - 1;
+        return v;
+    }
+    void split(size_t n, size_t& x, size_t& y, size_t& z) const {
+        x = n % nx;
+        y = (n / nx) % ny;
+        z = n / (nx * ny);
+    }
+};
+
+// D3Q7: 0, +x, -x, +y, -y, +z, -z; weights 1/4, 1/8 (:188-191).
+inline int c7x(int d) { return d == 1 ? 1 : d == 2 ? -1 : 0; }
+inline int c7y(int d) { return d == 3 ? 1 : d == 4 ? -1 : 0; }
+inline int c7z(int d) { return d == 5 ? 1 : d == 6 ? -1 : 0; }
+inline float w7(int d) { return d == 0 ? 0.25F : 0.125F; }
+
+inline void thermal_fill(float* g, const float* t, float value, size_t cells) {
+    for (size_t n = 0; n < cells; ++n)
+        for (int d = 0; d < 7; ++d) g[d * cells + n] = w7(d) * (t ? t[n] : value);
+}
+
+// step (:93-125): pull from x - c, BGK towards w T (1 + 4 c.u).
+inline void thermal_step(const float* in, float* out, const float* ux, const float* uy, const float* uz, Grid g,
+                         float omega) {
+    const size_t cells = g.cells();
+#pragma omp parallel for
+    for (long long nn = 0; nn < static_cast<long long>(cells); ++nn) {
+        const size_t n = static_cast<size_t>(nn);
+        size_t x, y, z;
+        g.split(n, x, y, z);
+        float local[7];
+        float temperature = 0.0F;
+        for (int d = 0; d < 7; ++d) {
+            const size_t src = g.at(Grid::shift(x, -c7x(d), g.nx), Grid::shift(y, -c7y(d), g.ny),
+                                    Grid::shift(z, -c7z(d), g.nz));
+            local[d] = in[d * cells + src];
+            temperature += local[d];
+        }
+        for (int d = 0; d < 7; ++d) {
+            const float cu = static_cast<float>(c7x(d)) * ux[n] + static_cast<float>(c7y(d)) * uy[n] +
+                             static_cast<float>(c7z(d)) * uz[n];
+            const float geq = w7(d) * temperature * (1.0F + 4.0F * cu);
+            out[d * cells + n] = local[d] - omega * (local[d] - geq);
+        }
+    }
+}
+
+inline void thermal_scalar(const float* g, float* t, size_t cells) {
+    for (size_t n = 0; n < cells; ++n) {
+        float v = 0.0F;
+        for (int d = 0; d < 7; ++d) v += g[d * cells + n];
+        t[n] = v;
+    }
+}
+
+// add_boussinesq_acceleration (:140-155)
+inline void boussinesq(const float* t, float* ax, float* ay, float* az, size_t cells, float t_ref, float beta,
+                       float gx, float gy, float gz) {
+    for (size_t n = 0; n < cells; ++n) {
+        const float scale = -beta * (t[n] - t_ref);
+        ax[n] += scale * gx;
+        ay[n] += scale * gy;
+        az[n] += scale * gz;
+    }
+}
+
+// initialize_half_space_x (:225-236)
+inline void vof_half_space(float* phi, Grid g, float interface_x) {
+    for (size_t n = 0; n < g.cells(); ++n) {
+        const float left = static_cast<float>(n % g.nx), right = left + 1.0F;
+        phi[n] = interface_x <= left ? 0.0F : interface_x >= right ? 1.0F : interface_x - left;
+    }
+}
+
+// advect (:238-263): upwind face fluxes, clamped to [0, 1].
+inline void vof_advect(const float* phi, float* out, const float* ux, const float* uy, const float* uz, Grid g,
+                       float dt) {
+    for (size_t n = 0; n < g.cells(); ++n) {
+        size_t x, y, z;
+        g.split(n, x, y, z);
+        const size_t xp = g.at(Grid::shift(x, 1, g.nx), y, z), xm = g.at(Grid::shift(x, -1, g.nx), y, z);
+        const size_t yp = g.at(x, Grid::shift(y, 1, g.ny), z), ym = g.at(x, Grid::shift(y, -1, g.ny), z);
+        const size_t zp = g.at(x, y, Grid::shift(z, 1, g.nz)), zm = g.at(x, y, Grid::shift(z, -1, g.nz));
+        const float uxp = 0.5F * (ux[n] + ux[xp]), uxm = 0.5F * (ux[xm] + ux[n]);
+        const float uyp = 0.5F * (uy[n] + uy[yp]), uym = 0.5F * (uy[ym] + uy[n]);
+        const float uzp = 0.5F * (uz[n] + uz[zp]), uzm = 0.5F * (uz[zm] + uz[n]);
+        const float fxp = uxp >= 0.0F ? uxp * phi[n] : uxp * phi[xp];
+        const float fxm = uxm >= 0.0F ? uxm * phi[xm] : uxm * phi[n];
+        const float fyp = uyp >= 0.0F ? uyp * phi[n] : uyp * phi[yp];
+        const float fym = uym >= 0.0F ? uym * phi[ym] : uym * phi[n];
+        const float fzp = uzp >= 0.0F ? uzp * phi[n] : uzp * phi[zp];
+        const float fzm = uzm >= 0.0F ? uzm * phi[zm] : uzm * phi[n];
+        float value = phi[n] - dt * ((fxp - fxm) + (fyp - fym) + (fzp - fzm));
+        value = value < 0.0F ? 0.0F : value > 1.0F ? 1.0F : value;
+        out[n] = value;
+    }
+}
+
+inline void grad(const float* phi, Grid g, size_t n, float& gx, float& gy, float& gz) {
+    size_t x, y, z;
+    g.split(n, x, y, z);
+    gx = 0.5F * (phi[g.at(Grid::shift(x, 1, g.nx), y, z)] - phi[g.at(Grid::shift(x, -1, g.nx), y, z)]);
+    gy = 0.5F * (phi[g.at(x, Grid::shift(y, 1, g.ny), z)] - phi[g.at(x, Grid::shift(y, -1, g.ny), z)]);
+    gz = 0.5F * (phi[g.at(x, y, Grid::shift(z, 1, g.nz))] - phi[g.at(x, y, Grid::shift(z, -1, g.nz))]);
+}
+
+// refresh_geometry (:268-297): geo = [normal x | normal y | normal z | curvature].
+inline void vof_geometry(const float* phi, float* geo, Grid g) {
+    const size_t c = g.cells();
+    for (size_t n = 0; n < c; ++n) {
+        float gx, gy, gz;
+        grad(phi, g, n, gx, gy, gz);
+        const float mag = std::sqrt(gx * gx + gy * gy + gz * gz);
+        if (mag > 1.0e-12F) {
+            geo[n] = gx / mag;
+            geo[c + n] = gy / mag;
+            geo[2 * c + n] = gz / mag;
+        } else {
+            geo[n] = geo[c + n] = geo[2 * c + n] = 0.0F;
+        }
+    }
+    for (size_t n = 0; n < c; ++n) {
+        size_t x, y, z;
+        g.split(n, x, y, z);
+        const float dxn = geo[g.at(Grid::shift(x, 1, g.nx), y, z)] - geo[g.at(Grid::shift(x, -1, g.nx), y, z)];
+        const float dyn = geo[c + g.at(x, Grid::shift(y, 1, g.ny), z)] - geo[c + g.at(x, Grid::shift(y, -1, g.ny), z)];
+        const float dzn =
+            geo[2 * c + g.at(x, y, Grid::shift(z, 1, g.nz))] - geo[2 * c + g.at(x, y, Grid::shift(z, -1, g.nz))];
+        geo[3 * c + n] = -0.5F * ((dxn + dyn) + dzn);
+    }
+}
+
+// add_surface_tension_acceleration (:299-317), with the curvature of vof_geometry.
+inline void vof_tension(const float* phi, const float* curvature, float* ax, float* ay, float* az, Grid g,
+                        float sigma, float density) {
+    for (size_t n = 0; n < g.cells(); ++n) {
+        float gx, gy, gz;
+        grad(phi, g, n, gx, gy, gz);
+        const float c = sigma * curvature[n] / density;
+        ax[n] += c * gx;
+        ay[n] += c * gy;
+        az[n] += c * gz;
+    }
+}
+
+inline long long floor_to_ll(float v) {
+    long long i = static_cast<long long>(v);
+    if (static_cast<float>(i) > v) --i;
+    return i;
+}
+inline float periodic_coordinate(float value, size_t extent) {
+    const float e = static_cast<float>(extent);
+    const long long turns = floor_to_ll(value / e);
+    value -= static_cast<float>(turns) * e;
+    if (value >= e) value -= e;
+    if (value < 0.0F) value += e;
+    return value;
+}
+inline size_t wrap_cell(long long v, size_t extent) {
+    const long long e = static_cast<long long>(extent);
+    v %= e;
+    if (v < 0) v += e;
+    return static_cast<size_t>(v);
+}
+
+// advance (:346-366), particles in order (the two-way sums in that order).
+inline void particles_advance(float* s, size_t count, const float* ux, const float* uy, const float* uz, float* ax,
+                              float* ay, float* az, Grid g, float dt, bool two_way, float gx, float gy, float gz) {
+    for (size_t p = 0; p < count; ++p) {
+        float px = periodic_coordinate(s[p], g.nx), py = periodic_coordinate(s[count + p], g.ny),
+              pz = periodic_coordinate(s[2 * count + p], g.nz);
+        float vx = s[3 * count + p], vy = s[4 * count + p], vz = s[5 * count + p];
+        const float mass = s[6 * count + p], response = s[7 * count + p];
+        const long long x0 = floor_to_ll(px), y0 = floor_to_ll(py), z0 = floor_to_ll(pz);
+        const float tx = px - static_cast<float>(x0), ty = py - static_cast<float>(y0), tz = pz - static_cast<float>(z0);
+        float fu = 0.0F, fv = 0.0F, fw = 0.0F;
+        size_t ids[8];
+        float weights[8];
+        int k = 0;
+        for (int dz = 0; dz <= 1; ++dz)
+            for (int dy = 0; dy <= 1; ++dy)
+                for (int dx = 0; dx <= 1; ++dx) {
+                    const float wx = dx ? tx : 1.0F - tx, wy = dy ? ty : 1.0F - ty, wz = dz ? tz : 1.0F - tz;
+                    const size_t n = g.at(wrap_cell(x0 + dx, g.nx), wrap_cell(y0 + dy, g.ny), wrap_cell(z0 + dz, g.nz));
+                    const float w = wx * wy * wz;
+                    ids[k] = n;
+                    weights[k] = w;
+                    ++k;
+                    fu += w * ux[n];
+                    fv += w * uy[n];
+                    fw += w * uz[n];
+                }
+        const float dragx = mass * (fu - vx) / response, dragy = mass * (fv - vy) / response,
+                    dragz = mass * (fw - vz) / response;
+        if (two_way)
+            for (int j = 0; j < 8; ++j) {
+                ax[ids[j]] += -dragx * weights[j];
+                ay[ids[j]] += -dragy * weights[j];
+                az[ids[j]] += -dragz * weights[j];
+            }
+        vx += dt * (dragx / mass + gx);
+        vy += dt * (dragy / mass + gy);
+        vz += dt * (dragz / mass + gz);
+        px = periodic_coordinate(px + dt * vx, g.nx);
+        py = periodic_coordinate(py + dt * vy, g.ny);
+        pz = periodic_coordinate(pz + dt * vz, g.nz);
+        s[p] = px;
+        s[count + p] = py;
+        s[2 * count + p] = pz;
+        s[3 * count + p] = vx;
+        s[4 * count + p] = vy;
+        s[5 * count + p] = vz;
+    }
+}
+
+// q_criterion (:394-404)
+inline void q_criterion(const float* ux, const float* uy, const float* uz, float* q, Grid g) {
+    for (size_t n = 0; n < g.cells(); ++n) {
+        size_t x, y, z;
+        g.split(n, x, y, z);
+        float gr[3][3] = {};
+        for (int axis = 0; axis < 3; ++axis) {
+            size_t xp = x, xm = x, yp = y, ym = y, zp = z, zm = z;
+            if (axis == 0) {
+                xp = Grid::shift(x, 1, g.nx);
+                xm = Grid::shift(x, -1, g.nx);
+            } else if (axis == 1) {
+                yp = Grid::shift(y, 1, g.ny);
+                ym = Grid::shift(y, -1, g.ny);
+            } else {
+                zp = Grid::shift(z, 1, g.nz);
+                zm = Grid::shift(z, -1, g.nz);
+            }
+            const size_t a = g.at(xp, yp, zp), b = g.at(xm, ym, zm);
+            gr[0][axis] = 0.5F * (ux[a] - ux[b]);
+            gr[1][axis] = 0.5F * (uy[a] - uy[b]);
+            gr[2][axis] = 0.5F * (uz[a] - uz[b]);
+        }
+        float s2 = 0.0F, o2 = 0.0F;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const float sij = 0.5F * (gr[i][j] + gr[j][i]), oij = 0.5F * (gr[i][j] - gr[j][i]);
+                s2 += sij * sij;
+                o2 += oij * oij;
+            }
+        q[n] = 0.5F * (o2 - s2);
+    }
+}
+}  // namespace mp
+
 }  // namespace orc
 
 // ---------------------------------------------------------------- C ABI for the tests (ctypes)
@@ -1350,4 +1613,38 @@ ORC_API int orc_voxelize(const double* tris, size_t count, size_t nx, size_t ny,
 ORC_API int orc_transform_surface(const double* tris, size_t count, const double* tr, const double* axis,
                                   double angle, double* out) {
     return guarded([&] { orc::transform(tris, count, tr, axis, angle, out); });
+}
+// resident coupling modules (periodic float lattices)
+ORC_API void orc_thermal_fill(float* g, const float* t, float value, size_t cells) { orc::mp::thermal_fill(g, t, value, cells); }
+ORC_API void orc_thermal_step(const float* in, float* out, const float* ux, const float* uy, const float* uz,
+                              size_t nx, size_t ny, size_t nz, float omega) {
+    orc::mp::thermal_step(in, out, ux, uy, uz, {nx, ny, nz}, omega);
+}
+ORC_API void orc_thermal_scalar(const float* g, float* t, size_t cells) { orc::mp::thermal_scalar(g, t, cells); }
+ORC_API void orc_boussinesq(const float* t, float* ax, float* ay, float* az, size_t cells, float t_ref, float beta,
+                            float gx, float gy, float gz) {
+    orc::mp::boussinesq(t, ax, ay, az, cells, t_ref, beta, gx, gy, gz);
+}
+ORC_API void orc_vof_half_space(float* phi, size_t nx, size_t ny, size_t nz, float x) {
+    orc::mp::vof_half_space(phi, {nx, ny, nz}, x);
+}
+ORC_API void orc_vof_advect(const float* phi, float* out, const float* ux, const float* uy, const float* uz, size_t nx,
+                            size_t ny, size_t nz, float dt) {
+    orc::mp::vof_advect(phi, out, ux, uy, uz, {nx, ny, nz}, dt);
+}
+ORC_API void orc_vof_geometry(const float* phi, float* geo, size_t nx, size_t ny, size_t nz) {
+    orc::mp::vof_geometry(phi, geo, {nx, ny, nz});
+}
+ORC_API void orc_vof_tension(const float* phi, const float* curvature, float* ax, float* ay, float* az, size_t nx,
+                             size_t ny, size_t nz, float sigma, float density) {
+    orc::mp::vof_tension(phi, curvature, ax, ay, az, {nx, ny, nz}, sigma, density);
+}
+ORC_API void orc_particles_advance(float* s, size_t count, const float* ux, const float* uy, const float* uz, float* ax,
+                                   float* ay, float* az, size_t nx, size_t ny, size_t nz, float dt, int two_way,
+                                   float gx, float gy, float gz) {
+    orc::mp::particles_advance(s, count, ux, uy, uz, ax, ay, az, {nx, ny, nz}, dt, two_way != 0, gx, gy, gz);
+}
+ORC_API void orc_q_criterion(const float* ux, const float* uy, const float* uz, float* q, size_t nx, size_t ny,
+                             size_t nz) {
+    orc::mp::q_criterion(ux, uy, uz, q, {nx, ny, nz});
 }
