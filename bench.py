@@ -2,7 +2,7 @@
 """Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
 
     python bench.py [--steps K] [--warmup W]
-                    [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident[,config5]] [--no-cpu]
+                    [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard[,config5]] [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
@@ -11,7 +11,8 @@ geometry with FP16 and FP32 shifted-population storage, FP64 (the reference for 
 error of the others) and D3Q27, and the adjoint leg runs the
 checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps), and the
 voxelize leg the GPU STL voxelisation of a 16128-triangle sphere on 512^3 cells, and the resident
-leg the device-resident coupling modules (thermal, fill, particles, Q-criterion) at 256^3.  The opt-in
+leg the device-resident coupling modules (thermal, fill, particles, Q-criterion) at 256^3, and the shard
+leg the kernel on one GPU's share of a 768^3 lattice split in eight z-slabs.  The opt-in
 config5 leg runs forward + adjoint of a 768^3 FP16 duct past a sphere cluster (drag objective,
 binomial checkpointing) against the FP32-storage run.  Prints progress on stderr and ONE JSON
 line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
@@ -94,6 +95,31 @@ def run_leg(hip, name, lattice, storage, n, tau, steps, warmup):
         f"({100 * gbs / PEAK_GBS:.1f}% of 8 TB/s), mass drift {res['mass_rel_drift']:.2e}")
     s.close()
     return res, velocity
+
+
+def shard_leg(hip, shape=(768, 768, 96), steps=100, warmup=10, storages=("fp16s", "fp32")):
+    """The kernel at one GPU's share of a z-slab decomposition: 768 x 768 x 96 cells is one of eight
+    slabs of a 768^3 lattice.  On one GPU the slab is periodic in z where the distributed run would
+    exchange ghost planes; the per-cell work of the step is the same.  Periodic Taylor-Green start
+    (A 0.02, tau 0.6)."""
+    nx, ny, nz = shape
+    cells = nx * ny * nz
+    out = {"workload": f"D3Q19 {nx}x{ny}x{nz} periodic Taylor-Green slab (one GPU's share of 768^3 on 8)"}
+    for storage in storages:
+        s = hip.EsotericPullHipSolver("d3q19", nx, ny, nz, 0.6, storage=storage)
+        s.initialize_taylor_green(0.02)
+        s.step(warmup)
+        s.wait()
+        gpu_ms = s.step_timed(steps)
+        nbytes = cells * 2 * 19 * BYTES_PER_POP[storage]
+        launch_s = gpu_ms * 1e-3 / steps
+        out[storage] = {"kernel_ms": gpu_ms / steps, "mlups": cells / launch_s / 1e6,
+                        "algorithmic_bytes_per_launch": nbytes, "gbs": nbytes / launch_s / 1e9,
+                        "roofline_frac": nbytes / launch_s / 1e9 / PEAK_GBS, "finite": bool(np.isfinite(s.total_mass()))}
+        log(f"shard {storage}: {out[storage]['mlups']:.0f} MLUPS, kernel {gpu_ms / steps:.3f} ms, "
+            f"{out[storage]['gbs']:.0f} GB/s ({100 * out[storage]['roofline_frac']:.1f}% of 8 TB/s)")
+        s.close()
+    return out
 
 
 def duct_flags(n):
@@ -373,7 +399,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident")
+    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard")
     ap.add_argument("--config5-steps", type=int, default=1000)
     ap.add_argument("--config5-n", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
@@ -403,6 +429,9 @@ def main():
             continue
         if name == "resident":
             legs[name] = resident_leg(hip)
+            continue
+        if name == "shard":
+            legs[name] = shard_leg(hip)
             continue
         lattice, storage = table[name]
         legs[name], vel = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
