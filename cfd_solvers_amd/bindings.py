@@ -116,6 +116,8 @@ _SIGNATURES = {
     "cfd_lbm_flow_diagnostics_destroy": [C.c_void_p],
     "cfd_lbm_q_criterion": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],
     "cfd_lbm_get_flags": [C.c_void_p, C.c_void_p],
+    "cfd_lbm_save_checkpoint": [C.c_void_p, C.c_char_p, C.c_int],
+    "cfd_lbm_load_checkpoint": [C.c_void_p, C.c_char_p],
 }
 
 OBJECTIVES = {"pressure_drop": 0, "drag": 1}
@@ -300,6 +302,14 @@ class EsotericPullHipSolver:
         ptrs = [C.c_void_p() for _ in range(4)]
         self.lib.call("cfd_lbm_refresh_device_macroscopic", self._h, *(C.byref(p) for p in ptrs))
         return tuple(p.value for p in ptrs)
+
+    def save_checkpoint(self, path, raw=True):
+        """CFDCHK3 checkpoint (distributed_pull.hpp:176-203) plus a trailer with the parity and,
+        with raw, the exact device state."""
+        self.lib.call("cfd_lbm_save_checkpoint", self._h, os.fsencode(path), int(bool(raw)))
+
+    def load_checkpoint(self, path):
+        self.lib.call("cfd_lbm_load_checkpoint", self._h, os.fsencode(path))
 
     def device_local_acceleration(self, clear=False):
         """(ax, ay, az) device pointers of the per-cell acceleration planes

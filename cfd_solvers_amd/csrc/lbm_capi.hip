@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <fstream>
 #include <functional>
 #include <memory>
 #include <stdexcept>
@@ -443,6 +444,166 @@ struct cfd_lbm_solver {
     }
 
     void host_flags_assign(const uint8_t* v) { this->host_flags.assign(v, v + cells); }
+
+    // ------------------------------------------------------------------ checkpoint / restart
+    // File: the CFDCHK3 layout of DistributedPullBlock::save_checkpoint
+    // (include/cfd/solvers/lbm/distributed_pull.hpp:176-203) for one brick covering the lattice,
+    // then a trailer the reference's loader does not read:
+    //   "CFDESO1\0", u32 parity (time_step & 1), u32 body convention (0: the two-grid block's
+    //   outgoing populations, 1: this solver's logical incoming populations), u32 storage,
+    //   u32 raw (1: the device state follows), and with raw: u32 row_shift, u32 block_elems,
+    //   u64 stride, u64 population bytes, the population array, u64 wall-density bytes, those bytes.
+    // Periodic lattices write the outgoing convention, f_out(d, x) = f_in(d, x + c_d), so that
+    // the reference's block restores them; lattices with flags write the logical populations.
+    template <class V> static void put(std::ofstream& o, const V& v) { o.write(reinterpret_cast<const char*>(&v), sizeof(V)); }
+    template <class V> static V get(std::ifstream& in) {
+        V v{};
+        in.read(reinterpret_cast<char*>(&v), sizeof(V));
+        return v;
+    }
+
+    // f_out(d, x) = f_in(d, x + c_d) (dir = +1) or its inverse (dir = -1), periodic.
+    template <int Q> void shift_streams(const float* in, float* out, int dir) const {
+        using L = cfd_hip::Lattice<Q>;
+        for (int d = 0; d < Q; ++d)
+            for (size_t z = 0; z < nz; ++z)
+                for (size_t y = 0; y < ny; ++y)
+                    for (size_t x = 0; x < nx; ++x) {
+                        const auto wrap = [](size_t v, int delta, size_t n) {
+                            const long long s = static_cast<long long>(v) + delta + static_cast<long long>(n);
+                            return static_cast<size_t>(s % static_cast<long long>(n));
+                        };
+                        const size_t xs = wrap(x, dir * L::cx[d], nx), ys = wrap(y, dir * L::cy[d], ny),
+                                     zs = wrap(z, dir * L::cz[d], nz);
+                        out[d * cells + (z * ny + y) * nx + x] = in[d * cells + (zs * ny + ys) * nx + xs];
+                    }
+    }
+    void shift_streams(const float* in, float* out, int dir) const {
+        if (q == 9) shift_streams<9>(in, out, dir);
+        else if (q == 19) shift_streams<19>(in, out, dir);
+        else shift_streams<27>(in, out, dir);
+    }
+
+    void save_checkpoint(const char* path, bool raw) {
+        if (!path) throw std::invalid_argument("save_checkpoint needs a path");
+        use_device();
+        std::vector<float> logical(static_cast<size_t>(q) * cells);
+        download_populations(logical.data());
+        const bool periodic = host_flags.empty();
+        std::vector<float> body;
+        if (periodic) {
+            body.resize(logical.size());
+            shift_streams(logical.data(), body.data(), +1);
+        }
+        std::ofstream o(path, std::ios::binary | std::ios::trunc);
+        if (!o) throw std::runtime_error(std::string("cannot open checkpoint for writing: ") + path);
+        o.write("CFDCHK3\0", 8);
+        put(o, uint32_t{1});
+        put(o, uint32_t{0x01020304U});
+        put(o, static_cast<uint32_t>(q));
+        const uint64_t ext[3] = {nx, ny, nz}, zero[3] = {0, 0, 0};
+        for (uint64_t v : ext) put(o, v);  // global
+        for (uint64_t v : zero) put(o, v); // begin
+        for (uint64_t v : ext) put(o, v);  // extent
+        put(o, static_cast<uint64_t>(t));
+        put(o, tau);
+        put(o, ax);
+        put(o, ay);
+        put(o, az);
+        const std::vector<float>& out = periodic ? body : logical;
+        o.write(reinterpret_cast<const char*>(out.data()), static_cast<std::streamsize>(out.size() * sizeof(float)));
+        o.write("CFDESO1\0", 8);
+        put(o, static_cast<uint32_t>(t & 1U));
+        put(o, static_cast<uint32_t>(periodic ? 0U : 1U));
+        put(o, static_cast<uint32_t>(storage));
+        put(o, static_cast<uint32_t>(raw ? 1U : 0U));
+        if (raw) {
+            put(o, geo.row_shift);
+            put(o, geo.block_elems);
+            put(o, geo.stride);
+            put(o, static_cast<uint64_t>(f.bytes));
+            std::vector<unsigned char> host(f.bytes);
+            d2h(host.data(), f.ptr, f.bytes);
+            std::vector<unsigned char> walls(solid_rho ? solid_rho.bytes : 0);
+            if (solid_rho) d2h(walls.data(), solid_rho.ptr, solid_rho.bytes);
+            sync();
+            o.write(reinterpret_cast<const char*>(host.data()), static_cast<std::streamsize>(host.size()));
+            put(o, static_cast<uint64_t>(walls.size()));
+            o.write(reinterpret_cast<const char*>(walls.data()), static_cast<std::streamsize>(walls.size()));
+        }
+        if (!o) throw std::runtime_error(std::string("checkpoint write failed: ") + path);
+    }
+
+    // load_checkpoint (distributed_pull.hpp:205-245) for this solver: the same checks of magic,
+    // format, lattice, extents and configuration; then the raw device state when the trailer has one
+    // for this solver's storage and layout (exact for every storage, walls included), otherwise
+    // the float populations (a two-grid file restores periodic lattices only).
+    void load_checkpoint(const char* path) {
+        if (!path) throw std::invalid_argument("load_checkpoint needs a path");
+        std::ifstream in(path, std::ios::binary);
+        if (!in) throw std::runtime_error(std::string("cannot open checkpoint for reading: ") + path);
+        char magic[8] = {};
+        in.read(magic, 8);
+        if (!in || std::memcmp(magic, "CFDCHK3\0", 8) != 0) throw std::runtime_error("invalid checkpoint magic");
+        const auto version = get<uint32_t>(in), endian = get<uint32_t>(in), stored_q = get<uint32_t>(in);
+        if (version != 1 || endian != 0x01020304U || stored_q != static_cast<uint32_t>(q))
+            throw std::runtime_error("unsupported checkpoint format/descriptor");
+        uint64_t global[3], begin[3], extent[3];
+        for (auto& v : global) v = get<uint64_t>(in);
+        for (auto& v : begin) v = get<uint64_t>(in);
+        for (auto& v : extent) v = get<uint64_t>(in);
+        const uint64_t mine[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; ++a)
+            if (global[a] != mine[a] || extent[a] != mine[a] || begin[a] != 0)
+                throw std::runtime_error("checkpoint decomposition does not match block");
+        const auto step = get<uint64_t>(in);
+        const float ctau = get<float>(in), cax = get<float>(in), cay = get<float>(in), caz = get<float>(in);
+        if (ctau != tau || cax != ax || cay != ay || caz != az)
+            throw std::runtime_error("checkpoint LBM configuration mismatch");
+        std::vector<float> body(static_cast<size_t>(q) * cells);
+        in.read(reinterpret_cast<char*>(body.data()), static_cast<std::streamsize>(body.size() * sizeof(float)));
+        if (!in) throw std::runtime_error("checkpoint read failed");
+        uint32_t convention = 0, stored_storage = 0, raw = 0;
+        in.read(magic, 8);
+        const bool trailer = in && std::memcmp(magic, "CFDESO1\0", 8) == 0;
+        if (trailer) {
+            (void)get<uint32_t>(in);  // parity, implied by the time step
+            convention = get<uint32_t>(in);
+            stored_storage = get<uint32_t>(in);
+            raw = get<uint32_t>(in);
+            if (!in) throw std::runtime_error("checkpoint read failed");
+        }
+        use_device();
+        if (raw) {
+            const auto rs = get<uint32_t>(in), be = get<uint32_t>(in);
+            const auto st = get<uint64_t>(in), fb = get<uint64_t>(in);
+            if (stored_storage == static_cast<uint32_t>(storage) && rs == geo.row_shift && be == geo.block_elems &&
+                st == geo.stride && fb == f.bytes) {
+                std::vector<unsigned char> host(fb);
+                in.read(reinterpret_cast<char*>(host.data()), static_cast<std::streamsize>(fb));
+                const auto wb = get<uint64_t>(in);
+                std::vector<unsigned char> walls(wb);
+                in.read(reinterpret_cast<char*>(walls.data()), static_cast<std::streamsize>(wb));
+                if (!in) throw std::runtime_error("checkpoint read failed");
+                if (wb != (solid_rho ? solid_rho.bytes : 0))
+                    throw std::runtime_error("checkpoint wall state does not match the solver's flags");
+                t = step;
+                h2d(f.ptr, host.data(), fb);
+                if (wb) h2d(solid_rho.ptr, walls.data(), wb);
+                sync();
+                return;
+            }
+        }
+        if (convention == 0 && !host_flags.empty())
+            throw std::runtime_error("a two-grid checkpoint restores a periodic lattice only");
+        std::vector<float> logical;
+        if (convention == 0) {
+            logical.resize(body.size());
+            shift_streams(body.data(), logical.data(), -1);
+        }
+        t = step;
+        upload_populations(convention == 0 ? logical.data() : body.data());
+    }
 
     // set_solid_mask(voxelize_surface(surface, grid)) (examples/real_lbm3d_model.cpp:146-151) on the
     // device: the mask is voxelised straight into the new flag array, solid cells become walls at
@@ -1240,6 +1401,14 @@ int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_
         if (forward_ms) *forward_ms = need(s).last_forward_ms;
         if (reverse_ms) *reverse_ms = s->last_reverse_ms;
     });
+}
+
+int cfd_lbm_save_checkpoint(cfd_lbm_solver* s, const char* path, int raw, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).save_checkpoint(path, raw != 0); });
+}
+
+int cfd_lbm_load_checkpoint(cfd_lbm_solver* s, const char* path, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).load_checkpoint(path); });
 }
 
 int cfd_lbm_voxelize_surface(const double* triangles, size_t count, size_t nx, size_t ny, size_t nz,

@@ -195,6 +195,20 @@ CFD_LBM_API int cfd_lbm_checkpoint_plan(uint64_t steps, uint64_t snapshots, uint
 CFD_LBM_API int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_ms, char* error,
                                        size_t error_capacity);
 
+/* Checkpoint / restart.  The file starts with the CFDCHK3 layout of DistributedPullBlock::
+ * save_checkpoint (include/cfd/solvers/lbm/distributed_pull.hpp:176-203) for one brick covering
+ * the lattice: magic "CFDCHK3", version 1, endian marker, q, global/begin/extent (u64), time step,
+ * tau and acceleration (f32), then q x cells float populations.  Periodic lattices store the
+ * two-grid block's outgoing populations f(d, x + c_d), so the reference's load_checkpoint
+ * (:205-245) restores them; lattices with flags store the logical populations.  A trailer
+ * ("CFDESO1") records the parity of the time step, the body's convention and the storage, and
+ * with raw != 0 the device state itself, which restores every storage (FP16S included) and the
+ * wall densities bit for bit.  load_checkpoint reads either kind of file; the flags, wall
+ * velocities and other settings must be set as when the file was written. */
+CFD_LBM_API int cfd_lbm_save_checkpoint(cfd_lbm_solver* s, const char* path, int raw, char* error,
+                                        size_t error_capacity);
+CFD_LBM_API int cfd_lbm_load_checkpoint(cfd_lbm_solver* s, const char* path, char* error, size_t error_capacity);
+
 /* voxelize_surface -- src/lbm/voxelize.cpp:5 (include/cfd/solvers/lbm/voxelize.hpp:9), run on the GPU
  * as the reference's device variant does (src/sycl/voxelize_sycl.cpp:13-68).  triangles: count x 9
  * doubles (a.x a.y a.z b.x b.y b.z c.x c.y c.z, the layout of cfd::io::Triangle3, host memory);

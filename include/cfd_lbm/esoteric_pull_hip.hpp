@@ -170,6 +170,11 @@ public:
         mask_.resize(cells_);
         for (std::size_t i = 0; i < cells_; ++i) mask_[i] = (flags_[i] & CFD_LBM_WALL) != 0U ? 1U : 0U;
     }
+    // Checkpoint / restart in the CFDCHK3 layout (distributed_pull.hpp:176-245) plus the parity.
+    void save_checkpoint(const std::string& path, bool raw = true) {
+        call(cfd_lbm_save_checkpoint(h_, path.c_str(), raw ? 1 : 0, err_, sizeof(err_)));
+    }
+    void load_checkpoint(const std::string& path) { call(cfd_lbm_load_checkpoint(h_, path.c_str(), err_, sizeof(err_))); }
     void clear_solid_mask() {
         flags_.clear();
         mask_.clear();

@@ -267,6 +267,19 @@ def ref_voxelize(triangles, shape, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), nativ
     return _vox(L.ref_voxelize, L, "ref", triangles, shape, lo, hi)
 
 
+def ref_checkpoint_tg(shape, tau, amplitude, steps, path):
+    """The reference's DistributedPullBlock<D3Q19> (one brick): Taylor-Green start, `steps` virtual
+    steps, CFDCHK3 file at `path` (distributed_pull.hpp:176-203)."""
+    L = ref_lib()
+    _check(L.ref_dist_tg_checkpoint(*shape, tau, amplitude, steps, os.fsencode(path)), L, "ref")
+
+
+def ref_checkpoint_restart(shape, tau, path_in, steps, path_out):
+    """The reference block restored from `path_in` (:205-245), `steps` steps, saved to `path_out`."""
+    L = ref_lib()
+    _check(L.ref_dist_restart(*shape, tau, os.fsencode(path_in), steps, os.fsencode(path_out)), L, "ref")
+
+
 def ref_transform_surface(triangles, translation, axis, angle):
     L = ref_lib()
     return _xform(L.ref_transform_surface, L, "ref", triangles, translation, axis, angle)
@@ -414,6 +427,10 @@ def ref_lib(native=False):
         L.ref_d2q9_mass.argtypes = [C.c_void_p]
         L.ref_d2q9_mass.restype = C.c_double
         L.ref_voxelize.argtypes = [_dp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp, _u8p]
+        L.ref_dist_tg_checkpoint.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_float, C.c_size_t,
+                                             C.c_char_p]
+        L.ref_dist_restart.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_char_p, C.c_size_t,
+                                       C.c_char_p]
         L.ref_transform_surface.argtypes = [_dp, C.c_size_t, _dp, _dp, C.c_double, _dp]
         _ref[key] = L
     return _ref[key]

@@ -6,8 +6,11 @@
 //   cfd::lbm::PrecisionPullSolver<D,double> include/cfd/solvers/lbm/precision_pull.hpp
 //   cfd::lbm::D2Q9Solver              src/lbm/d2q9.cpp
 //   cfd::lbm::voxelize_surface, transform_surface   src/lbm/voxelize.cpp:5-6
+//   cfd::lbm::DistributedPullBlock<D3Q19>  include/cfd/solvers/lbm/distributed_pull.hpp (one brick;
+//                                          CFDCHK3 save/load :176-245, virtual_step)
 // through ctypes.  Nothing here re-implements the reference; it only forwards.
 #include "cfd/solvers/lbm/d2q9.hpp"
+#include "cfd/solvers/lbm/distributed_pull.hpp"
 #include "cfd/solvers/lbm/esoteric_pull.hpp"
 #include "cfd/solvers/lbm/one_step_pull.hpp"
 #include "cfd/solvers/lbm/precision_pull.hpp"
@@ -279,5 +282,33 @@ REF_API int ref_transform_surface(const double* tris, std::size_t count, const d
             const double v[9] = {x.a.x, x.a.y, x.a.z, x.b.x, x.b.y, x.b.z, x.c.x, x.c.y, x.c.z};
             std::memcpy(out + 9 * t, v, sizeof(v));
         }
+    });
+}
+
+// One D3Q19 brick covering the lattice (tests/test_main.cpp:225-266 uses 12x10x8, tau 0.71):
+// Taylor-Green start, `steps` virtual steps, CFDCHK3 file at `path`.
+REF_API int ref_dist_tg_checkpoint(std::size_t nx, std::size_t ny, std::size_t nz, float tau, float amplitude,
+                                   std::size_t steps, const char* path) {
+    return guarded([&] {
+        const cfd::distributed::Extent3 global{nx, ny, nz};
+        const cfd::lbm::DistributedLbmConfig config{global, tau};
+        std::vector<cfd::lbm::DistributedPullBlock<cfd::lbm::D3Q19Descriptor>> blocks;
+        blocks.emplace_back(config, cfd::distributed::make_brick(global, cfd::distributed::ProcessGrid{1, 1, 1}, 0));
+        blocks[0].initialize_taylor_green(amplitude);
+        for (std::size_t i = 0; i < steps; ++i) cfd::lbm::virtual_step(blocks);
+        blocks[0].save_checkpoint(path);
+    });
+}
+// Restores `path_in` into a fresh brick, runs `steps` virtual steps and saves to `path_out`.
+REF_API int ref_dist_restart(std::size_t nx, std::size_t ny, std::size_t nz, float tau, const char* path_in,
+                             std::size_t steps, const char* path_out) {
+    return guarded([&] {
+        const cfd::distributed::Extent3 global{nx, ny, nz};
+        const cfd::lbm::DistributedLbmConfig config{global, tau};
+        std::vector<cfd::lbm::DistributedPullBlock<cfd::lbm::D3Q19Descriptor>> blocks;
+        blocks.emplace_back(config, cfd::distributed::make_brick(global, cfd::distributed::ProcessGrid{1, 1, 1}, 0));
+        blocks[0].load_checkpoint(path_in);
+        for (std::size_t i = 0; i < steps; ++i) cfd::lbm::virtual_step(blocks);
+        blocks[0].save_checkpoint(path_out);
     });
 }
