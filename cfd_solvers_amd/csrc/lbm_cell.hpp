@@ -32,11 +32,10 @@ template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t
     // collision and keeps them live (2 VGPRs each), which halves the occupancy.
     asm volatile("" : "+v"(byte_off));
     T* p = reinterpret_cast<T*>(static_cast<unsigned char*>(base) + byte_off);
-#if defined(CFD_LBM_NT_STORE)
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
+    // Streaming stores: no slot written in a step is read again within that step, and the whole
+    // lattice passes through the caches every step (measured ~1 % faster than plain stores).
+    if constexpr (__is_same(T, F32Dev)) __builtin_nontemporal_store(v.v, reinterpret_cast<float*>(p));
+    else __builtin_nontemporal_store(v, p);
 }
 
 // Loads the logical populations of a non-wall cell (bounce-back aware) and records the slots.
