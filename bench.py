@@ -425,8 +425,8 @@ def main():
     ap.add_argument("--no-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
-              "sharding of the lattice is a design (DESIGN.md), not a measured path.", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the z-slab "
+              "sharding of the lattice over several GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import cfd_solvers_amd as hip

@@ -1,7 +1,7 @@
 """Parity of the HIP esoteric-pull path (through the C ABI) with the CPU oracle and with the
 reference's own outputs (tests/golden).  The FP32 kernels are compiled with -ffp-contract=off
-and keep the reference's operation order, so FP32 fields are compared bit for bit; the stated
-north-star bar (rho, u within 1e-6 relative) is asserted as well.
+and keep the reference's operation order, so FP32 fields are compared bit for bit; the 1e-6
+relative bar on rho and u is asserted as well.
 """
 import numpy as np
 import pytest
