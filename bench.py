@@ -20,7 +20,9 @@ binomial checkpointing) against the FP32-storage run.  Prints progress on stderr
 line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
 (apps/cfd-bench.cpp:111-117); the roofline uses algorithmic HBM bytes per launch (each fluid
 population read and written once, plus one read of the flag array) over the kernel time measured
-with HIP events on the solver's stream.
+with HIP events on the solver's stream.  Two probes run in the same process: a plain 2 GiB copy
+and the in-place esoteric-pull access pattern alone (the bandwidth ceiling of an in-place
+stream-collide), with the headline kernel's fraction of the latter.
 """
 from __future__ import annotations
 
@@ -359,6 +361,27 @@ def copy_probe():
     return {"bytes_per_copy": 2 * nbytes, "ms": ms.value, "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS}
 
 
+def inplace_probe(n=256, repeats=20):
+    """The in-place esoteric-pull access pattern alone (hbm_probe_inplace: D3Q19, periodic, the
+    row-blocked layout of the solver's defaults -- 8 rows per block for FP32, 32 for FP16 --
+    streaming stores, trivial arithmetic): the bandwidth ceiling of an in-place stream-collide in
+    that layout.  19 reads + 19 writes per cell."""
+    import ctypes as C
+
+    lib = C.CDLL(str(ROOT / "cfd_solvers_amd" / "libhbm_probe.so"))
+    lib.hbm_probe_inplace.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
+    out = {}
+    for name, elem, rs in (("fp32", 4, 3), ("fp16", 2, 5)):
+        ms = C.c_float()
+        if lib.hbm_probe_inplace(elem, n, n, rs, repeats, C.byref(ms)) != 0:
+            return None
+        nbytes = n ** 3 * 19 * 2 * elem
+        gbs = nbytes / (ms.value * 1e-3) / 1e9
+        out[name] = {"ms": ms.value, "bytes": nbytes, "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS}
+        log(f"in-place pattern probe {name}: {ms.value:.3f} ms -> {gbs:.0f} GB/s ({100 * gbs / PEAK_GBS:.1f}% of 8 TB/s)")
+    return out
+
+
 def cpu_baseline(target_seconds=15.0, voxelize_mesh=None):
     """The reference's EsotericPullSolver<D3Q19> (oracle/_ref, built from the reference sources
     like its Release build) on the host cores: 256^3 periodic Taylor-Green, cfd-bench protocol
@@ -469,6 +492,7 @@ def main():
                 log(f"leg {name}: velocity relative L2 error vs FP64 {err:.2e}")
     velocities.clear()
     probe = None if args.no_probe else copy_probe()
+    inplace = None if args.no_probe else inplace_probe()
     head = legs.get("fp32") or next((v for v in legs.values() if "achieved_gbs" in v), None)
     if head is None:  # opt-in runs of the adjoint / config5 legs only
         print(json.dumps({"metric": METRIC, "value": None, "unit": "MLUPS", "n_gpus": 1, "legs": legs}), flush=True)
@@ -504,7 +528,12 @@ def main():
                      "bytes_per_launch": head["algorithmic_bytes_per_launch"]},
         "legs": legs,
         "copy_probe": probe,
+        "inplace_probe": inplace,
     }
+    if inplace:
+        result["roofline"]["frac_of_inplace_probe"] = head["achieved_gbs"] / inplace["fp32"]["gbs"]
+        if "fp16" in legs:
+            legs["fp16"]["frac_of_inplace_probe"] = legs["fp16"]["achieved_gbs"] / inplace["fp16"]["gbs"]
     if "fp16" in legs and "fp32" in legs:
         result["fp16_over_fp32"] = legs["fp16"]["mlups"] / legs["fp32"]["mlups"]
     if not args.no_cpu:
