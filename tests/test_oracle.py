@@ -111,6 +111,23 @@ def test_noisy_taylor_green_matches_live_reference(q):
     assert np.array_equal(o.raw(), r.raw().astype(np.float64))
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_cpu_device_gate_config_matches_live_reference(q):
+    """The configuration of the reference's CPU/device gate (tests/test_main.cpp:587-615) on the
+    reference's own EsotericPullSolver: the restatement agrees bit for bit (the HIP path is held to
+    the restatement in tests/test_gpu_parity.py)."""
+    n = (16, 14, 12 if q != 9 else 1)
+    accel = (1.0e-6, -0.5e-6, 0.25e-6)
+    o = O.Oracle(q, *n, F32(0.72), tuple(F32(a) for a in accel))
+    o.init_taylor_green(F32(0.018))
+    r = O.RefEsoteric(q, *n, 0.72, accel)
+    r.init_taylor_green(0.018)
+    o.step(12)
+    r.step(12)
+    assert np.array_equal(fields(o), np.stack(r.macroscopic()).reshape(4, -1).astype(np.float64))
+
+
 # ---------------------------------------------------------------- bounded path
 def test_reference_bounded_defect_is_recorded(golden):
     """The reference's all-fluid-mask step differs from its periodic step by ~19% of |u|."""
