@@ -302,6 +302,8 @@ def resident_leg(hip, n=256, reps=50, particles=1 << 20):
     synchronisations (launch gaps included), so it bounds the kernel time from above.  Algorithmic
     bytes per cell: thermal 7 reads + 7 writes + 3 velocity reads = 68 B; Boussinesq 60 B (the
     temperature plane's refresh, 7 reads + 1 write, then T read + acceleration read/write); advection fill read + write + 3 velocities = 20 B;
+    interface geometry (normals: fill read + 3 writes; curvature: 3 reads + 1 write) 32 B; surface
+    tension (geometry, then fill and curvature reads + acceleration read/write) 64 B;
     Q-criterion 3 velocities + 1 write = 16 B; per particle 8 state reads + 6 writes = 56 B plus
     the 8-cell velocity gather."""
     flow = hip.EsotericPullHipSolver("d3q19", n, n, n, 0.6)
@@ -335,6 +337,8 @@ def resident_leg(hip, n=256, reps=50, particles=1 << 20):
             ("thermal_step", lambda: thermal.step(macro, 1), 68 * cells),
             ("boussinesq", lambda: thermal.add_boussinesq_acceleration(force, 300.0, 1e-4, (0, -1, 0)), 60 * cells),
             ("fill_advect", lambda: surface.advect(macro, 0.1), 20 * cells),
+            ("interface_geometry", lambda: surface.refresh_geometry(), 32 * cells),
+            ("surface_tension", lambda: surface.add_surface_tension_acceleration(force, 0.072, 1.0), 64 * cells),
             ("q_criterion", lambda: diag.q_criterion(macro), 16 * cells),
             ("particles_one_way", lambda: parts.advance(macro, force, 0.05, False), 56 * particles)):
         ms = timed(fn, reps)

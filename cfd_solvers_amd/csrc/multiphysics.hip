@@ -39,38 +39,59 @@ void check(hipError_t e, const char* what) {
 
 struct Dims {
     uint32_t nx, ny, nz, cells;
-    uint32_t remap;  // 1: stencil kernels give each of the 8 XCDs one contiguous slab of blocks
 };
-
-// Cell of this thread in the stencil kernels.  The hardware deals consecutive workgroups
-// round-robin to the 8 XCDs; with remap, XCD k takes blocks [k B/8, (k+1) B/8) so that a cell's
-// y and z neighbours are read through the same L2.  On by default for the Q-criterion only
-// (measured faster there, slower for the thermal step and the fill advection);
-// CFD_LBM_XCD_REMAP=1 switches it on for every stencil kernel.
-__device__ __forceinline__ uint32_t stencil_cell(const Dims& d) {
-    uint32_t b = blockIdx.x;
-    if (d.remap) b = (b & 7U) * (gridDim.x >> 3) + (b >> 3);
-    return b * blockDim.x + threadIdx.x;
-}
 
 constexpr int kBlock = 256;
 constexpr int kReduceBlocks = 1024;
 
-__device__ __forceinline__ uint32_t up(uint32_t v, uint32_t n) { return v + 1U == n ? 0U : v + 1U; }
-__device__ __forceinline__ uint32_t down(uint32_t v, uint32_t n) { return v == 0U ? n - 1U : v - 1U; }
 __device__ __forceinline__ uint32_t at(uint32_t x, uint32_t y, uint32_t z, const Dims& d) {
     return (z * d.ny + y) * d.nx + x;
 }
 
-// Periodic neighbours of cell n along +-x, +-y, +-z.
+// A cell of the stencil kernels and its periodic neighbours along +-x, +-y, +-z (linear indices).
 struct Around {
-    uint32_t xp, xm, yp, ym, zp, zm;
+    uint32_t n, xp, xm, yp, ym, zp, zm;
 };
-__device__ __forceinline__ Around around(uint32_t n, const Dims& d) {
-    const uint32_t x = n % d.nx, y = (n / d.nx) % d.ny, z = n / (d.nx * d.ny);
-    return {at(up(x, d.nx), y, z, d), at(down(x, d.nx), y, z, d), at(x, up(y, d.ny), z, d),
-            at(x, down(y, d.ny), z, d), at(x, y, up(z, d.nz), d), at(x, y, down(z, d.nz), d)};
+
+// The stencil kernels run on a 3-D launch shape (stencil_shape): x and y from the block and thread
+// indices, one z plane per grid layer, so the coordinates need no divisions.  Each thread takes
+// Rows rows, blockDim.y apart, so that a workgroup covers Rows * blockDim.y consecutive rows and
+// reads most y neighbours through its own XCD's L2 (round-robin dispatch deals consecutive
+// workgroups to the 8 XCDs).  The wrapped neighbour offsets are formed in modular unsigned
+// arithmetic; every result lies in [0, cells).
+__device__ __forceinline__ Around around(const Dims& d, uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t plane = d.nx * d.ny;
+    const uint32_t n = z * plane + y * d.nx + x;
+    Around a;
+    a.n = n;
+    a.xp = x + 1U == d.nx ? n + 1U - d.nx : n + 1U;
+    a.xm = x == 0U ? n + d.nx - 1U : n - 1U;
+    a.yp = y + 1U == d.ny ? n + d.nx - plane : n + d.nx;
+    a.ym = y == 0U ? n + plane - d.nx : n - d.nx;
+    a.zp = z + 1U == d.nz ? n + plane - d.cells : n + plane;
+    a.zm = z == 0U ? n + d.cells - plane : n - plane;
+    return a;
 }
+template <uint32_t Rows, class F> __device__ __forceinline__ void for_stencil_cells(const Dims& d, F&& f) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t y0 = blockIdx.y * Rows * blockDim.y + threadIdx.y;
+    if (x >= d.nx) return;
+#pragma unroll
+    for (uint32_t k = 0; k < Rows; ++k) {
+        const uint32_t y = y0 + k * blockDim.y;
+        if (y < d.ny) f(around(d, x, y, blockIdx.z));
+    }
+}
+
+// Rows per thread of each stencil kernel, the fastest of 1, 2, 4 and 8 at 256^3 (kernel times
+// with 1 / 2 / 4 / 8 rows: thermal step 198 / 207 / 215 / 238 us, fill advection 88 / 80 / 79 /
+// 88 us, interface geometry 125 / 124 / 120 / 134 us, Q-criterion 89 / 70 / 60 / 60 us).  The
+// D3Q7 step streams 14 planes of its own and gains nothing from sharing rows.
+constexpr uint32_t kRowsThermal = 1;
+constexpr uint32_t kRowsAdvect = 4;
+constexpr uint32_t kRowsGeometry = 4;
+constexpr uint32_t kRowsTension = 2;
+constexpr uint32_t kRowsQ = 4;
 
 // ------------------------------------------------------------------ D3Q7 thermal lattice
 // Directions 0, +x, -x, +y, -y, +z, -z; weights 1/4 and 1/8 (resident_multiphysics_sycl.hpp:188-191).
@@ -92,24 +113,24 @@ __global__ void __launch_bounds__(kBlock) thermal_fill(float* g, const float* fi
 __global__ void __launch_bounds__(kBlock) thermal_step(const float* __restrict__ in, float* __restrict__ out,
                                                        const float* __restrict__ ux, const float* __restrict__ uy,
                                                        const float* __restrict__ uz, Dims d, float omega) {
-    const uint32_t n = stencil_cell(d);
-    if (n >= d.cells) return;
-    const Around a = around(n, d);
-    const uint32_t src[7] = {n, a.xm, a.xp, a.ym, a.yp, a.zm, a.zp};
-    float local[7];
-    float temperature = 0.0F;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) {
-        local[q] = in[static_cast<size_t>(q) * d.cells + src[q]];
-        temperature += local[q];
-    }
-    const float u = ux[n], v = uy[n], w = uz[n];
-#pragma unroll
-    for (int q = 0; q < 7; ++q) {
-        const float cu = q7_cx(q) * u + q7_cy(q) * v + q7_cz(q) * w;
-        const float geq = q7_weight(q) * temperature * (1.0F + 4.0F * cu);
-        out[static_cast<size_t>(q) * d.cells + n] = local[q] - omega * (local[q] - geq);
-    }
+    for_stencil_cells<kRowsThermal>(d, [&](const Around& a) {
+        const uint32_t n = a.n;
+        const uint32_t src[7] = {n, a.xm, a.xp, a.ym, a.yp, a.zm, a.zp};
+        float local[7];
+        float temperature = 0.0F;
+    #pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            local[q] = in[static_cast<size_t>(q) * d.cells + src[q]];
+            temperature += local[q];
+        }
+        const float u = ux[n], v = uy[n], w = uz[n];
+    #pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            const float cu = q7_cx(q) * u + q7_cy(q) * v + q7_cz(q) * w;
+            const float geq = q7_weight(q) * temperature * (1.0F + 4.0F * cu);
+            out[static_cast<size_t>(q) * d.cells + n] = local[q] - omega * (local[q] - geq);
+        }
+    });
 }
 
 __global__ void __launch_bounds__(kBlock) thermal_scalar(const float* __restrict__ g, float* __restrict__ s, Dims d) {
@@ -179,22 +200,25 @@ __global__ void __launch_bounds__(kBlock) vof_fill(float* phi, const float* fiel
 __global__ void __launch_bounds__(kBlock) vof_advect(const float* __restrict__ phi, float* __restrict__ out,
                                                      const float* __restrict__ ux, const float* __restrict__ uy,
                                                      const float* __restrict__ uz, Dims d, float dt) {
-    const uint32_t n = stencil_cell(d);
-    if (n >= d.cells) return;
-    const Around a = around(n, d);
-    const float uxp = 0.5F * (ux[n] + ux[a.xp]), uxm = 0.5F * (ux[a.xm] + ux[n]);
-    const float uyp = 0.5F * (uy[n] + uy[a.yp]), uym = 0.5F * (uy[a.ym] + uy[n]);
-    const float uzp = 0.5F * (uz[n] + uz[a.zp]), uzm = 0.5F * (uz[a.zm] + uz[n]);
-    const float p = phi[n];
-    const float fxp = uxp >= 0.0F ? uxp * p : uxp * phi[a.xp];
-    const float fxm = uxm >= 0.0F ? uxm * phi[a.xm] : uxm * p;
-    const float fyp = uyp >= 0.0F ? uyp * p : uyp * phi[a.yp];
-    const float fym = uym >= 0.0F ? uym * phi[a.ym] : uym * p;
-    const float fzp = uzp >= 0.0F ? uzp * p : uzp * phi[a.zp];
-    const float fzm = uzm >= 0.0F ? uzm * phi[a.zm] : uzm * p;
-    float value = p - dt * ((fxp - fxm) + (fyp - fym) + (fzp - fzm));
-    value = value < 0.0F ? 0.0F : value > 1.0F ? 1.0F : value;
-    out[n] = value;
+    for_stencil_cells<kRowsAdvect>(d, [&](const Around& a) {
+        const uint32_t n = a.n;
+        const float uxp = 0.5F * (ux[n] + ux[a.xp]), uxm = 0.5F * (ux[a.xm] + ux[n]);
+        const float uyp = 0.5F * (uy[n] + uy[a.yp]), uym = 0.5F * (uy[a.ym] + uy[n]);
+        const float uzp = 0.5F * (uz[n] + uz[a.zp]), uzm = 0.5F * (uz[a.zm] + uz[n]);
+        // All six neighbour fills are loaded unconditionally (one memory round trip), then selected.
+        const float p = phi[n];
+        const float pxp = phi[a.xp], pxm = phi[a.xm], pyp = phi[a.yp], pym = phi[a.ym], pzp = phi[a.zp],
+                    pzm = phi[a.zm];
+        const float fxp = uxp >= 0.0F ? uxp * p : uxp * pxp;
+        const float fxm = uxm >= 0.0F ? uxm * pxm : uxm * p;
+        const float fyp = uyp >= 0.0F ? uyp * p : uyp * pyp;
+        const float fym = uym >= 0.0F ? uym * pym : uym * p;
+        const float fzp = uzp >= 0.0F ? uzp * p : uzp * pzp;
+        const float fzm = uzm >= 0.0F ? uzm * pzm : uzm * p;
+        float value = p - dt * ((fxp - fxm) + (fyp - fym) + (fzp - fzm));
+        value = value < 0.0F ? 0.0F : value > 1.0F ? 1.0F : value;
+        out[n] = value;
+    });
 }
 
 __device__ __forceinline__ void fill_gradient(const float* phi, const Around& a, float& gx, float& gy, float& gz) {
@@ -205,49 +229,51 @@ __device__ __forceinline__ void fill_gradient(const float* phi, const Around& a,
 
 // Unit interface normal grad(phi)/|grad(phi)| (zero where the gradient vanishes).
 __global__ void __launch_bounds__(kBlock) vof_normals(const float* __restrict__ phi, float* __restrict__ geo, Dims d) {
-    const uint32_t n = stencil_cell(d);
-    if (n >= d.cells) return;
-    float gx, gy, gz;
-    fill_gradient(phi, around(n, d), gx, gy, gz);
-    const float mag = sqrtf(gx * gx + gy * gy + gz * gz);
-    float* nxv = geo;
-    float* nyv = geo + d.cells;
-    float* nzv = geo + 2 * static_cast<size_t>(d.cells);
-    if (mag > 1.0e-12F) {
-        nxv[n] = gx / mag;
-        nyv[n] = gy / mag;
-        nzv[n] = gz / mag;
-    } else {
-        nxv[n] = 0.0F;
-        nyv[n] = 0.0F;
-        nzv[n] = 0.0F;
-    }
+    for_stencil_cells<kRowsGeometry>(d, [&](const Around& a) {
+        const uint32_t n = a.n;
+        float gx, gy, gz;
+        fill_gradient(phi, a, gx, gy, gz);
+        const float mag = sqrtf(gx * gx + gy * gy + gz * gz);
+        float* nxv = geo;
+        float* nyv = geo + d.cells;
+        float* nzv = geo + 2 * static_cast<size_t>(d.cells);
+        if (mag > 1.0e-12F) {
+            nxv[n] = gx / mag;
+            nyv[n] = gy / mag;
+            nzv[n] = gz / mag;
+        } else {
+            nxv[n] = 0.0F;
+            nyv[n] = 0.0F;
+            nzv[n] = 0.0F;
+        }
+    });
 }
 
 // curvature = -div(n) by central differences (resident_multiphysics_sycl.hpp:287-296).
 __global__ void __launch_bounds__(kBlock) vof_curvature(float* geo, Dims d) {
-    const uint32_t n = stencil_cell(d);
-    if (n >= d.cells) return;
-    const Around a = around(n, d);
-    const float* nxv = geo;
-    const float* nyv = geo + d.cells;
-    const float* nzv = geo + 2 * static_cast<size_t>(d.cells);
-    geo[3 * static_cast<size_t>(d.cells) + n] =
-        -0.5F * ((nxv[a.xp] - nxv[a.xm]) + (nyv[a.yp] - nyv[a.ym]) + (nzv[a.zp] - nzv[a.zm]));
+    for_stencil_cells<kRowsGeometry>(d, [&](const Around& a) {
+        const uint32_t n = a.n;
+        const float* nxv = geo;
+        const float* nyv = geo + d.cells;
+        const float* nzv = geo + 2 * static_cast<size_t>(d.cells);
+        geo[3 * static_cast<size_t>(d.cells) + n] =
+            -0.5F * ((nxv[a.xp] - nxv[a.xm]) + (nyv[a.yp] - nyv[a.ym]) + (nzv[a.zp] - nzv[a.zm]));
+    });
 }
 
 // a += (sigma kappa / rho) grad(phi)  (resident_multiphysics_sycl.hpp:299-317)
 __global__ void __launch_bounds__(kBlock) vof_tension(const float* __restrict__ phi, const float* __restrict__ curvature,
                                                       float* ax, float* ay, float* az, Dims d, float sigma,
                                                       float density) {
-    const uint32_t n = stencil_cell(d);
-    if (n >= d.cells) return;
-    float gx, gy, gz;
-    fill_gradient(phi, around(n, d), gx, gy, gz);
-    const float c = sigma * curvature[n] / density;
-    ax[n] += c * gx;
-    ay[n] += c * gy;
-    az[n] += c * gz;
+    for_stencil_cells<kRowsTension>(d, [&](const Around& a) {
+        const uint32_t n = a.n;
+        float gx, gy, gz;
+        fill_gradient(phi, a, gx, gy, gz);
+        const float c = sigma * curvature[n] / density;
+        ax[n] += c * gx;
+        ay[n] += c * gy;
+        az[n] += c * gz;
+    });
 }
 
 // ------------------------------------------------------------------ particles
@@ -330,31 +356,44 @@ __global__ void __launch_bounds__(kBlock) particles_advance(float* s, uint32_t c
 // Q = (|Omega|^2 - |S|^2) / 2 from central differences (resident_multiphysics_sycl.hpp:394-404).
 __global__ void __launch_bounds__(kBlock) q_criterion(const float* __restrict__ ux, const float* __restrict__ uy,
                                                       const float* __restrict__ uz, float* __restrict__ q, Dims d) {
-    const uint32_t n = stencil_cell(d);
-    if (n >= d.cells) return;
-    const Around a = around(n, d);
-    const uint32_t p[3] = {a.xp, a.yp, a.zp}, m[3] = {a.xm, a.ym, a.zm};
-    float g[3][3];
-#pragma unroll
-    for (int axis = 0; axis < 3; ++axis) {
-        g[0][axis] = 0.5F * (ux[p[axis]] - ux[m[axis]]);
-        g[1][axis] = 0.5F * (uy[p[axis]] - uy[m[axis]]);
-        g[2][axis] = 0.5F * (uz[p[axis]] - uz[m[axis]]);
-    }
-    float s2 = 0.0F, o2 = 0.0F;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const float sij = 0.5F * (g[i][j] + g[j][i]), oij = 0.5F * (g[i][j] - g[j][i]);
-            s2 += sij * sij;
-            o2 += oij * oij;
+    for_stencil_cells<kRowsQ>(d, [&](const Around& a) {
+        const uint32_t n = a.n;
+        const uint32_t p[3] = {a.xp, a.yp, a.zp}, m[3] = {a.xm, a.ym, a.zm};
+        float g[3][3];
+    #pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+            g[0][axis] = 0.5F * (ux[p[axis]] - ux[m[axis]]);
+            g[1][axis] = 0.5F * (uy[p[axis]] - uy[m[axis]]);
+            g[2][axis] = 0.5F * (uz[p[axis]] - uz[m[axis]]);
         }
-    q[n] = 0.5F * (o2 - s2);
+        float s2 = 0.0F, o2 = 0.0F;
+    #pragma unroll
+        for (int i = 0; i < 3; ++i)
+    #pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float sij = 0.5F * (g[i][j] + g[j][i]), oij = 0.5F * (g[i][j] - g[j][i]);
+                s2 += sij * sij;
+                o2 += oij * oij;
+            }
+        q[n] = 0.5F * (o2 - s2);
+    });
 }
 
 // ------------------------------------------------------------------ host side
 uint32_t blocks_for(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+// Launch shape of the stencil kernels: bx x by threads over x and y (bx the smallest power of two
+// from 64 to 256 that is >= nx, or 256; by = 256 / bx), rows rows per thread, one z plane per grid
+// layer.
+struct StencilShape {
+    dim3 grid, block;
+};
+StencilShape stencil_shape(const Dims& d, uint32_t rows) {
+    uint32_t bx = 64;
+    while (bx < static_cast<uint32_t>(kBlock) && bx < d.nx) bx <<= 1;
+    const uint32_t by = kBlock / bx;
+    return {dim3((d.nx + bx - 1) / bx, (d.ny + by * rows - 1) / (by * rows), d.nz), dim3(bx, by, 1)};
+}
 
 struct Stats {
     uint64_t h2d = 0, d2h = 0, d2d = 0, syncs = 0;
@@ -371,11 +410,13 @@ struct Module {
         if (flow.nx < min_extent || flow.ny < min_extent || flow.nz < min_extent)
             throw std::invalid_argument(std::string("invalid resident ") + what + " lattice");
         dims = {static_cast<uint32_t>(flow.nx), static_cast<uint32_t>(flow.ny), static_cast<uint32_t>(flow.nz),
-                static_cast<uint32_t>(flow.cells), 0U};
-        const uint64_t blocks = (flow.cells + kBlock - 1) / kBlock;
-        can_remap = blocks % 8 == 0 && blocks >= 64;
-        if (const char* v = std::getenv("CFD_LBM_XCD_REMAP")) dims.remap = (std::atoi(v) != 0 && can_remap) ? 1U : 0U;
+                static_cast<uint32_t>(flow.cells)};
+        const StencilShape widest = stencil_shape(dims, 1);
+        if (widest.grid.y > 65535U || widest.grid.z > 65535U)  // grid limits of the y and z launch dimensions
+            throw std::invalid_argument(std::string("resident ") + what + " lattice too tall: at most 65535 z planes and " +
+                                        std::to_string(65535U * widest.block.y) + " rows per plane");
     }
+    StencilShape shape(uint32_t rows) const { return stencil_shape(dims, rows); }
     void use() const { check(hipSetDevice(flow.device), "hipSetDevice"); }
     float* alloc(uint64_t floats) const {
         void* p = nullptr;
@@ -418,7 +459,6 @@ struct Module {
     Module(const Module&) = delete;
     Module& operator=(const Module&) = delete;
     double* partial = nullptr;
-    bool can_remap = false;  // the block count allows the XCD slab remap
 };
 
 void need_views(std::initializer_list<const void*> v, const char* what) {
@@ -493,8 +533,9 @@ struct cfd_lbm_surface : Module {
         check(hipGetLastError(), "vof_fill");
     }
     void refresh_geometry() {
-        vof_normals<<<blocks_for(dims.cells), kBlock, 0, flow.stream>>>(fill, geometry, dims);
-        vof_curvature<<<blocks_for(dims.cells), kBlock, 0, flow.stream>>>(geometry, dims);
+        const StencilShape sh = shape(kRowsGeometry);
+        vof_normals<<<sh.grid, sh.block, 0, flow.stream>>>(fill, geometry, dims);
+        vof_curvature<<<sh.grid, sh.block, 0, flow.stream>>>(geometry, dims);
         check(hipGetLastError(), "vof_geometry");
     }
 };
@@ -579,9 +620,9 @@ int cfd_lbm_thermal_step(cfd_lbm_thermal* t, const float* ux, const float* uy, c
         need_obj(t).use();
         need_views({ux, uy, uz}, "thermal step");
         const float omega = 1.0F / t->tau;
+        const StencilShape sh = t->shape(kRowsThermal);
         for (uint64_t k = 0; k < count; ++k) {
-            thermal_step<<<blocks_for(t->dims.cells), kBlock, 0, t->flow.stream>>>(t->g, t->next, ux, uy, uz, t->dims,
-                                                                                   omega);
+            thermal_step<<<sh.grid, sh.block, 0, t->flow.stream>>>(t->g, t->next, ux, uy, uz, t->dims, omega);
             check(hipGetLastError(), "thermal_step");
             std::swap(t->g, t->next);
         }
@@ -678,7 +719,8 @@ int cfd_lbm_surface_advect(cfd_lbm_surface* s, const float* ux, const float* uy,
         need_obj(s).use();
         need_views({ux, uy, uz}, "free-surface advection");
         if (!(dt > 0.0F)) throw std::invalid_argument("invalid resident free-surface advection");
-        vof_advect<<<blocks_for(s->dims.cells), kBlock, 0, s->flow.stream>>>(s->fill, s->next, ux, uy, uz, s->dims, dt);
+        const StencilShape sh = s->shape(kRowsAdvect);
+        vof_advect<<<sh.grid, sh.block, 0, s->flow.stream>>>(s->fill, s->next, ux, uy, uz, s->dims, dt);
         check(hipGetLastError(), "vof_advect");
         std::swap(s->fill, s->next);
     });
@@ -708,7 +750,8 @@ int cfd_lbm_surface_add_surface_tension(cfd_lbm_surface* s, float* ax, float* ay
         if (!(surface_tension >= 0.0F) || !(density > 0.0F))
             throw std::invalid_argument("invalid resident surface-tension coupling");
         s->refresh_geometry();
-        vof_tension<<<blocks_for(s->dims.cells), kBlock, 0, s->flow.stream>>>(
+        const StencilShape sh = s->shape(kRowsTension);
+        vof_tension<<<sh.grid, sh.block, 0, s->flow.stream>>>(
             s->fill, s->geometry + 3ull * s->dims.cells, ax, ay, az, s->dims, surface_tension, density);
         check(hipGetLastError(), "vof_tension");
     });
@@ -798,10 +841,8 @@ int cfd_lbm_q_criterion(cfd_lbm_flow_diagnostics* d, const float* ux, const floa
     return guarded(error, cap, [&] {
         need_obj(d).use();
         need_views({ux, uy, uz}, "Q-criterion");
-        // The slab remap measured 27 % faster for this kernel at 256^3 (slower for the others).
-        Dims dims = d->dims;
-        dims.remap = d->can_remap ? 1U : 0U;
-        q_criterion<<<blocks_for(dims.cells), kBlock, 0, d->flow.stream>>>(ux, uy, uz, d->q, dims);
+        const StencilShape sh = d->shape(kRowsQ);
+        q_criterion<<<sh.grid, sh.block, 0, d->flow.stream>>>(ux, uy, uz, d->q, d->dims);
         check(hipGetLastError(), "q_criterion");
         if (q) *q = d->q;
     });

@@ -163,6 +163,45 @@ def test_q_criterion_matches_oracle(hip, oracle_mod):
     flow.close()
 
 
+@pytest.mark.parametrize("shape", [(300, 37, 11), (65, 5, 3), (3, 129, 4)])
+def test_stencils_on_ragged_shapes(hip, oracle_mod, shape):
+    """The stencil kernels' 3-D launch (64-256-wide blocks, several rows per thread) on extents that
+    leave partial blocks in x and y: every kernel bit for bit against the oracle."""
+    nx, ny, nz = shape
+    report("resident_ragged", shape=shape)
+    flow = hip.EsotericPullHipSolver("d3q19", nx, ny, nz, 0.7)
+    flow.initialize_taylor_green(0.04, 0.01, 3)
+    flow.step(3)
+    macro = flow.refresh_device_macroscopic()
+    u = velocity(flow, macro)
+    r = oracle_mod.Resident(nx, ny, nz)
+    rng = np.random.default_rng(nx)
+    T = (300.0 + rng.uniform(-2, 2, flow.shape)).astype(np.float32)
+    thermal = hip.ResidentThermalD3Q7(flow, 0.02)
+    thermal.initialize_fields(T)
+    thermal.step(macro, 3)
+    tau = np.float32(0.5) + np.float32(4.0) * np.float32(0.02)
+    want_t = r.thermal_scalar(r.thermal_step(r.thermal_equilibrium(T), u, np.float32(1.0) / tau, 3))
+    assert np.array_equal(thermal.scalar(), want_t)
+    phi = rng.uniform(0, 1, flow.shape).astype(np.float32)
+    s = hip.ResidentFreeSurface(flow)
+    s.upload_fill(phi)
+    s.advect(macro, 0.5)
+    want = r.advect(phi, u, 0.5)
+    assert np.array_equal(s.fill(), want)
+    geo = s.refresh_geometry()
+    assert np.array_equal(np.stack([flow.device_download(p).reshape(flow.shape) for p in geo]), r.geometry(want))
+    force = flow.device_local_acceleration(True)
+    s.add_surface_tension_acceleration(force, 0.05, 1.1)
+    a = np.stack([flow.device_download(p).reshape(flow.shape) for p in force])
+    assert np.array_equal(a, r.surface_tension(want, np.zeros((3,) + flow.shape), 0.05, 1.1))
+    d = hip.ResidentFlowDiagnostics(flow)
+    assert np.array_equal(flow.device_download(d.q_criterion(macro)).reshape(flow.shape), r.q_criterion(u))
+    for m in (d, s, thermal):
+        m.close()
+    flow.close()
+
+
 def test_device_acceleration_drives_the_flow(hip):
     """Buoyancy written into the device acceleration planes steers the next steps exactly as the same
     field set from the host does."""
