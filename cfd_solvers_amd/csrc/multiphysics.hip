@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -292,6 +293,9 @@ __device__ __forceinline__ float periodic_coordinate(float value, uint32_t exten
 }
 __device__ __forceinline__ uint32_t wrap_cell(long long v, uint32_t extent) {
     const long long e = static_cast<long long>(extent);
+    // Corners of a wrapped position lie in [0, extent + 1]: one subtraction (the 64-bit remainder
+    // below gives the same value there and keeps any other input inside the lattice).
+    if (v >= 0 && v < 2 * e) return static_cast<uint32_t>(v >= e ? v - e : v);
     v %= e;
     if (v < 0) v += e;
     return static_cast<uint32_t>(v);
@@ -543,11 +547,17 @@ struct cfd_lbm_surface : Module {
 struct cfd_lbm_particles : Module {
     uint32_t count = 0;
     float* state = nullptr;
+    // Device slot j holds the caller's particle order[j]: upload sorts the particles by the cell
+    // they start in, so that neighbouring lanes gather neighbouring cells' velocities.  Each
+    // particle's update is independent of the others, so the order changes no particle's result.
+    std::vector<uint32_t> order;
     cfd_lbm_particles(cfd_lbm_solver* s, size_t n) : Module(s, 2, "particle") {
         if (n > 0x7FFFFFFFull) throw std::invalid_argument("too many particles");
         count = static_cast<uint32_t>(n);
         use();
         if (count) state = alloc(8ull * count);
+        order.resize(count);
+        for (uint32_t j = 0; j < count; ++j) order[j] = j;
     }
     ~cfd_lbm_particles() {
         (void)hipStreamSynchronize(flow.stream);
@@ -783,13 +793,30 @@ int cfd_lbm_particles_upload(cfd_lbm_particles* p, const double* particles, size
         if (!count) return;
         if (!particles) throw std::invalid_argument("particles_upload needs count x 8 values");
         p->use();
-        std::vector<float> host(8ull * count);
+        std::vector<uint64_t> key(count);
         for (size_t i = 0; i < count; ++i) {
             const double* q = particles + 8 * i;
             if (!(q[6] > 0.0) || !(q[7] > 0.0)) throw std::invalid_argument("particle mass and response time must be positive");
             for (int k = 0; k < 6; ++k)
                 if (!std::isfinite(q[k])) throw std::invalid_argument("particle position and velocity must be finite");
-            for (int k = 0; k < 8; ++k) host[k * count + i] = static_cast<float>(q[k]);
+            auto cell = [](double v, uint32_t extent) {
+                const double e = static_cast<double>(extent);
+                double w = std::fmod(v, e);
+                if (w < 0.0) w += e;
+                const uint64_t c = static_cast<uint64_t>(w);
+                return c < extent ? c : uint64_t{0};
+            };
+            const uint64_t n = (cell(q[2], p->dims.nz) * p->dims.ny + cell(q[1], p->dims.ny)) * p->dims.nx +
+                               cell(q[0], p->dims.nx);
+            key[i] = (n << 32) | i;  // cell, then the caller's order: a deterministic permutation
+        }
+        std::sort(key.begin(), key.end());
+        p->order.resize(count);
+        std::vector<float> host(8ull * count);
+        for (size_t j = 0; j < count; ++j) {
+            const uint32_t i = static_cast<uint32_t>(key[j] & 0xFFFFFFFFull);
+            p->order[j] = i;
+            for (int k = 0; k < 8; ++k) host[k * count + j] = static_cast<float>(particles[8 * static_cast<size_t>(i) + k]);
         }
         p->h2d(p->state, host.data(), host.size() * sizeof(float));
         p->sync();
@@ -804,8 +831,9 @@ int cfd_lbm_particles_download(cfd_lbm_particles* p, double* particles, size_t c
         std::vector<float> host(8ull * count);
         p->d2h(host.data(), p->state, host.size() * sizeof(float));
         p->sync();
-        for (size_t i = 0; i < count; ++i)
-            for (int k = 0; k < 8; ++k) particles[8 * i + k] = static_cast<double>(host[k * count + i]);
+        for (size_t j = 0; j < count; ++j)
+            for (int k = 0; k < 8; ++k)
+                particles[8 * static_cast<size_t>(p->order[j]) + k] = static_cast<double>(host[k * count + j]);
     });
 }
 int cfd_lbm_particles_advance(cfd_lbm_particles* p, const float* ux, const float* uy, const float* uz, float* ax,
