@@ -332,7 +332,8 @@ def resident_leg(hip, n=256, reps=50, particles=1 << 20):
         flow.wait()
         return (time.perf_counter() - t0) * 1e3 / count
 
-    out = {"workload": f"D3Q19 {n}^3 Taylor-Green flow, FP32 views; {particles} particles", "cells": cells}
+    out = {"workload": f"D3Q19 {n}^3 Taylor-Green flow, FP32 views; {particles} particles", "cells": cells,
+           "particles": particles}
     for name, fn, nbytes in (
             ("thermal_step", lambda: thermal.step(macro, 1), 68 * cells),
             ("boussinesq", lambda: thermal.add_boussinesq_acceleration(force, 300.0, 1e-4, (0, -1, 0)), 60 * cells),
@@ -386,7 +387,7 @@ def inplace_probe(n=256, repeats=20):
     return out
 
 
-def cpu_baseline(target_seconds=15.0, voxelize_mesh=None):
+def cpu_baseline(target_seconds=15.0, voxelize_mesh=None, resident=False):
     """The reference's EsotericPullSolver<D3Q19> (oracle/_ref, built from the reference sources
     like its Release build) on the host cores: 256^3 periodic Taylor-Green, cfd-bench protocol
     (apps/cfd-bench.cpp:174-191) on a bounded number of steps.  Falls back to the CPU
@@ -436,6 +437,41 @@ def cpu_baseline(target_seconds=15.0, voxelize_mesh=None):
         out["voxelize"] = {"value": vn ** 3 / vdt, "unit": "cells/s", "cores": 1, "kind": "reference",
                            "sample": f"{vn}^3 cells in [0,1]^3, {len(voxelize_mesh)}-triangle sphere, {vdt:.1f} s"}
         log(f"cpu baseline voxelize (reference): {vn ** 3 / vdt:.0f} cells/s on 1 thread ({vdt:.1f} s)")
+    if resident:
+        out["resident"] = resident_cpu_sample(oracle)
+    return out
+
+
+def resident_cpu_sample(oracle, n=160, particles=1 << 18, repeats=3):
+    """The resident modules' CPU restatement (oracle/, one thread; the reference's modules are
+    SYCL-only) on an n^3 periodic lattice with the resident leg's kind of fields: the fastest of
+    `repeats` calls of each module, inputs converted beforehand.  Rates in cells/s (particles/s for
+    the push)."""
+    rng = np.random.default_rng(0)
+    r = oracle.Resident(n, n, n)
+    shape = (n, n, n)
+    u = [np.ascontiguousarray(rng.uniform(-0.05, 0.05, shape), np.float32).reshape(-1) for _ in range(3)]
+    g = r.thermal_equilibrium(np.full(shape, 300.0, np.float32))
+    phi = np.ascontiguousarray(r.half_space(n / 2 + 0.5)).reshape(-1)
+    zero = [np.zeros(n ** 3, np.float32) for _ in range(3)]
+    pts = np.column_stack([rng.uniform(0, n, (particles, 3)), np.zeros((particles, 3)), np.ones(particles),
+                           np.full(particles, 0.5)])
+    cases = (("thermal_step", lambda: r.thermal_step(g, u, 1.0 / 0.58, 1), n ** 3),
+             ("fill_advect", lambda: r.advect(phi, u, 0.1), n ** 3),
+             ("interface_geometry", lambda: r.geometry(phi), n ** 3),
+             ("surface_tension", lambda: r.surface_tension(phi, zero, 0.072, 1.0), n ** 3),
+             ("q_criterion", lambda: r.q_criterion(u), n ** 3),
+             ("particles_one_way", lambda: r.particles_advance(pts, u, zero, 0.05, False), particles))
+    out = {"kind": "port", "cores": 1,
+           "sample": f"{n}^3 periodic lattice, {particles} particles, fastest of {repeats} calls per module, one thread"}
+    for name, fn, units in cases:
+        dt = float("inf")
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            fn()
+            dt = min(dt, time.perf_counter() - t0)
+        out[name] = {"value": units / dt, "unit": "particles/s" if name.startswith("particles") else "cells/s"}
+        log(f"cpu baseline resident {name} (port): {units / dt:.3g} {out[name]['unit']} ({dt:.2f} s)")
     return out
 
 
@@ -546,11 +582,20 @@ def main():
             from cfd_solvers_amd import geometry as G
 
             mesh = G.sphere_surface((.5, .5, .5), .37, 64, 128)
-        result["cpu_baseline"] = cpu_baseline(voxelize_mesh=mesh)
+        result["cpu_baseline"] = cpu_baseline(voxelize_mesh=mesh, resident="resident" in legs)
         vox = result["cpu_baseline"].pop("voxelize", None)
         if vox is not None:
             legs["voxelize"]["cpu_reference"] = vox
             legs["voxelize"]["gpu_over_cpu_reference"] = legs["voxelize"]["gcells_per_s"] * 1e9 / vox["value"]
+        res = result["cpu_baseline"].pop("resident", None)
+        if res is not None:
+            legs["resident"]["cpu_port"] = res
+            cells = legs["resident"]["cells"]
+            for name, v in res.items():
+                if isinstance(v, dict) and name in legs["resident"]:
+                    units = legs["resident"]["particles"] if name.startswith("particles") else cells
+                    gpu_rate = units / (legs["resident"][name]["ms"] * 1e-3)
+                    legs["resident"][name]["gpu_over_cpu_port"] = gpu_rate / v["value"]
     print(json.dumps(result), flush=True)
 
 

@@ -1292,7 +1292,6 @@ inline void thermal_fill(float* g, const float* t, float value, size_t cells) {
 inline void thermal_step(const float* in, float* out, const float* ux, const float* uy, const float* uz, Grid g,
                          float omega) {
     const size_t cells = g.cells();
-#pragma omp parallel for
     for (long long nn = 0; nn < static_cast<long long>(cells); ++nn) {
         const size_t n = static_cast<size_t>(nn);
         size_t x, y, z;
