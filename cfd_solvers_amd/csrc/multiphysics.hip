@@ -495,14 +495,23 @@ struct cfd_lbm_thermal : Module {
         if (!(diffusivity > 0.0F) || !std::isfinite(diffusivity)) throw std::invalid_argument("invalid resident thermal lattice");
         tau = 0.5F + 4.0F * diffusivity;  // D = c_s^2 (tau - 1/2) with c_s^2 = 1/4
         use();
-        g = alloc(7ull * dims.cells);
-        next = alloc(7ull * dims.cells);
-        fill(nullptr, 0.0F);
+        try {
+            g = alloc(7ull * dims.cells);
+            next = alloc(7ull * dims.cells);
+            fill(nullptr, 0.0F);
+        } catch (...) {
+            release();  // a constructor that throws runs no destructor
+            throw;
+        }
     }
-    ~cfd_lbm_thermal() {
+    ~cfd_lbm_thermal() { release(); }
+    void release() {
         (void)hipStreamSynchronize(flow.stream);
-        for (float* p : {g, next, scalar})
-            if (p) (void)hipFree(p);
+        for (float** p : {&g, &next, &scalar})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
     }
     void fill(const float* device_field, float value) {
         thermal_fill<<<blocks_for(dims.cells), kBlock, 0, flow.stream>>>(g, device_field, value, dims);
@@ -522,15 +531,24 @@ struct cfd_lbm_surface : Module {
     float* geometry = nullptr;  // normal x, y, z and curvature planes
     explicit cfd_lbm_surface(cfd_lbm_solver* s) : Module(s, 2, "free-surface") {
         use();
-        fill = alloc(dims.cells);
-        next = alloc(dims.cells);
-        geometry = alloc(4ull * dims.cells);
-        init(nullptr, 0.0F, false, 0.0F);
+        try {
+            fill = alloc(dims.cells);
+            next = alloc(dims.cells);
+            geometry = alloc(4ull * dims.cells);
+            init(nullptr, 0.0F, false, 0.0F);
+        } catch (...) {
+            release();  // a constructor that throws runs no destructor
+            throw;
+        }
     }
-    ~cfd_lbm_surface() {
+    ~cfd_lbm_surface() { release(); }
+    void release() {
         (void)hipStreamSynchronize(flow.stream);
-        for (float* p : {fill, next, geometry})
-            if (p) (void)hipFree(p);
+        for (float** p : {&fill, &next, &geometry})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
     }
     void init(const float* device_field, float value, bool half, float x) {
         vof_fill<<<blocks_for(dims.cells), kBlock, 0, flow.stream>>>(fill, device_field, value, half ? 1 : 0, x, dims);
