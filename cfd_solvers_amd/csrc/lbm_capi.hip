@@ -584,11 +584,13 @@ struct cfd_lbm_solver {
                 std::vector<unsigned char> host(fb);
                 in.read(reinterpret_cast<char*>(host.data()), static_cast<std::streamsize>(fb));
                 const auto wb = get<uint64_t>(in);
+                if (!in) throw std::runtime_error("checkpoint read failed");
+                // Checked before the buffer is sized from the file.
+                if (wb != (solid_rho ? solid_rho.bytes : 0))
+                    throw std::runtime_error("checkpoint wall state does not match the solver's flags");
                 std::vector<unsigned char> walls(wb);
                 in.read(reinterpret_cast<char*>(walls.data()), static_cast<std::streamsize>(wb));
                 if (!in) throw std::runtime_error("checkpoint read failed");
-                if (wb != (solid_rho ? solid_rho.bytes : 0))
-                    throw std::runtime_error("checkpoint wall state does not match the solver's flags");
                 t = step;
                 h2d(f.ptr, host.data(), fb);
                 if (wb) h2d(solid_rho.ptr, walls.data(), wb);

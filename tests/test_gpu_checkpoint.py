@@ -92,7 +92,13 @@ def test_checkpoint_errors(hip, tmp_path):
     with pytest.raises(RuntimeError, match="decomposition"):
         t.load_checkpoint(A)
     u, _ = cavity(hip, "fp32")
+    walled = tmp_path / "walled.bin"
+    u.save_checkpoint(walled)  # raw device state with the wall densities
     u.close()
+    plain = hip.EsotericPullHipSolver("d3q19", 20, 18, 16, 0.62)
+    with pytest.raises(RuntimeError, match="wall state does not match"):
+        plain.load_checkpoint(walled)
+    plain.close()
     w = hip.EsotericPullHipSolver("d3q19", 12, 10, 8, float(np.float32(0.71)))
     w.set_flags(np.where(np.arange(960).reshape(8, 10, 12) % 97 == 0, 0x80, 0).astype(np.uint8))
     with pytest.raises(RuntimeError, match="periodic lattice only"):
