@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -184,9 +185,12 @@ class EsotericPullHipSolver:
                       STORAGE[storage], C.byref(h))
         self._h = h
         self.cells = nx * ny * nz
+        self._modules = weakref.WeakSet()  # resident modules on this solver's stream
 
     def close(self):
         if getattr(self, "_h", None):
+            for m in list(getattr(self, "_modules", ())):
+                m.close()  # they run on this solver's stream: destroy them first
             self.lib.call("cfd_lbm_destroy", self._h)
             self._h = None
 
@@ -508,6 +512,7 @@ class _Module:
         h = C.c_void_p()
         self.lib.call(f"cfd_lbm_{self._prefix}_create", flow._h, *args, C.byref(h))
         self._h = h
+        flow._modules.add(self)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -227,6 +227,21 @@ def test_device_acceleration_drives_the_flow(hip):
     b.close()
 
 
+def test_flow_close_closes_its_modules(hip):
+    """The modules run on the flow solver's stream: closing the flow first closes them, and a closed
+    module reports a null handle instead of touching the destroyed stream."""
+    flow = hip.EsotericPullHipSolver("d3q19", 8, 8, 8, 0.7)
+    thermal = hip.ResidentThermalD3Q7(flow, 0.01)
+    surface = hip.ResidentFreeSurface(flow)
+    flow.close()
+    assert thermal._h is None and surface._h is None
+    with pytest.raises(RuntimeError, match="null module handle"):
+        thermal.total_scalar()
+    surface.close()  # closing again is harmless
+    with pytest.raises(RuntimeError, match="null solver handle"):
+        hip.ResidentParticles(flow, 1)
+
+
 def test_module_errors(hip):
     f64 = hip.EsotericPullHipSolver("d3q19", 8, 8, 8, 0.7, storage="fp64")
     with pytest.raises(RuntimeError, match="FP32 device views"):
