@@ -4,7 +4,7 @@
 
 Every HIP source is compiled with ``--offload-arch=gfx950 -ffp-contract=off`` (the collision
 arithmetic then matches the reference's FP32 arithmetic bit for bit) and with packed-FP32
-instructions disabled.  The apps (cfd-bench, cfd-solve) are plain C++ linked to the library.
+instructions disabled.  The apps (cfd-bench, cfd-solve, cfd-distributed) are plain C++ linked to the library.
 """
 from __future__ import annotations
 
@@ -20,7 +20,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libcfd_lbm_hip.so"
 PROBE = PKG / "libhbm_probe.so"
-APPS = {"cfd-bench": PKG / "apps" / "cfd_bench.cpp", "cfd-solve": PKG / "apps" / "cfd_solve.cpp"}
+APPS = {"cfd-bench": PKG / "apps" / "cfd_bench.cpp", "cfd-solve": PKG / "apps" / "cfd_solve.cpp",
+        "cfd-distributed": PKG / "apps" / "cfd_distributed.cpp"}
 
 HIP_SOURCES = ["lbm_kernels.hip", "lbm_adjoint.hip", "voxelize.hip", "multiphysics.hip", "lbm_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

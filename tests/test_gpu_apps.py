@@ -1,6 +1,7 @@
 """The drop-in command-line surface on the GPU: cfd-bench with the reference's flags and output
 schema (apps/cfd-bench.cpp:47-191) and --backend hip, the -hip cases of cfd-solve
-(apps/cfd-solve.cpp:59-73), and the package's smoke entry point."""
+(apps/cfd-solve.cpp:59-73), cfd-distributed --backend hip on one rank with the reference's
+checkpoint/restart files (apps/cfd-distributed.cpp:142-174), and the package's smoke entry point."""
 import json
 import subprocess
 import sys
@@ -15,6 +16,9 @@ pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parents[1]
 BENCH = ROOT / "cfd_solvers_amd" / "cfd-bench"
 SOLVE = ROOT / "cfd_solvers_amd" / "cfd-solve"
+DISTRIBUTED = ROOT / "cfd_solvers_amd" / "cfd-distributed"
+GOLDEN = ROOT / "tests" / "golden"
+CHK_BYTES = 116 + 4 * 19 * 12 * 10 * 8  # CFDCHK3 header + body of the 12x10x8 D3Q19 fixtures
 
 
 def run(args, timeout=120):
@@ -68,6 +72,46 @@ def test_cfd_solve_cases(case):
         assert float(fields["relative_L2"]) < 0.02
     if case in ("lbm-d3q19-hip", "lbm-d3q27-hip", "lbm-d2q9-inplace-hip"):
         assert float(fields["mass_error_rel"]) < 1e-5 and float(fields["MLUPS"]) > 0
+
+
+def test_cfd_distributed_restarts_from_the_reference_files(tmp_path):
+    """The reference's DistributedPullBlock wrote CFDCHK3 files at steps 5 and 8 of its restart test
+    (12x10x8, tau 0.71, Taylor-Green 0.02; tests/test_main.cpp:225-266).  Restarting the one-rank HIP
+    run from the step-5 file and stepping 3 times writes the step-8 file byte for byte (header and
+    body; a trailer with the parity follows)."""
+    report("cfd_distributed_restart")
+    (tmp_path / "ref.rank0.bin").write_bytes((GOLDEN / "cfdchk3_d3q19_12x10x8_step5.bin").read_bytes())
+    out = run([DISTRIBUTED, "--backend", "hip", "--nx", 12, "--ny", 10, "--nz", 8, "--tau", "0.71", "--steps", 3,
+               "--restart", tmp_path / "ref", "--checkpoint", tmp_path / "out"])
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().splitlines()
+    fields = dict(kv.split("=", 1) for kv in lines[-2].split() if "=" in kv)
+    assert fields["backend"] == "hip-esoteric" and fields["ranks"] == "1" and fields["process_grid"] == "1x1x1"
+    assert fields["global"] == "12x10x8" and fields["steps"] == "3" and fields["halo_bytes_per_step"] == "0"
+    assert lines[-1].startswith("rank=0 local_rank=0/1 ") and lines[-1].endswith("brick=12x10x8 begin=0,0,0")
+    saved = (tmp_path / "out.rank0.bin").read_bytes()
+    want = (GOLDEN / "cfdchk3_d3q19_12x10x8_step8.bin").read_bytes()
+    assert len(want) == CHK_BYTES and saved[:CHK_BYTES] == want
+    assert abs(float(fields["mass"]) - 960.0) < 1e-3  # rho = 1 on 960 cells, conserved
+
+
+def test_cfd_distributed_fresh_run_csv_and_rejections(tmp_path):
+    out = run([DISTRIBUTED, "--backend", "hip", "--lattice", "d3q27", "--nx", 16, "--ny", 12, "--nz", 10,
+               "--steps", 4, "--csv", "--checkpoint", tmp_path / "c"])
+    assert out.returncode == 0, out.stderr
+    row = out.stdout.strip().splitlines()[-1].split(",")
+    assert len(row) == 16 and row[:10] == ["hip-esoteric", "d3q27", "1", "1", "1", "1", "16", "12", "10", "4"]
+    assert float(row[10]) > 0 and float(row[11]) > 0 and abs(float(row[15]) - 1920.0) < 1e-3
+    chk = (tmp_path / "c.rank0.bin").read_bytes()
+    assert chk[:8] == b"CFDCHK3\0" and len(chk) > 116 + 4 * 27 * 1920
+    again = run([DISTRIBUTED, "--backend", "hip", "--lattice", "d3q27", "--nx", 16, "--ny", 12, "--nz", 10,
+                 "--steps", 1, "--csv", "--restart", tmp_path / "c"])
+    assert again.returncode == 0, again.stderr
+    assert run([DISTRIBUTED, "--backend", "cpu"]).returncode == 2  # the MPI CPU path stays the reference's
+    assert run([DISTRIBUTED, "--backend", "hip", "--lattice", "d2q9"]).returncode == 1
+    bad = run([DISTRIBUTED, "--backend", "hip", "--nx", 12, "--ny", 10, "--nz", 9, "--tau", "0.71", "--steps", 1,
+               "--restart", GOLDEN / "cfdchk3_d3q19_12x10x8_step5"])
+    assert bad.returncode == 1 and "cfd-distributed:" in bad.stderr  # no such rank file
 
 
 def test_smoke_entry_point():
