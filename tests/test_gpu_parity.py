@@ -367,6 +367,51 @@ def test_shifted_boundaries_parity(hip, oracle_mod, storage):
     assert_fields(gpu_fields(g), o.macroscopic(), mask=flags != 0x81)
 
 
+RAGGED = [(0, 19, "fp32", (67, 9, 7)), (1, 27, "fp32", (130, 5, 6)), (2, 19, "fp16s", (300, 4, 3)),
+          (3, 19, "fp32s", (3, 40, 9)), (4, 27, "fp64", (65, 10, 5)), (5, 19, "fp64", (96, 7, 2)),
+          (6, 27, "fp16s", (129, 6, 4)), (7, 19, "fp32", (257, 3, 3))]
+
+
+@pytest.mark.parametrize("seed,q,storage,n", RAGGED)
+def test_random_walls_on_ragged_lattices(hip, oracle_mod, seed, q, storage, n):
+    """Seeded random geometry on extents that leave partial tiles and 64-cell segments (nx above
+    256, not a multiple of 64, 3 cells wide; nz = 2): solid boxes, scattered wall cells and a moving
+    y = ny-1 plane with one of three wall velocities, uniform forcing, Taylor-Green start with noise.
+    Every storage and both 3-D lattices, bit for bit against the oracle in the fluid cells after an
+    odd number of steps (both parities)."""
+    report("ragged", seed=seed, q=q, storage=storage, shape=n)
+    rng = np.random.default_rng(100 + seed)
+    nx, ny, nz = n
+    f = np.zeros((nz, ny, nx), np.uint8)
+    for _ in range(int(rng.integers(1, 4))):
+        lo = [int(rng.integers(0, m)) for m in (nz, ny, nx)]
+        hi = [a + int(rng.integers(1, max(2, m // 3 + 1))) for a, m in zip(lo, (nz, ny, nx))]
+        f[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = 0x80
+    f[rng.random(f.shape) < 0.03] = 0x80
+    if seed % 4 != 3:
+        f[:, -1, :] = 0x80 | int(rng.integers(1, 4))
+    flags = f.reshape(-1)
+    # The C ABI takes float wall velocities; the oracle gets the same float-rounded values.
+    velocities = {k: tuple(float(np.float32(v)) for v in rng.uniform(-0.05, 0.05, 3)) for k in (1, 2, 3)}
+    g, o = make_pair(hip, oracle_mod, q, n, 0.62, (2e-6, -1e-6, 5e-7), storage=storage)
+    for s in (g, o):
+        s.set_flags(flags)
+        for k, u in velocities.items():
+            s.set_wall_velocity(k, u)
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    g.initialize_taylor_green(0.03, 1e-4, seed)
+    o.init_taylor_green(f32(0.03), f32(1e-4), seed)
+    g.step(9)
+    o.step(9)
+    fluid = flags == 0
+    assert fluid.sum() > flags.size // 4
+    assert_fields(gpu_fields(g, f64=storage == "fp64"), o.macroscopic(), mask=fluid)
+    if storage != "fp64":
+        gl = g.logical_populations().astype(np.float64)
+        assert np.array_equal(gl[:, fluid], o.logical().reshape(gl.shape)[:, fluid])
+    g.close()
+
+
 @pytest.mark.parametrize("q", [19, 27])
 def test_fp64_storage_parity(hip, oracle_mod, q):
     report("fp64", q=q)
