@@ -74,6 +74,25 @@ def test_adjoint_matches_logical_space_oracle(hip, oracle_mod, storage, tol):
     assert np.all(grad.reshape(-1)[flags != 0] == 0.0)
 
 
+@pytest.mark.parametrize("n,storage,tol", [((67, 6, 5), "fp64", 1e-9), ((130, 5, 4), "fp32", 2e-4),
+                                           ((3 + 64, 9, 3), "fp32", 2e-4)])
+def test_adjoint_on_ragged_ducts(hip, oracle_mod, n, storage, tol):
+    """The adjoint kernels on extents that leave partial tiles and 64-cell segments (one fluid row
+    in z for the last case), against the FP64 logical-space oracle."""
+    report("adjoint_ragged", shape=n, storage=storage)
+    flags = duct(*n)
+    blend = 0.05 * design(flags, seed=n[0])
+    T, W, every = 21, 6, 5
+    g = gpu_duct(hip, n, blend, storage=storage)
+    o = oracle_duct(oracle_mod, n, blend, storage="fp64")
+    J, grad, du = g.adjoint_run(T, W, every, 1, n[0] - 2)
+    Jo, grado, duo = o.adjoint_run(T, W, 1, n[0] - 2)
+    assert abs(J - Jo) <= tol * abs(Jo)
+    assert np.abs(grad - grado).max() <= tol * np.abs(grado).max()
+    assert abs(du - duo) <= tol * abs(duo)
+    assert np.abs(grado).max() > 0 and np.all(grad.reshape(-1)[flags != 0] == 0.0)
+
+
 def test_adjoint_periodic_porous_matches_oracle(hip, oracle_mod):
     """No flags: periodic lattice, uniform-random PBB, Taylor-Green start, objective planes x=2, 9."""
     report("adjoint_periodic")
