@@ -42,6 +42,7 @@ _SIGNATURES = {
     "cfd_lbm_initialize_fields": [C.c_void_p, _f32p, _f32p, _f32p, _f32p],
     "cfd_lbm_upload_populations": [C.c_void_p, _f32p],
     "cfd_lbm_download_populations": [C.c_void_p, _f32p],
+    "cfd_lbm_download_raw_slots": [C.c_void_p, _f32p],
     "cfd_lbm_set_flags": [C.c_void_p, C.c_void_p],
     "cfd_lbm_set_wall_velocity": [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float],
     "cfd_lbm_set_porosity": [C.c_void_p, C.c_void_p],
@@ -222,6 +223,13 @@ class EsotericPullHipSolver:
     def logical_populations(self):
         out = np.empty(self.q * self.cells, dtype=np.float32)
         self.lib.call("cfd_lbm_download_populations", self._h, out)
+        return out.reshape(self.q, self.cells)
+
+    def raw_storage(self):
+        """The in-place slot array, shape (q, cells), in the reference's order (raw_storage,
+        esoteric_pull.hpp:276-278), each slot as a full float population."""
+        out = np.empty(self.q * self.cells, dtype=np.float32)
+        self.lib.call("cfd_lbm_download_raw_slots", self._h, out)
         return out.reshape(self.q, self.cells)
 
     # boundaries and forcing ---------------------------------------------------------

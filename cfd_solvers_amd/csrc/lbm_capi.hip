@@ -393,6 +393,25 @@ struct cfd_lbm_solver {
         }
     }
 
+    // The in-place slot array as full populations in the order d * cells + n (raw_storage).
+    void download_raw_slots(float* out) {
+        if (!out) throw std::invalid_argument("download_raw_slots requires q*cells values");
+        use_device();
+        DeviceBuffer<unsigned char> buf;
+        buf.allocate(static_cast<size_t>(q) * cells * real);
+        ops->slots(state(), buf.ptr);
+        const size_t n = static_cast<size_t>(q) * cells;
+        if (real == 4) {
+            d2h(out, buf.ptr, n * 4);
+            sync();
+        } else {
+            std::vector<double> tmp(n);
+            d2h(tmp.data(), buf.ptr, n * 8);
+            sync();
+            for (size_t i = 0; i < n; ++i) out[i] = static_cast<float>(tmp[i]);
+        }
+    }
+
     void set_flags(const uint8_t* host_flags) {
         use_device();
         DeviceBuffer<unsigned char> fresh;
@@ -1185,6 +1204,10 @@ int cfd_lbm_upload_populations(cfd_lbm_solver* s, const float* logical, char* er
 
 int cfd_lbm_download_populations(cfd_lbm_solver* s, float* logical, char* error, size_t cap) {
     return guarded(error, cap, [&] { need(s).download_populations(logical); });
+}
+
+int cfd_lbm_download_raw_slots(cfd_lbm_solver* s, float* slots, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).download_raw_slots(slots); });
 }
 
 int cfd_lbm_set_flags(cfd_lbm_solver* s, const uint8_t* flags, char* error, size_t cap) {

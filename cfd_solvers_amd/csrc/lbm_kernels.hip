@@ -364,6 +364,27 @@ __global__ __launch_bounds__(256) void download_logical(const void* __restrict__
     });
 }
 
+// Every slot of the in-place array, decoded to a full population, in the order d * cells + n of
+// the reference's raw storage (esoteric_pull.hpp:276-278): plane d of cell n holds whichever
+// population the esoteric scheme left there at this parity.
+template <int Q, class T>
+__global__ __launch_bounds__(256) void download_slots(const void* __restrict__ f, const Geometry g, uint32_t cells,
+                                                      typename Storage<T>::Real* __restrict__ out) {
+    using S = Storage<T>;
+    uint32_t x, y, z;
+    if (!cell_coords(g, x, y, z)) return;
+    const uint32_t row = z * g.ny + y;
+    const uint32_t n = row * g.nx + x;
+    const uint32_t e = row_base(g, row) + x;
+    static_for<Q>([&](auto DD) {
+        constexpr int d = decltype(DD)::value;
+        // The slot of plane d holds a population of direction d or of its opposite, which has the
+        // same weight (hence the same shift).
+        out[static_cast<uint64_t>(d) * cells + n] =
+            full_population<S, Q, d>(S::template decode<Q, d>(load_at<T>(plane<T>(f, g, d), e * sizeof(T))));
+    });
+}
+
 // ------------------------------------------------------------------ FP64 reductions
 // Per block: mass (sum of per-cell population sums; wall cells contribute their frozen density),
 // kinetic energy 0.5 rho |u|^2 and max |u| over non-wall cells.  Fixed tree order: deterministic.
@@ -589,6 +610,11 @@ template <int Q, class T> struct Ops final : KernelOps {
                                *static_cast<const Physics<Real>*>(st.physics), static_cast<Real*>(out));
         });
         check(hipGetLastError(), "download_logical launch");
+    }
+    void slots(const LaunchState& st, void* out) const override {
+        hipLaunchKernelGGL((download_slots<Q, T>), grid_of(st.geo), block_of(st.geo), 0, st.stream, st.f, st.geo,
+                           st.cells, static_cast<Real*>(out));
+        check(hipGetLastError(), "download_slots launch");
     }
     void diagnostics(const LaunchState& st, bool odd, bool bounded, double* partial, double* out) const override {
         per_parity(st, odd, bounded, [&](auto O, auto B) {

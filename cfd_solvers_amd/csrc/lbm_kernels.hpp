@@ -35,6 +35,8 @@ struct KernelOps {
                       const void* solid_rho) const = 0;
     virtual void macroscopic(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
     virtual void download(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
+    // out: q*cells Real, slot (d, n) of the in-place array as a full population, order d*cells + n
+    virtual void slots(const LaunchState& st, void* out) const = 0;
     virtual void diagnostics(const LaunchState& st, bool odd, bool bounded, double* partial, double* out) const = 0;
     // ids: q*stride uint32 scratch; on return it holds, per slot, the identifier d*cells + n of the
     // population stored there (0xFFFFFFFF if none); load: q*cells identifiers of the slots read.

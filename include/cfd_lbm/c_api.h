@@ -83,6 +83,13 @@ CFD_LBM_API int cfd_lbm_upload_populations(cfd_lbm_solver* s, const float* logic
 CFD_LBM_API int cfd_lbm_download_populations(cfd_lbm_solver* s, float* logical, char* error,
                                              size_t error_capacity);
 
+/* raw_storage -- esoteric_pull.hpp:276-278: the in-place slot array (q*cells) in the reference's order
+ * d*cells + n, at the current parity, each slot decoded to a full float population (FP16S/FP32S
+ * shifted states add w_d back; FP64 rounds to float).  On a periodic FP32 lattice it equals the
+ * reference's raw storage after the same steps.  Lattices with flags hold this build's in-place
+ * halfway bounce-back slots. */
+CFD_LBM_API int cfd_lbm_download_raw_slots(cfd_lbm_solver* s, float* slots, char* error, size_t error_capacity);
+
 /* set_solid_mask / clear_solid_mask -- esoteric_pull.hpp:163-175, generalised to uint8 flags
  * (cells values; NULL clears).  Mask value 1 in the reference maps to CFD_LBM_WALL. */
 CFD_LBM_API int cfd_lbm_set_flags(cfd_lbm_solver* s, const uint8_t* flags, char* error, size_t error_capacity);

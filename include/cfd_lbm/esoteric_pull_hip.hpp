@@ -252,6 +252,13 @@ public:
         call(cfd_lbm_download_populations(h_, out.data(), err_, sizeof(err_)));
         return out;
     }
+    // raw_storage (esoteric_pull.hpp:276-278): the in-place slot array in the order d*cells + n,
+    // copied to the host, each slot as a full float population.
+    [[nodiscard]] std::vector<float> raw_storage() const {
+        std::vector<float> out(static_cast<std::size_t>(q) * cells_);
+        call(cfd_lbm_download_raw_slots(h_, out.data(), err_, sizeof(err_)));
+        return out;
+    }
     void upload_populations(std::span<const float> logical) {
         require_cells(logical.size() == static_cast<std::size_t>(q) * cells_, "population array size mismatch");
         call(cfd_lbm_upload_populations(h_, logical.data(), err_, sizeof(err_)));
@@ -284,6 +291,24 @@ public:
         void* s = nullptr;
         call(cfd_lbm_stream(h_, &s, err_, sizeof(err_)));
         return s;
+    }
+    // queue() of the device sibling (esoteric_pull_sycl.hpp:375-376): here the HIP stream every
+    // launch of this solver is ordered on (a hipStream_t).
+    [[nodiscard]] void* queue() const { return stream(); }
+
+    // device_local_acceleration (esoteric_pull_sycl.hpp:342-353): the per-cell acceleration planes
+    // on the device, allocated (zero) on first use and optionally cleared; the next steps apply them.
+    struct DeviceAccelerationView {
+        float* ax{};
+        float* ay{};
+        float* az{};
+        std::size_t cells{};
+    };
+    [[nodiscard]] DeviceAccelerationView device_local_acceleration(bool clear = false) {
+        void* p[3] = {};
+        call(cfd_lbm_device_local_acceleration(h_, clear ? 1 : 0, &p[0], &p[1], &p[2], err_, sizeof(err_)));
+        has_local_ = true;
+        return {static_cast<float*>(p[0]), static_cast<float*>(p[1]), static_cast<float*>(p[2]), cells_};
     }
     [[nodiscard]] HipTransferStats transfer_stats() const { return stats(false); }
     void reset_transfer_stats() const { (void)stats(true); }

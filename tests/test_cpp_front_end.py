@@ -58,6 +58,10 @@ template <class D> double exercise() {
     const auto v = s.refresh_device_macroscopic();
     const auto pops = s.logical_populations();
     s.upload_populations(pops);
+    const auto raw = s.raw_storage();
+    const auto acc = s.device_local_acceleration(true);
+    (void)s.device_local_acceleration();
+    (void)raw; (void)acc; (void)s.queue();
     const auto stats = s.transfer_stats();
     s.reset_transfer_stats();
     (void)v; (void)stats; (void)d;

@@ -131,6 +131,32 @@ def test_cpu_device_parity_gate(hip, oracle_mod, q):
     assert np.array_equal(gf, of)
 
 
+@pytest.mark.parametrize("q,storage", [(19, "fp32"), (27, "fp32"), (9, "fp32"), (19, "fp16s"), (19, "fp32s"),
+                                       (27, "fp64")])
+def test_raw_storage_matches_oracle(hip, oracle_mod, q, storage):
+    """raw_storage (esoteric_pull.hpp:276-278): the in-place slot array in the reference's order
+    d*cells + n, after 0, 7 and 8 steps (both parities), bit for bit against the oracle's slots
+    (whose periodic raw storage is bit-exact against the reference build's)."""
+    report("raw_storage", q=q, storage=storage)
+    n = (20, 16, 1) if q == 9 else (14, 11, 9)
+    g, o = make_pair(hip, oracle_mod, q, n, 0.66, (1e-6, -2e-6, 0.0 if q == 9 else 5e-7), storage=storage)
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    g.initialize_taylor_green(0.03, 1e-4, 5)
+    o.init_taylor_green(f32(0.03), f32(1e-4), 5)
+    for steps in (0, 7, 1):
+        g.step(steps)
+        o.step(steps)
+        got = g.raw_storage()
+        want = o.raw()
+        assert got.shape == want.shape == (q, int(np.prod(n)))
+        assert np.array_equal(got, want.astype(np.float32)), f"after {g.time_step} steps"
+        if storage != "fp64":
+            assert np.array_equal(got.astype(np.float64), want)
+    # Slots are not logical populations: at odd parity half of them sit in the partner plane.
+    assert not np.array_equal(g.raw_storage(), g.logical_populations())
+    g.close()
+
+
 def test_config1_taylor_green_64(hip, oracle_mod):
     """D3Q19 64^3 periodic, tau 0.6, TG amplitude 0.05 + seeded noise 1e-4, 10 + 100 steps."""
     report("config1", n=64)
