@@ -300,8 +300,8 @@ def resident_leg(hip, n=256, reps=50, particles=1 << 20):
     step, Boussinesq coupling, fill advection, Q-criterion and a one-way particle push.  Each figure
     is the wall time of `reps` back-to-back launches on the flow's stream between two
     synchronisations (launch gaps included), so it bounds the kernel time from above.  Algorithmic
-    bytes per cell: thermal 7 reads + 7 writes + 3 velocity reads = 68 B; Boussinesq 60 B (the
-    temperature plane's refresh, 7 reads + 1 write, then T read + acceleration read/write); advection fill read + write + 3 velocities = 20 B;
+    bytes per cell: thermal 7 reads + 7 writes + 3 velocity reads = 68 B; Boussinesq 56 B (one pass:
+    7 population reads, the temperature write, acceleration read/write); advection fill read + write + 3 velocities = 20 B;
     interface geometry (normals: fill read + 3 writes; curvature: 3 reads + 1 write) 32 B; surface
     tension (geometry, then fill and curvature reads + acceleration read/write) 64 B;
     Q-criterion 3 velocities + 1 write = 16 B; per particle 8 state reads + 6 writes = 56 B plus
@@ -336,7 +336,7 @@ def resident_leg(hip, n=256, reps=50, particles=1 << 20):
            "particles": particles}
     for name, fn, nbytes in (
             ("thermal_step", lambda: thermal.step(macro, 1), 68 * cells),
-            ("boussinesq", lambda: thermal.add_boussinesq_acceleration(force, 300.0, 1e-4, (0, -1, 0)), 60 * cells),
+            ("boussinesq", lambda: thermal.add_boussinesq_acceleration(force, 300.0, 1e-4, (0, -1, 0)), 56 * cells),
             ("fill_advect", lambda: surface.advect(macro, 0.1), 20 * cells),
             ("interface_geometry", lambda: surface.refresh_geometry(), 32 * cells),
             ("surface_tension", lambda: surface.add_surface_tension_acceleration(force, 0.072, 1.0), 64 * cells),

@@ -143,12 +143,20 @@ __global__ void __launch_bounds__(kBlock) thermal_scalar(const float* __restrict
     s[n] = value;
 }
 
-// a += -beta (T - T_ref) g  (resident_multiphysics_sycl.hpp:140-155)
-__global__ void __launch_bounds__(kBlock) boussinesq(const float* __restrict__ t, float* ax, float* ay, float* az, Dims d,
-                                                     float t_ref, float beta, float gx, float gy, float gz) {
+// The temperature refresh and the Boussinesq coupling a += -beta (T - T_ref) g
+// (resident_multiphysics_sycl.hpp:140-155) in one pass: T = the sum of the 7 populations (the
+// thermal_scalar order) is written to the temperature plane and drives the coupling.
+__global__ void __launch_bounds__(kBlock) boussinesq_from_populations(const float* __restrict__ g, float* __restrict__ t,
+                                                                      float* ax, float* ay, float* az, Dims d,
+                                                                      float t_ref, float beta, float gx, float gy,
+                                                                      float gz) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= d.cells) return;
-    const float scale = -beta * (t[n] - t_ref);
+    float value = 0.0F;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) value += g[static_cast<size_t>(q) * d.cells + n];
+    t[n] = value;
+    const float scale = -beta * (value - t_ref);
     ax[n] += scale * gx;
     ay[n] += scale * gy;
     az[n] += scale * gz;
@@ -677,9 +685,10 @@ int cfd_lbm_thermal_add_boussinesq(cfd_lbm_thermal* t, float* ax, float* ay, flo
         need_obj(t).use();
         need_views({ax, ay, az}, "Boussinesq coupling");
         if (!std::isfinite(t_ref) || !std::isfinite(beta)) throw std::invalid_argument("invalid resident Boussinesq coupling");
-        const float* temp = t->refresh_scalar();
-        boussinesq<<<blocks_for(t->dims.cells), kBlock, 0, t->flow.stream>>>(
-            temp, ax, ay, az, t->dims, t_ref, beta, static_cast<float>(gx), static_cast<float>(gy), static_cast<float>(gz));
+        if (!t->scalar) t->scalar = t->alloc(t->dims.cells);
+        boussinesq_from_populations<<<blocks_for(t->dims.cells), kBlock, 0, t->flow.stream>>>(
+            t->g, t->scalar, ax, ay, az, t->dims, t_ref, beta, static_cast<float>(gx), static_cast<float>(gy),
+            static_cast<float>(gz));
         check(hipGetLastError(), "boussinesq");
     });
 }
