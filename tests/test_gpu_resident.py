@@ -163,10 +163,11 @@ def test_q_criterion_matches_oracle(hip, oracle_mod):
     flow.close()
 
 
-@pytest.mark.parametrize("shape", [(300, 37, 11), (65, 5, 3), (3, 129, 4)])
+@pytest.mark.parametrize("shape", [(300, 37, 11), (65, 5, 3), (3, 129, 4), (64, 128, 6)])
 def test_stencils_on_ragged_shapes(hip, oracle_mod, shape):
     """The stencil kernels' 3-D launch (64-256-wide blocks, several rows per thread) on extents that
-    leave partial blocks in x and y: every kernel bit for bit against the oracle."""
+    leave partial blocks in x and y, and on one whose planes hold a multiple of 8 blocks for every
+    kernel (the per-XCD block remap): every kernel bit for bit against the oracle."""
     nx, ny, nz = shape
     report("resident_ragged", shape=shape)
     flow = hip.EsotericPullHipSolver("d3q19", nx, ny, nz, 0.7)
