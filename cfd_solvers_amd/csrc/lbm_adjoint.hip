@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
     }
     Real f[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(fstate, g, p.links, h, n, f, s);
+    load_cell<Q, T, Odd, Bounded>(fstate, g, link_desc<Bounded>(p.links, n), h, f, s);
     if constexpr (Storage<T>::kShifted)  // the transposed collision takes the populations themselves
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void drag_partial(const void* __restrict__ f, 
         const Neighbourhood h = neighbourhood(g, x, y, z);
         Real fin[Q];
         PairSlots<Q> s;
-        load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+        load_cell<Q, T, Odd, Bounded>(f, g, link_desc<Bounded>(p.links, n), h, fin, s);
         static_for<Q - 1>([&](auto DD) {
             constexpr int d = decltype(DD)::value + 1;
             const int cd = a.drag_axis == 0 ? L::cx[d] : (a.drag_axis == 1 ? L::cy[d] : L::cz[d]);
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void plane_partial(const void* __restrict__ f,
         if (fluid) {
             Real fin[Q];
             PairSlots<Q> s;
-            load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+            load_cell<Q, T, Odd, Bounded>(f, g, link_desc<Bounded>(p.links, n), h, fin, s);
             Real rho = Real(0);
             static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
             const double rho_d = Storage<T>::kShifted

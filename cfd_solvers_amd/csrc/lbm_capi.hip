@@ -84,7 +84,7 @@ struct cfd_lbm_solver {
     Geometry geo{};
     hipStream_t stream = nullptr;
     const KernelOps* ops = nullptr;
-    DeviceBuffer<unsigned char> f, flags, links, link_vel, interior, solid_rho, gamma, lacc, macro, partial, reduce_out;
+    DeviceBuffer<unsigned char> f, flags, links, link_vel, segments, solid_rho, gamma, lacc, macro, partial, reduce_out;
     float wall_u[8][3] = {};
     float u_peak = 0.0F, rho_out = 1.0F;
     bool les = false;  // Smagorinsky LES (esoteric_pull.hpp:72-97, InPlaceLbmConfig::smagorinsky_les)
@@ -239,7 +239,7 @@ struct cfd_lbm_solver {
         p.solid_rho = reinterpret_cast<const Real*>(solid_rho.ptr);
         p.links = reinterpret_cast<const uint32_t*>(links.ptr);
         p.link_vel = reinterpret_cast<const uint32_t*>(link_vel.ptr);
-        p.interior = interior.ptr;
+        p.segments = reinterpret_cast<const uint16_t*>(segments.ptr);
         p.seg_x = seg_x();
         p.cells = static_cast<uint32_t>(cells);
         std::memcpy(physics, &p, sizeof(p));
@@ -432,7 +432,7 @@ struct cfd_lbm_solver {
     }
 
     // Makes the device flags in `fresh` (empty: periodic, no flags) the solver's flags: walls that
-    // stop being walls are thawed, new walls frozen, links and the interior map rebuilt.  The host
+    // stop being walls are thawed, new walls frozen, links and the segment spans rebuilt.  The host
     // copy (host_flags) must already hold the same values.
     void install_flags(DeviceBuffer<unsigned char>& fresh) {
         const bool have = static_cast<bool>(fresh);
@@ -447,12 +447,12 @@ struct cfd_lbm_solver {
             flags.reset();
             links.reset();
             link_vel.reset();
-            interior.reset();
+            segments.reset();
             solid_rho.reset();
             return;
         }
-        if (!interior) interior.allocate(static_cast<size_t>(seg_x()) * ny * nz);
-        ops->interior(st, fresh.ptr, seg_x(), interior.ptr);
+        if (!segments) segments.allocate(static_cast<size_t>(seg_x()) * ny * nz * sizeof(uint16_t));
+        ops->segments(st, fresh.ptr, seg_x(), reinterpret_cast<uint16_t*>(segments.ptr));
         if (!links) links.allocate(cells * sizeof(uint32_t));
         if (!link_vel) link_vel.allocate(cells * vel_words() * sizeof(uint32_t));
         ops->links(st, fresh.ptr, reinterpret_cast<uint32_t*>(links.ptr), reinterpret_cast<uint32_t*>(link_vel.ptr));
@@ -1115,7 +1115,7 @@ struct cfd_lbm_solver {
 
     size_t population_bytes() const { return static_cast<size_t>(q) * cells * elem; }
     size_t resident_bytes() const {
-        return f.bytes + flags.bytes + links.bytes + link_vel.bytes + interior.bytes + solid_rho.bytes + gamma.bytes + lacc.bytes + macro.bytes + partial.bytes +
+        return f.bytes + flags.bytes + links.bytes + link_vel.bytes + segments.bytes + solid_rho.bytes + gamma.bytes + lacc.bytes + macro.bytes + partial.bytes +
                reduce_out.bytes;
     }
 };

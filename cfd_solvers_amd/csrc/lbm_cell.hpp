@@ -38,18 +38,23 @@ template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t
     else __builtin_nontemporal_store(v, p);
 }
 
+// The link descriptor of cell n on a bounded lattice (0 on a periodic one).
+template <bool Bounded> __device__ __forceinline__ uint32_t link_desc(const uint32_t* links, uint32_t n) {
+    if constexpr (Bounded) return links[n];
+    else return 0U;
+}
+
 // Loads the logical populations of a non-wall cell (bounce-back aware) and records the slots.
-// n is the cell's linear index (for its link descriptor); slots come from the neighbourhood.
+// desc is the cell's link descriptor (used only when Bounded); slots come from the neighbourhood.
 template <int Q, class T, bool Odd, bool Bounded>
-__device__ __forceinline__ void load_cell(const void* f, const Geometry& g, const uint32_t* links,
-                                          const Neighbourhood& h, uint32_t n,
+__device__ __forceinline__ void load_cell(const void* f, const Geometry& g, uint32_t desc, const Neighbourhood& h,
                                           typename Storage<T>::Real (&fin)[Q], PairSlots<Q>& s) {
     using L = Lattice<Q>;
     using S = Storage<T>;
     const uint32_t off_n = h.cell(0, 0, 0) * static_cast<uint32_t>(sizeof(T));
     s.self = off_n;
     const uint32_t plane_bytes = static_cast<uint32_t>(g.stride * sizeof(T));
-    if constexpr (Bounded) s.desc = links[n];
+    if constexpr (Bounded) s.desc = desc;
     fin[0] = S::template decode<Q, 0>(load_at<T>(plane<T>(f, g, 0), off_n));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;

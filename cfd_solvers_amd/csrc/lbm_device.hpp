@@ -351,7 +351,7 @@ template <class Real> struct Physics {
     const Real* solid_rho;      // density frozen in wall cells
     const uint32_t* links;      // per-cell link descriptor (bounded lattices)
     const uint32_t* link_vel;   // per cell, (q-1+9)/10 words of 3-bit wall-velocity indices per link
-    const uint8_t* interior;    // per 64-cell row segment: 1 if its 3x3x66 neighbourhood is all fluid
+    const uint16_t* segments;   // per 64-cell row segment: span lo | hi << 8 of its non-plain cells, 0 if none
     uint32_t seg_x;             // segments per row, ceil(nx / 64)
     uint32_t cells;
 };

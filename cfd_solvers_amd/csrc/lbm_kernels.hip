@@ -22,11 +22,12 @@ namespace {
 template <int Q, class T, bool Odd, bool Bounded, bool Forced>
 __device__ __forceinline__ void update_cell(const void* src, void* dst, const Geometry& g,
                                             const Physics<typename Storage<T>::Real>& p, const Neighbourhood& h,
-                                            uint8_t flag, uint32_t n, uint32_t x, uint32_t y, uint32_t z) {
+                                            uint8_t flag, uint32_t desc, uint32_t n, uint32_t x, uint32_t y,
+                                            uint32_t z) {
     using Real = typename Storage<T>::Real;
     Real fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(src, g, p.links, h, n, fin, s);
+    load_cell<Q, T, Odd, Bounded>(src, g, desc, h, fin, s);
     const Real rho = collide<Q, Forced, Storage<T>>(p, g, flag, n, y, z, fin);
     store_cell<Q, T, Odd, Bounded>(dst, g, p, n, x, y, z, rho, fin, s);
 }
@@ -47,25 +48,30 @@ __global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const vo
     const Neighbourhood h = neighbourhood(g, x, y, z);
     const uint32_t n = h.lin(0, 0, 0);
     if constexpr (Bounded) {
-        // Wavefronts whose whole neighbourhood is plain fluid take the periodic path: no link
-        // loads, no bounce-back selects.  A wavefront covers one 64-cell row segment when
-        // bx >= 64, so the branch is uniform there.
-        if (p.interior[(z * g.ny + y) * p.seg_x + (x >> 6)] != 0U) {
-            update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, n, x, y, z);
+        // A 64-cell row segment whose cells are all plain fluid (span 0) takes the periodic path:
+        // no link loads, no bounce-back selects.  A wavefront covers one segment when bx >= 64,
+        // so the branch is uniform there.  Otherwise only the cells inside the segment's span
+        // [lo, hi) can have a nonzero link descriptor; the others skip its load.
+        const uint32_t span = p.segments[(z * g.ny + y) * p.seg_x + (x >> 6)];
+        if (span == 0U) {
+            update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, 0U, n, x, y, z);
             return;
         }
-        const uint32_t type = (p.links[n] >> kTypeShift) & 3U;
+        const uint32_t xi = x & 63U;
+        const uint32_t desc = (xi >= (span & 0xFFU) && xi < (span >> 8)) ? p.links[n] : 0U;
+        const uint32_t type = (desc >> kTypeShift) & 3U;
         if (type == kTypeWall) return;
-        update_cell<Q, T, Odd, true, Forced>(src, dst, g, p, h, static_cast<uint8_t>(type), n, x, y, z);
+        update_cell<Q, T, Odd, true, Forced>(src, dst, g, p, h, static_cast<uint8_t>(type), desc, n, x, y, z);
     } else {
-        update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, n, x, y, z);
+        update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, 0U, n, x, y, z);
     }
 }
 
-// 1 for a 64-cell row segment whose cells and all their neighbours (x from 64s-1 to 64s+64,
-// y +- 1, z +- 1, periodic) are plain fluid.
-__global__ __launch_bounds__(256) void mark_interior(const uint8_t* __restrict__ flags, const Geometry g,
-                                                     uint32_t seg_x, uint8_t* __restrict__ interior) {
+// The span of each 64-cell row segment: cells [lo, hi) of the segment that are not plain fluid
+// with a plain-fluid 3x3x3 neighbourhood (periodic), stored as lo | hi << 8; 0 when every cell is
+// (the interior segments).  Cells outside the span have link descriptor 0.
+__global__ __launch_bounds__(256) void mark_segments(const uint8_t* __restrict__ flags, const Geometry g,
+                                                     uint32_t seg_x, uint16_t* __restrict__ segments) {
     const uint32_t total = seg_x * g.ny * g.nz;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
@@ -73,24 +79,32 @@ __global__ __launch_bounds__(256) void mark_interior(const uint8_t* __restrict__
     const uint32_t r = k / seg_x;
     const uint32_t y = r % g.ny;
     const uint32_t z = r / g.ny;
-    const int x0 = static_cast<int>(sx * 64U) - 1;
-    const int x1 = static_cast<int>(min(sx * 64U + 64U, g.nx));  // inclusive: last cell + 1
-    uint8_t all_fluid = 1U;
-    for (int dz = -1; dz <= 1 && all_fluid; ++dz) {
-        const uint32_t zz = (z + g.nz + static_cast<uint32_t>(dz)) % g.nz;
-        for (int dy = -1; dy <= 1 && all_fluid; ++dy) {
+    const uint32_t xs = sx * 64U, xe = min(xs + 64U, g.nx);
+    uint32_t rows[9];
+    for (int dz = -1, i = 0; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy, ++i) {
+            const uint32_t zz = (z + g.nz + static_cast<uint32_t>(dz)) % g.nz;
             const uint32_t yy = (y + g.ny + static_cast<uint32_t>(dy)) % g.ny;
-            const uint32_t row = (zz * g.ny + yy) * g.nx;
-            for (int xx = x0; xx <= x1; ++xx) {
-                const uint32_t xw = static_cast<uint32_t>((xx + static_cast<int>(g.nx)) % static_cast<int>(g.nx));
-                if (flags[row + xw] != kFluid) {
-                    all_fluid = 0U;
-                    break;
-                }
-            }
+            rows[i] = (zz * g.ny + yy) * g.nx;
         }
+    // column(x): the 3x3 flags at x in the rows above are all plain fluid.
+    auto column = [&](uint32_t xx) {
+        for (int i = 0; i < 9; ++i)
+            if (flags[rows[i] + xx] != kFluid) return false;
+        return true;
+    };
+    bool left = column(xs == 0U ? g.nx - 1U : xs - 1U), mid = column(xs);
+    uint32_t lo = 64U, hi = 0U;
+    for (uint32_t x = xs; x < xe; ++x) {
+        const bool right = column(x + 1U == g.nx ? 0U : x + 1U);
+        if (!(left && mid && right)) {
+            lo = min(lo, x - xs);
+            hi = x - xs + 1U;
+        }
+        left = mid;
+        mid = right;
     }
-    interior[k] = all_fluid;
+    segments[k] = hi == 0U ? uint16_t{0} : static_cast<uint16_t>(lo | (hi << 8));
 }
 
 // Link descriptor of every cell from the flags (layout in lbm_device.hpp).
@@ -262,7 +276,7 @@ __global__ __launch_bounds__(256) void freeze_wall_density(const void* __restric
     if (old_flags && is_wall(old_flags[n])) return;
     Real fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, false>(f, g, nullptr, h, n, fin, s);
+    load_cell<Q, T, Odd, false>(f, g, 0U, h, fin, s);
     Real rho = Real(0);
     static_for<Q>([&](auto DD) { rho += fin[decltype(DD)::value]; });
     if constexpr (Storage<T>::kShifted) rho = Real(1) + rho * static_cast<Real>(Storage<T>::kInvScale);
@@ -327,7 +341,7 @@ __global__ __launch_bounds__(256) void macroscopic(const void* __restrict__ f, c
     }
     Real fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+    load_cell<Q, T, Odd, Bounded>(f, g, link_desc<Bounded>(p.links, n), h, fin, s);
     Real rho, ux, uy, uz;
     cell_state<Q, Storage<T>>(p, g, flag, n, y, z, fin, rho, ux, uy, uz);
     out[n] = rho;
@@ -357,7 +371,7 @@ __global__ __launch_bounds__(256) void download_logical(const void* __restrict__
     }
     Real fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+    load_cell<Q, T, Odd, Bounded>(f, g, link_desc<Bounded>(p.links, n), h, fin, s);
     static_for<Q>([&](auto DD) {
         constexpr int d = decltype(DD)::value;
         out[static_cast<uint64_t>(d) * p.cells + n] = full_population<Storage<T>, Q, d>(fin[d]);
@@ -411,7 +425,7 @@ __global__ __launch_bounds__(256) void diagnostics_partial(const void* __restric
         } else {
             Real fin[Q];
             PairSlots<Q> s;
-            load_cell<Q, T, Odd, Bounded>(f, g, p.links, h, n, fin, s);
+            load_cell<Q, T, Odd, Bounded>(f, g, link_desc<Bounded>(p.links, n), h, fin, s);
             Real sum = Real(0);
             static_for<Q>([&](auto DD) { sum += fin[decltype(DD)::value]; });
             // Shifted storage: the sum of deviations is rho - 1 (added in FP64).
@@ -500,7 +514,7 @@ __global__ __launch_bounds__(256) void probe_load(const uint32_t* __restrict__ i
     }
     uint32_t fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, links, h, n, fin, s);
+    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, link_desc<Bounded>(links, n), h, fin, s);
     for (int d = 0; d < Q; ++d) load[static_cast<uint64_t>(d) * cells + n] = fin[d];
 }
 
@@ -514,7 +528,7 @@ __global__ __launch_bounds__(256) void probe_store(uint32_t* __restrict__ ids, c
     if (Bounded && is_wall(p.flags[n])) return;
     uint32_t fin[Q];
     PairSlots<Q> s;
-    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, p.links, h, n, fin, s);  // records the slots
+    load_cell<Q, uint32_t, Odd, Bounded>(ids, g, link_desc<Bounded>(p.links, n), h, fin, s);  // records the slots
     uint32_t fout[Q];
     for (int d = 0; d < Q; ++d) fout[d] = d * p.cells + n;
     store_cell<Q, uint32_t, Odd, Bounded>(ids, g, p, n, x, y, z, 1U, fout, s);
@@ -575,11 +589,11 @@ template <int Q, class T> struct Ops final : KernelOps {
                                st.f, st.geo, flags, old_flags, static_cast<Real*>(solid_rho));
         check(hipGetLastError(), "freeze_wall_density launch");
     }
-    void interior(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint8_t* out) const override {
+    void segments(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint16_t* out) const override {
         const uint32_t total = seg_x * st.geo.ny * st.geo.nz;
-        hipLaunchKernelGGL(mark_interior, dim3((total + 255U) / 256U), dim3(256), 0, st.stream, flags, st.geo, seg_x,
+        hipLaunchKernelGGL(mark_segments, dim3((total + 255U) / 256U), dim3(256), 0, st.stream, flags, st.geo, seg_x,
                            out);
-        check(hipGetLastError(), "mark_interior launch");
+        check(hipGetLastError(), "mark_segments launch");
     }
     void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
               const void* solid_rho) const override {
