@@ -211,7 +211,10 @@ class EsotericPullHipSolver:
         self.lib.call("cfd_lbm_initialize_taylor_green", self._h, amplitude, noise, seed)
 
     def initialize_fields(self, rho, ux, uy, uz):
+        """initialize(initializer) (esoteric_pull.hpp:312-333) with host fields of `cells` values each."""
         arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (rho, ux, uy, uz)]
+        if any(a.size != self.cells for a in arrs):
+            raise ValueError("initial LBM field size mismatch: rho, ux, uy, uz need cells values each")
         self.lib.call("cfd_lbm_initialize_fields", self._h, *arrs)
 
     def upload_populations(self, logical):

@@ -26,6 +26,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 
 
 def build(reference: str | None = None, quiet: bool = True) -> None:
@@ -78,6 +79,8 @@ def lib():
         L.orc_f16_to_f32.restype = C.c_float
         L.orc_uniform01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_uniform01.restype = C.c_double
+        L.orc_splitmix64.argtypes = [_u64p, _u64p, C.c_size_t]
+        L.orc_uniform01_cells.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, _dp]
         L.orc_voxelize.argtypes = [_dp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp, _u8p]
         S3 = [C.c_size_t] * 3
         L.orc_thermal_fill.argtypes = [_fp, C.c_void_p, C.c_float, C.c_size_t]
@@ -223,6 +226,21 @@ def f32_to_f16_bits(values):
 def f16_bits_to_f32(bits):
     L = lib()
     return np.array([L.orc_f16_to_f32(int(b)) for b in np.asarray(bits, dtype=np.uint16)], dtype=np.float32)
+
+
+def splitmix64(values):
+    """The SplitMix64 output function of the oracle, elementwise over uint64 values."""
+    a = np.ascontiguousarray(values, dtype=np.uint64).reshape(-1)
+    out = np.empty_like(a)
+    lib().orc_splitmix64(a, out, a.size)
+    return out
+
+
+def uniform01_cells(seed, comp, n):
+    """The oracle's noise variates uniform01(seed, cell, comp) for cells 0..n-1."""
+    out = np.empty(n)
+    lib().orc_uniform01_cells(seed, comp, n, out)
+    return out
 
 
 def uniform01(seed, cell, comp):
@@ -401,6 +419,7 @@ def ref_lib(native=False):
         L.ref_eso_raw.argtypes = [C.c_void_p, _fp]
         L.ref_eso_set_raw.argtypes = [C.c_void_p, _fp]
         L.ref_eso_set_mask.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_eso_set_local_acceleration.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ref_eso_time_step.argtypes = [C.c_void_p]
         L.ref_eso_time_step.restype = C.c_uint64
         L.ref_pull_create.argtypes = L.ref_eso_create.argtypes
@@ -490,6 +509,16 @@ class RefEsoteric:
         a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
         assert a.size == self.q * self.cells
         _check(self.L.ref_eso_set_raw(self.h, a), self.L, "ref")
+
+    def set_local_acceleration(self, ax, ay=None, az=None):
+        """set_local_body_acceleration (esoteric_pull.hpp:144-151); ax None clears it (:153-157)."""
+        if ax is None:
+            self._lacc = None
+            _check(self.L.ref_eso_set_local_acceleration(self.h, None, None, None), self.L, "ref")
+            return
+        self._lacc = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (ax, ay, az)]
+        assert all(a.size == self.cells for a in self._lacc)
+        _check(self.L.ref_eso_set_local_acceleration(self.h, *(a.ctypes.data for a in self._lacc)), self.L, "ref")
 
     def set_mask(self, mask):
         if mask is None:

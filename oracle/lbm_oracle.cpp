@@ -1602,6 +1602,15 @@ ORC_API int orc_adjoint_run(void* h, int kind, uint64_t steps, uint64_t window, 
 }
 ORC_API uint16_t orc_f32_to_f16(float f) { return orc::f32_to_f16_rne(f); }
 ORC_API float orc_f16_to_f32(uint16_t h) { return orc::f16_to_f32(h); }
+// The SplitMix64 output function: the generator seeded with s returns splitmix64(s + k * gamma)
+// as its k-th output (k = 0, 1, ...).  Vectorised for the tests that pin it.
+ORC_API void orc_splitmix64(const uint64_t* in, uint64_t* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = orc::splitmix64(in[i]);
+}
+// uniform01(seed, cell, component) for cells [0, n) of one component (the noise of init_taylor_green).
+ORC_API void orc_uniform01_cells(uint64_t seed, uint64_t comp, size_t n, double* out) {
+    for (size_t i = 0; i < n; ++i) out[i] = static_cast<double>(orc::uniform01<float>(seed, i, comp));
+}
 ORC_API double orc_uniform01(uint64_t seed, uint64_t cell, uint64_t comp) {
     return static_cast<double>(orc::uniform01<float>(seed, cell, comp));
 }

@@ -41,6 +41,7 @@ struct EsoBase {
     virtual void raw(float* out) const = 0;
     virtual void set_raw(const float* in) = 0;
     virtual void set_mask(const std::uint8_t* m) = 0;
+    virtual void set_local_acc(const float* ax, const float* ay, const float* az) = 0;
     virtual std::uint64_t time_step() const = 0;
     virtual std::size_t cells() const = 0;
 };
@@ -78,6 +79,16 @@ template <class D> struct Eso final : EsoBase {
     void set_mask(const std::uint8_t* m) override {
         if (m) s.set_solid_mask(std::span<const std::uint8_t>(m, s.cells()));
         else s.clear_solid_mask();
+    }
+    // set_local_body_acceleration / clear_local_body_acceleration (esoteric_pull.hpp:144-157).
+    void set_local_acc(const float* ax, const float* ay, const float* az) override {
+        if (!ax) {
+            s.clear_local_body_acceleration();
+            return;
+        }
+        const std::size_t n = s.cells();
+        s.set_local_body_acceleration(std::span<const float>(ax, n), std::span<const float>(ay, n),
+                                      std::span<const float>(az, n));
     }
     std::uint64_t time_step() const override { return s.time_step(); }
     std::size_t cells() const override { return s.cells(); }
@@ -181,6 +192,9 @@ REF_API int ref_eso_raw(void* h, float* out) { return guarded([&] { static_cast<
 REF_API int ref_eso_set_raw(void* h, const float* in) { return guarded([&] { static_cast<EsoBase*>(h)->set_raw(in); }); }
 REF_API int ref_eso_set_mask(void* h, const std::uint8_t* m) {
     return guarded([&] { static_cast<EsoBase*>(h)->set_mask(m); });
+}
+REF_API int ref_eso_set_local_acceleration(void* h, const float* ax, const float* ay, const float* az) {
+    return guarded([&] { static_cast<EsoBase*>(h)->set_local_acc(ax, ay, az); });
 }
 REF_API std::uint64_t ref_eso_time_step(void* h) { return static_cast<EsoBase*>(h)->time_step(); }
 

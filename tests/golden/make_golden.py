@@ -142,14 +142,41 @@ def precision_fp64():
     np.savez_compressed(OUT / "precision_fp64.npz", **out)
 
 
+def local_acceleration():
+    """set_local_body_acceleration (esoteric_pull.hpp:144-151; applied in the collision at :414-420
+    and in compute_macroscopic at :180-209) with a seeded random field on top of a uniform
+    acceleration, Taylor-Green 0.03, tau 0.7, 9 steps (both parities): the inputs, the fields at
+    step 0 and step 9 and the raw storage at step 9."""
+    out = {}
+    for q, n in ((9, (16, 12, 1)), (19, (10, 8, 6)), (27, (8, 6, 5))):
+        accel = (1.0e-6, -0.5e-6, 2.0e-6 if q != 9 else 0.0)
+        rng = np.random.default_rng(14 + q)
+        cells = n[0] * n[1] * n[2]
+        local = rng.uniform(-4.0e-6, 4.0e-6, (3, cells)).astype(np.float32)
+        if q == 9:
+            local[2] = 0.0
+        r = O.RefEsoteric(q, *n, 0.7, accel)
+        r.init_taylor_green(0.03)
+        r.set_local_acceleration(*local)
+        out[f"q{q}_shape"] = np.array(n)
+        out[f"q{q}_accel"] = np.array(accel)
+        out[f"q{q}_local"] = local
+        out[f"q{q}_fields0"] = np.stack(r.macroscopic())
+        r.step(9)
+        out[f"q{q}_fields9"] = np.stack(r.macroscopic())
+        out[f"q{q}_raw9"] = r.raw()
+    np.savez_compressed(OUT / "local_acceleration.npz", **out)
+
+
+FIXTURES = {"index_maps": index_maps, "parity_runs": parity_runs, "taylor_green_32": taylor_green_32,
+            "bounded_defect": bounded_defect, "d2q9_walls": d2q9_walls, "precision_fp64": precision_fp64,
+            "local_acceleration": local_acceleration}
+
 if __name__ == "__main__":
+    # python tests/golden/make_golden.py [fixture ...]   (default: all of them)
     if not O.ref_available():
         sys.exit("oracle/_ref/libref_lbm.so is not built (make -C oracle all, with the reference tree present)")
-    index_maps()
-    parity_runs()
-    taylor_green_32()
-    bounded_defect()
-    d2q9_walls()
-    precision_fp64()
+    for name in sys.argv[1:] or list(FIXTURES):
+        FIXTURES[name]()
     for f in sorted(OUT.glob("*.npz")):
         print(f.name, f.stat().st_size)

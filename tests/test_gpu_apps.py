@@ -69,7 +69,9 @@ def test_cfd_solve_cases(case):
             continue
         assert v == v and abs(v) != float("inf"), f"{key}={value}"
     if case == "lbm-poiseuille-hip":
-        assert float(fields["relative_L2"]) < 0.02
+        # The reference's gate is 1 % (tests/test_main.cpp:449-477, D2Q9); D3Q19 halfway
+        # bounce-back at tau 0.8 and 30 fluid rows measures ~1.1e-3.
+        assert float(fields["relative_L2"]) < 2e-3
     if case in ("lbm-d3q19-hip", "lbm-d3q27-hip", "lbm-d2q9-inplace-hip"):
         assert float(fields["mass_error_rel"]) < 1e-5 and float(fields["MLUPS"]) > 0
 

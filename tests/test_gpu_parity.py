@@ -562,3 +562,118 @@ def test_smagorinsky_rejects_shifted_storage(hip):
     g = hip.EsotericPullHipSolver(19, 8, 8, 8, 0.56, storage="fp16s")
     with pytest.raises(RuntimeError, match="FP32 or FP64"):
         g.set_smagorinsky(True)
+
+
+# ---------------------------------------------------------------------- set-up paths, invariants
+def test_initialize_fields_roundtrip_and_size_check(hip, oracle_mod):
+    """initialize(initializer) with host fields (esoteric_pull.hpp:312-333): the fields read back
+    equal the oracle's bit for bit and the inputs to float rounding; wrong-sized inputs are
+    refused before anything crosses the ABI."""
+    report("initialize_fields")
+    n = (20, 14, 9)
+    rng = np.random.default_rng(21)
+    shape = n[::-1]
+    rho = rng.uniform(0.95, 1.05, shape).astype(np.float32)
+    u = [rng.uniform(-0.03, 0.03, shape).astype(np.float32) for _ in range(3)]
+    g, o = make_pair(hip, oracle_mod, 19, n, 0.7)
+    g.initialize_fields(rho, *u)
+    o.init_fields(*(a.astype(np.float64) for a in (rho, *u)))
+    gf = gpu_fields(g)
+    assert_fields(gf, o.macroscopic())
+    want = np.stack([rho, *u]).reshape(4, -1).astype(np.float64)
+    assert np.abs(gf - want).max() < 1e-7
+    for bad in ((rho.reshape(-1)[:-1], *u), (1.0, *u), (rho, u[0], u[1], np.zeros(3, np.float32))):
+        with pytest.raises(ValueError, match="size mismatch"):
+            g.initialize_fields(*bad)
+    g.step(3)
+    o.step(3)
+    assert_fields(gpu_fields(g), o.macroscopic())
+
+
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_local_acceleration_matches_reference_fixture(hip, golden, q):
+    """set_local_body_acceleration (esoteric_pull.hpp:144-159) on a seeded random field plus a
+    uniform acceleration: fields at steps 0 and 9 and the raw storage at step 9 equal the
+    reference's own outputs bit for bit (tests/golden/local_acceleration.npz)."""
+    report("local_accel_fixture", q=q)
+    d = golden("local_acceleration")
+    nx, ny, nz = (int(v) for v in d[f"q{q}_shape"])
+    accel = tuple(float(a) for a in d[f"q{q}_accel"])
+    g = hip.EsotericPullHipSolver(q, nx, ny, nz, 0.7, accel)
+    g.initialize_taylor_green(0.03)
+    g.set_local_body_acceleration(*d[f"q{q}_local"])
+    assert np.array_equal(gpu_fields(g), d[f"q{q}_fields0"].reshape(4, -1).astype(np.float64))
+    g.step(9)
+    assert np.array_equal(gpu_fields(g), d[f"q{q}_fields9"].reshape(4, -1).astype(np.float64))
+    assert np.array_equal(g.raw_storage(), d[f"q{q}_raw9"])
+
+
+LAUNCH_SHAPES = [{}, {"CFD_LBM_BX": "32"}, {"CFD_LBM_BX": "64"}, {"CFD_LBM_BX": "128"}, {"CFD_LBM_BX": "256"},
+                 {"CFD_LBM_XCD_REMAP": "1"}, {"CFD_LBM_WAVES": "8"}, {"CFD_LBM_BLOCK_CELLS": "4096"},
+                 {"CFD_LBM_LAYOUT": "planes"}]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16s"])
+def test_results_invariant_under_launch_shape_and_layout(hip, monkeypatch, storage):
+    """Race freedom of the in-place step (docs/ESOTERIC_PULL.md:36-40: every slot is read and
+    written by one cell only): the state after both parities is bit-identical under any tile width,
+    the per-XCD block order, another occupancy, another block size and plain planes."""
+    n = (200, 60, 48)
+    f = np.zeros(n[::-1], np.uint8)
+    f[:, -1, :] = 0x81
+    f[:, :, 0] = f[:, :, -1] = 0x80
+    f[:, 0, :] = 0x80
+    f[0, :, :] = f[-1, :, :] = 0x80
+    f[20:30, 25:35, 90:110] = 0x80  # an obstacle away from the walls
+    results = []
+    for env in LAUNCH_SHAPES:
+        report("launch_shape", storage=storage, env=env)
+        for k in ("CFD_LBM_BX", "CFD_LBM_XCD_REMAP", "CFD_LBM_WAVES", "CFD_LBM_BLOCK_CELLS", "CFD_LBM_LAYOUT"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        g = hip.EsotericPullHipSolver(19, *n, 0.55, storage=storage)
+        g.set_flags(f)
+        g.set_wall_velocity(1, (0.1, 0.0, 0.0))
+        g.initialize_taylor_green(0.02, 1e-4, 3)
+        g.step(5)
+        odd = (gpu_fields(g), g.logical_populations())
+        g.step(1)
+        even = (gpu_fields(g), g.logical_populations())
+        results.append((odd, even))
+        g.close()
+    for env, (odd, even) in zip(LAUNCH_SHAPES[1:], results[1:]):
+        for got, want in ((odd, results[0][0]), (even, results[0][1])):
+            assert np.array_equal(got[0], want[0]), f"fields differ under {env}"
+            assert np.array_equal(got[1], want[1]), f"populations differ under {env}"
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16s"])
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_nonfinite_population_is_flagged(hip, storage, bad):
+    """Failure detection: one non-finite population makes the FP64 reductions non-finite (the
+    reference's guard is an isfinite check on such a scalar).  After one step it has reached
+    exactly the cell and its 18 neighbours: the population is pulled into the collision of its
+    own cell, and that cell's outputs stream one link each."""
+    report("nonfinite", storage=storage, bad=bad)
+    nx, ny, nz = 16, 12, 10
+    g = hip.EsotericPullHipSolver(19, nx, ny, nz, 0.6, storage=storage)
+    g.initialize_taylor_green(0.02)
+    pops = g.logical_populations().copy()
+    x, y, z = 7, 5, 4
+    c = (z * ny + y) * nx + x
+    pops[11, c] = bad
+    g.upload_populations(pops)
+    mass, ke, vmax = g.diagnostics()
+    assert not np.isfinite(mass) and not np.isfinite(ke)
+    assert not np.isfinite(g.total_mass())
+    g.step(1)
+    assert not np.isfinite(g.diagnostics()[0])
+    rho = g.download_macroscopic().rho
+    cx = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0]
+    cy = [0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1]
+    cz = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1]
+    want = np.zeros((nz, ny, nx), bool)
+    for d in range(19):
+        want[z + cz[d], y + cy[d], x + cx[d]] = True
+    assert np.array_equal(~np.isfinite(rho), want)

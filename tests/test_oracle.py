@@ -576,3 +576,154 @@ def test_smagorinsky_behaviour():
     assert np.array_equal(a.raw(), b.raw())
     with pytest.raises(RuntimeError, match="FP32 or FP64"):
         O.Oracle(19, 8, 8, 8, 0.56, storage="fp16s").set_smagorinsky(True)
+
+
+# ---------------------------------------------------------------- seeded noise (SplitMix64)
+# The generator of Steele, Lea & Flood (OOPSLA 2014), as published in Vigna's splitmix64.c: seeded
+# with s, its k-th output is the output function applied to s + k * 0x9E3779B97F4A7C15.  The
+# expected values are the published ones (Rosetta Code, "Pseudo-random numbers/Splitmix64"): the
+# first five outputs for seed 1234567, and the bucket counts of floor(5 * next / 2^64) over 100 000
+# outputs for seed 987654321.
+GOLDEN_GAMMA = 0x9E3779B97F4A7C15
+
+
+def _stream(seed, count):
+    k = np.arange(count, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return O.splitmix64(np.uint64(seed) + k * np.uint64(GOLDEN_GAMMA))
+
+
+def test_splitmix64_matches_published_outputs():
+    assert _stream(1234567, 5).tolist() == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                            4593380528125082431, 16408922859458223821]
+    draws = _stream(987654321, 100000)
+    buckets = np.floor(draws.astype(np.float64) / 2.0 ** 64 * 5).astype(np.int64)
+    assert np.bincount(buckets, minlength=5).tolist() == [20027, 19892, 20073, 19978, 20030]
+
+
+def test_noise_variates_are_uniform_on_the_2_pow_24_grid():
+    """uniform01(seed, cell, component): [0, 1) on multiples of 2^-24, mean 1/2 and variance 1/12
+    over 10^6 cells, flat over 64 bins (chi-square), uncorrelated across components, neighbouring
+    cells and seeds."""
+    n = 1_000_000
+    u = [O.uniform01_cells(1, c, n) for c in range(3)]
+    for v in u:
+        assert v.min() >= 0.0 and v.max() < 1.0
+        assert np.array_equal(v * 2.0 ** 24, np.floor(v * 2.0 ** 24))
+        assert abs(v.mean() - 0.5) < 5 * np.sqrt(1 / 12 / n)
+        assert abs(v.var() - 1 / 12) < 5 * np.sqrt(1 / 180 / n)
+        hist = np.bincount((v * 64).astype(np.int64), minlength=64)
+        chi2 = float(((hist - n / 64) ** 2 / (n / 64)).sum())
+        assert chi2 < 120  # 63 degrees of freedom: P(chi2 > 120) < 1e-5
+    for a, b in ((u[0], u[1]), (u[1], u[2]), (u[0][1:], u[0][:-1]), (u[0], O.uniform01_cells(2, 0, n))):
+        assert abs(np.corrcoef(a, b)[0, 1]) < 5 / np.sqrt(n)
+    # The scalar export agrees with the vector one.
+    assert O.uniform01(1, 12345, 2) == u[2][12345]
+
+
+@pytest.mark.parametrize("q", [9, 19])
+def test_taylor_green_noise_lands_on_its_cell_and_component(q):
+    """init_taylor_green(A, eps, seed) adds eps * (2 u(seed, n, c) - 1) to velocity component c of
+    cell n: the noisy field minus the clean one equals that, to float rounding."""
+    n = (12, 10, 8 if q != 9 else 1)
+    eps = 1e-3
+    a, b = O.Oracle(q, *n, 0.6), O.Oracle(q, *n, 0.6)
+    a.init_taylor_green(F32(0.05))
+    b.init_taylor_green(F32(0.05), F32(eps), 7)
+    ua, ub = fields(a)[1:], fields(b)[1:]
+    cells = int(np.prod(n))
+    for c in range(3 if q != 9 else 2):
+        want = F32(eps) * (2 * O.uniform01_cells(7, c, cells) - 1)
+        # u is recovered from FP32 equilibrium populations: a few ulps of |u| ~ 0.05 (1e-8 each).
+        assert np.abs((ub[c] - ua[c]) - want).max() < 2e-7
+    if q == 9:
+        assert np.array_equal(ub[2], ua[2])
+
+
+# ---------------------------------------------------------------- local body acceleration
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_local_acceleration_matches_reference_fixture(golden, q):
+    """set_local_body_acceleration (esoteric_pull.hpp:144-159, used at :180-209 and :414-420) on a
+    seeded random field plus a uniform acceleration: fields at steps 0 and 9 and the raw storage at
+    step 9 equal the reference's bit for bit (fixture from tests/golden/make_golden.py)."""
+    d = golden("local_acceleration")
+    n = tuple(int(v) for v in d[f"q{q}_shape"])
+    accel = tuple(F32(a) for a in d[f"q{q}_accel"])
+    local = d[f"q{q}_local"]
+    o = O.Oracle(q, *n, F32(0.7), accel)
+    o.init_taylor_green(F32(0.03))
+    o.set_local_acceleration(*local.astype(np.float64))
+    assert np.array_equal(fields(o), d[f"q{q}_fields0"].reshape(4, -1).astype(np.float64))
+    o.step(9)
+    assert np.array_equal(fields(o), d[f"q{q}_fields9"].reshape(4, -1).astype(np.float64))
+    assert np.array_equal(o.raw(), d[f"q{q}_raw9"].astype(np.float64))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_local_acceleration_matches_live_reference(q):
+    """The same against the live reference build, on another lattice and seed, with the field
+    cleared half-way (clear_local_body_acceleration, esoteric_pull.hpp:153-157)."""
+    n = (14, 9, 7 if q != 9 else 1)
+    cells = int(np.prod(n))
+    accel = (F32(-2e-6), F32(1e-6), F32(5e-7) if q != 9 else 0.0)
+    local = np.random.default_rng(100 + q).uniform(-1e-5, 1e-5, (3, cells)).astype(np.float32)
+    if q == 9:
+        local[2] = 0.0
+    o = O.Oracle(q, *n, F32(0.62), accel)
+    r = O.RefEsoteric(q, *n, 0.62, accel)
+    o.init_taylor_green(F32(0.02))
+    r.init_taylor_green(0.02)
+    o.set_local_acceleration(*local.astype(np.float64))
+    r.set_local_acceleration(*local)
+    o.step(5)
+    r.step(5)
+    assert np.array_equal(fields(o), np.stack(r.macroscopic()).reshape(4, -1).astype(np.float64))
+    o.set_local_acceleration(None)
+    r.set_local_acceleration(None)
+    o.step(4)
+    r.step(4)
+    assert np.array_equal(fields(o), np.stack(r.macroscopic()).reshape(4, -1).astype(np.float64))
+    assert np.array_equal(o.raw(), r.raw().astype(np.float64))
+
+
+# ---------------------------------------------------------------- pressure-drop objective
+def _plane_drop(rho, f, x_in, x_out):
+    """Mean rho/3 over the fluid cells of plane x_in minus the same over plane x_out (numpy)."""
+    fluid = f == O.FLUID
+    return rho[:, :, x_in][fluid[:, :, x_in]].mean() / 3 - rho[:, :, x_out][fluid[:, :, x_out]].mean() / 3
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16s"])
+def test_pressure_drop_is_the_plane_mean_density_difference(storage):
+    """objective(pressure_drop) against numpy on the macroscopic density of a developing duct flow,
+    for several plane pairs, and against the closed form of a linear density ramp."""
+    o, f = _duct_oracle(0.02 * np.ones(16 * 8 * 8), 0.05)
+    if storage != "fp64":
+        o2 = O.Oracle(19, 16, 8, 8, 0.6, storage=storage)
+        o2.set_flags(f)
+        o2.set_inlet_outlet(0.05, 1.0)
+        o2.set_gamma(np.where(f.reshape(-1) == 0, 0.02, 0.0))
+        o2.init_uniform()
+        o = o2
+    o.step(37)
+    rho = o.macroscopic()[0]
+    # The plane means are ~1/3; FP32 densities carry ~6e-8 relative rounding per cell.
+    tol = 1e-16 if storage == "fp64" else 5e-8
+    for x_in, x_out in ((1, 14), (3, 9), (12, 2), (1, 2)):
+        assert o.objective(x_in, x_out) == pytest.approx(_plane_drop(rho, f, x_in, x_out), rel=1e-13, abs=tol)
+    # Linear density ramp at rest, rho(x) = 1 + 0.01 x (given as float): j = (rho(x_in) - rho(x_out)) / 3.
+    # The tolerance is the storage's rounding of the equilibrium populations summed over a cell.
+    nx, ny, nz = 16, 8, 8
+    ramp = O.Oracle(19, nx, ny, nz, 0.6, storage=storage)
+    ramp.set_flags(f)
+    x = np.broadcast_to(np.arange(nx, dtype=np.float64), (nz, ny, nx))
+    z = np.zeros((nz, ny, nx))
+    ramp.init_fields(1.0 + 0.01 * x, z, z, z)
+    # At rest f_d = w_d rho with the lattice's float weights, whose sum is 1 + 1.5e-8 in float.
+    wsum = float(np.float32(1 / 3)) + 6 * float(np.float32(1 / 18)) + 12 * float(np.float32(1 / 36))
+    tol = {"fp64": 1e-15, "fp32": 5e-7, "fp16s": 3e-5}[storage]
+    real = (lambda v: v) if storage == "fp64" else F32  # the collision's precision
+    for x_in, x_out in ((1, 14), (5, 6), (13, 1)):
+        want = wsum * (real(1 + 0.01 * x_in) - real(1 + 0.01 * x_out)) / 3
+        assert ramp.objective(x_in, x_out) == pytest.approx(want, rel=0, abs=tol)
