@@ -804,6 +804,10 @@ struct cfd_lbm_solver {
         if (!offsets_fit(real))
             throw std::invalid_argument("lattice too large for the adjoint: two FP32 lambda planes must fit in 4 GiB");
         const size_t state_bytes = pop_elems * elem;
+        // Checkpoint buffers are carved from one allocation at a 4 KiB pitch, so every buffer
+        // starts as aligned as the allocation (hipMalloc: at least 256 B) whatever nx is; the
+        // unpadded state size is odd for odd nx.
+        const size_t pitch = (state_bytes + 4095) & ~static_cast<size_t>(4095);
         const size_t fixed = pop_elems * real + 2 * cells * 8 + 64;
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
@@ -813,22 +817,26 @@ struct cfd_lbm_solver {
         } else {
             if (snapshots == 0) {
                 const size_t usable = free_b - free_b / 16;
-                snapshots = usable > fixed + 2 * state_bytes ? (usable - fixed) / state_bytes - 1 : 0;
+                snapshots = usable > fixed + 2 * pitch ? (usable - fixed) / pitch - 1 : 0;
                 if (snapshots == 0) throw std::invalid_argument("no room for a checkpoint buffer in device memory");
             }
             snapshots = std::min<uint64_t>(snapshots, std::max<uint64_t>(steps - 1, 1));
             buffers = static_cast<size_t>(snapshots) + 1;
         }
-        const size_t need = buffers * state_bytes + fixed;
+        const size_t need = buffers * pitch + fixed;
         if (need > free_b)
             throw std::invalid_argument("checkpoints do not fit in device memory (" + std::to_string(need >> 20) +
                                         " MiB needed, " + std::to_string(free_b >> 20) +
                                         " MiB free): raise the checkpoint interval or lower the snapshot count");
-        // One allocation for all checkpoint states (state_bytes is a multiple of 1 KiB).
+        // One allocation for all checkpoint states, one buffer every `pitch` bytes.
         DeviceBuffer<unsigned char> pool_mem;
-        pool_mem.allocate(buffers * state_bytes);
+        pool_mem.allocate(buffers * pitch);
         std::vector<void*> pool(buffers);
-        for (size_t k = 0; k < buffers; ++k) pool[k] = pool_mem.ptr + k * state_bytes;
+        for (size_t k = 0; k < buffers; ++k) {
+            pool[k] = pool_mem.ptr + k * pitch;
+            if (reinterpret_cast<uintptr_t>(pool[k]) % 256 != 0)
+                throw std::runtime_error("checkpoint buffer is not 256-byte aligned");
+        }
         DeviceBuffer<unsigned char> lam, gg, gp, acc;
         lam.allocate(pop_elems * real);
         gg.allocate(cells * sizeof(double));
