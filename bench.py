@@ -2,8 +2,8 @@
 """Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
 
     python bench.py [--steps K] [--warmup W]
-                    [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1[,config5]]
-                    [--no-cpu]
+                    [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1,config5]
+                    [--config5-steps T] [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
@@ -13,14 +13,19 @@ error of the others) and D3Q27, and the adjoint leg runs the
 checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps), and the
 voxelize leg the GPU STL voxelisation of a 16128-triangle sphere on 512^3 cells, and the resident
 leg the device-resident coupling modules (thermal, fill, particles, Q-criterion) at 256^3, and the shard
-leg the kernel on one GPU's share of a 768^3 lattice split in eight z-slabs, and config1 the
-cache-resident 64^3 Taylor-Green correctness case.  The opt-in
-config5 leg runs forward + adjoint of a 768^3 FP16 duct past a sphere cluster (drag objective,
-binomial checkpointing) against the FP32-storage run.  Prints progress on stderr and ONE JSON
+leg the kernel and its halo pack/unpack on one GPU's share of a 768^3 lattice split in eight
+z-slabs, and config1 the
+cache-resident 64^3 Taylor-Green correctness case, and the config5
+leg forward + adjoint of a 768^3 duct past a sphere cluster (drag objective, binomial
+checkpointing) with FP16 and FP32 storage, on a horizon truncated to 200 steps by default, with the
+plan's cost at the specified 10^4 steps projected from the measured step times.  Prints progress on stderr and ONE JSON
 line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
-(apps/cfd-bench.cpp:111-117); the roofline uses algorithmic HBM bytes per launch (each fluid
-population read and written once, plus one read of the flag array) over the kernel time measured
-with HIP events on the solver's stream.  Two probes run in the same process: a plain 2 GiB copy
+(apps/cfd-bench.cpp:111-117), and `fluid_mlups` beside it counts only the cells a step updates
+(wall cells return at once); the roofline uses algorithmic HBM bytes per launch (each fluid
+population read and written once; the bounded kernel's metadata -- one 2-byte span per 64-cell row
+segment and a 4-byte link descriptor per cell inside a span -- is overhead, not algorithmic, and
+shows in the measured `traffic`) over the kernel time measured with HIP events on the solver's
+stream.  Two probes run in the same process: a plain 2 GiB copy
 and the in-place esoteric-pull access pattern alone (the bandwidth ceiling of an in-place
 stream-collide), with the headline kernel's fraction of the latter.
 """
@@ -82,13 +87,13 @@ def run_leg(hip, name, lattice, storage, n, tau, steps, warmup):
                          m.uz.reshape(-1)[fluid_mask]]).astype(np.float64)
     cells = n ** 3
     fluid = int(np.count_nonzero(flags == 0))
-    bytes_launch = fluid * 2 * q * BYTES_PER_POP[storage] + cells * 1
+    bytes_launch = fluid * 2 * q * BYTES_PER_POP[storage]
     launch_s = gpu_ms * 1e-3 / steps
     gbs = bytes_launch / launch_s / 1e9
     res = {
         "lattice": lattice.upper(), "storage": storage, "n": n, "cells": cells, "fluid_cells": fluid,
         "steps": steps, "warmup": warmup, "seconds": wall, "ms_per_step": wall * 1e3 / steps,
-        "mlups": cells * steps / wall / 1e6, "kernel_ms": gpu_ms / steps,
+        "mlups": cells * steps / wall / 1e6, "fluid_mlups": fluid * steps / wall / 1e6, "kernel_ms": gpu_ms / steps,
         "kernel_mlups": cells / launch_s / 1e6, "algorithmic_bytes_per_launch": bytes_launch,
         "bytes_per_fluid_cell_update": 2 * q * BYTES_PER_POP[storage],
         "achieved_gbs": gbs, "roofline_frac": gbs / PEAK_GBS,
@@ -118,14 +123,44 @@ def config1_leg(hip, n=64, steps=100, warmup=10):
     return out
 
 
-def shard_leg(hip, shape=(768, 768, 96), steps=100, warmup=10, storages=("fp16s", "fp32")):
+class _DeviceBuffers:
+    """hipMalloc / hipFree through the HIP runtime the solver library loaded (by soname)."""
+
+    def __init__(self):
+        import ctypes as C
+
+        self.C = C
+        self.rt = C.CDLL("libamdhip64.so.7")
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.live = []
+
+    def alloc(self, nbytes):
+        p = self.C.c_void_p()
+        if self.rt.hipMalloc(self.C.byref(p), int(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        self.live.append(p.value)
+        return p.value
+
+    def free(self):
+        for p in self.live:
+            self.rt.hipFree(p)
+        self.live.clear()
+
+
+def shard_leg(hip, shape=(768, 768, 96), steps=100, warmup=10, storages=("fp16s", "fp32"), exchanges=50):
     """The kernel at one GPU's share of a z-slab decomposition: 768 x 768 x 96 cells is one of eight
-    slabs of a 768^3 lattice.  On one GPU the slab is periodic in z where the distributed run would
-    exchange ghost planes; the per-cell work of the step is the same.  Periodic Taylor-Green start
-    (A 0.02, tau 0.6)."""
+    slabs of a 768^3 lattice, periodic Taylor-Green start (A 0.02, tau 0.6).  The slab runs the
+    periodic kernel; after each step a distributed run packs the crossing populations of its two
+    faces into an up and a down message (5 planes of 768 x 768 each), exchanges them with the
+    neighbouring slabs and unpacks what it receives (cfd_lbm_halo_pack / cfd_lbm_halo_unpack).  On
+    one GPU the slab is its own neighbour; the pack + unpack pair is timed here (wall time of
+    `exchanges` pairs between two synchronisations, launch gaps included, so an upper bound), the
+    transfer between GPUs is not."""
     nx, ny, nz = shape
     cells = nx * ny * nz
     out = {"workload": f"D3Q19 {nx}x{ny}x{nz} periodic Taylor-Green slab (one GPU's share of 768^3 on 8)"}
+    bufs = _DeviceBuffers()
     for storage in storages:
         s = hip.EsotericPullHipSolver("d3q19", nx, ny, nz, 0.6, storage=storage)
         s.initialize_taylor_green(0.02)
@@ -134,12 +169,28 @@ def shard_leg(hip, shape=(768, 768, 96), steps=100, warmup=10, storages=("fp16s"
         gpu_ms = s.step_timed(steps)
         nbytes = cells * 2 * 19 * BYTES_PER_POP[storage]
         launch_s = gpu_ms * 1e-3 / steps
+        msg = s.halo_message_bytes()
+        up, down = bufs.alloc(msg), bufs.alloc(msg)
+        s.halo_pack(up, down)
+        s.halo_unpack(up, down)
+        s.wait()
+        t0 = time.perf_counter()
+        for _ in range(exchanges):
+            s.halo_pack(up, down)
+            s.halo_unpack(up, down)
+        s.wait()
+        pair_ms = (time.perf_counter() - t0) * 1e3 / exchanges
         out[storage] = {"kernel_ms": gpu_ms / steps, "mlups": cells / launch_s / 1e6,
                         "algorithmic_bytes_per_launch": nbytes, "gbs": nbytes / launch_s / 1e9,
-                        "roofline_frac": nbytes / launch_s / 1e9 / PEAK_GBS, "finite": bool(np.isfinite(s.total_mass()))}
+                        "roofline_frac": nbytes / launch_s / 1e9 / PEAK_GBS, "finite": bool(np.isfinite(s.total_mass())),
+                        "halo": {"message_bytes": msg, "messages_per_step": 2,
+                                 "pack_unpack_ms_upper_bound": pair_ms,
+                                 "fraction_of_step": pair_ms / (gpu_ms / steps)}}
         log(f"shard {storage}: {out[storage]['mlups']:.0f} MLUPS, kernel {gpu_ms / steps:.3f} ms, "
-            f"{out[storage]['gbs']:.0f} GB/s ({100 * out[storage]['roofline_frac']:.1f}% of 8 TB/s)")
+            f"{out[storage]['gbs']:.0f} GB/s ({100 * out[storage]['roofline_frac']:.1f}% of 8 TB/s); "
+            f"halo pack + unpack of 2 x {msg / 1e6:.1f} MB <= {pair_ms * 1e3:.0f} us per step")
         s.close()
+        bufs.free()
     return out
 
 
@@ -210,55 +261,89 @@ def adjoint_leg(hip, n=128, steps=2000, window=200, every=100, kernel_steps=100)
     return res
 
 
-def config5_leg(hip, n=768, steps=1000, storages=("fp16s", "fp32")):
+def config5_leg(hip, n=768, steps=200, storages=("fp16s", "fp32"), kernel_steps=4, horizon=10000):
     """Flow past a cluster: D3Q19 n^3 duct (inlet peak 0.04, outlet rho 1), 64 non-overlapping spheres
     of radius U[10, 30] (seed 5) with centres in the middle 40 % of each axis, a partial-bounce-back
-    halo ramping from 0.5 at the sphere surfaces to 0 four cells out, tau 0.52; J = drag along x
-    averaged over all `steps` states; binomial checkpointing with as many state buffers as fit.
+    halo ramping from 0.5 at the sphere surfaces to 0 four cells out, tau 0.52, impulsive start
+    (u = (0.04, 0, 0) everywhere, so that the cluster carries drag from the first step); J = drag
+    along x averaged over all `steps` states; binomial checkpointing with as many state buffers as fit.
+    The horizon is truncated to `steps` (the specified T is `horizon` = 10^4 steps, which takes
+    minutes per storage); the cost of the plan at T = `horizon` with the same buffers is reported
+    from the closed form and the measured per-step times.  A second run per storage with a buffer
+    for every state (`kernel_steps` steps, no recomputation) times the transposed step alone.
     The FP16-storage run is compared with the FP32-storage run of the same problem."""
     from cfd_solvers_amd import geometry as G
 
-    shape = (n, n, n)
-    lo, hi = 0.3 * n, 0.7 * n
-    centres, radii = G.random_spheres(shape, 10, 30, seed=5, count=64, box=((lo, hi),) * 3, periodic_x=False)
-    flags = G.channel_flags(n, n, n, inlet_outlet=True)
-    body = G.rasterize(shape, centres, radii, periodic_x=False)
-    flags[body] = G.WALL
-    halo = G.porosity_halo(shape, centres, radii, 4.0, 0.5)
-    halo[flags != 0] = 0.0
-    flags, body, halo = flags.reshape(-1), body.reshape(-1), halo.reshape(-1)
+    t0 = time.perf_counter()
+    flags, body, halo = G.sphere_cluster_duct(n, seed=5)
     halo_cells = np.flatnonzero(halo > 0)
-    out = {"workload": f"D3Q19 {n}^3 duct, 64-sphere cluster + PBB halo, tau 0.52, inlet 0.04, drag, T {steps}",
-           "cells": n ** 3, "steps": steps, "spheres": 64, "body_cells": int(body.sum()),
+    fluid = int(np.count_nonzero(flags == 0)) + int(np.count_nonzero((flags == 1) | (flags == 2)))
+    log(f"leg config5: geometry {time.perf_counter() - t0:.1f} s, {int(body.sum())} body cells, "
+        f"{halo_cells.size} halo cells")
+    cells = n ** 3
+    out = {"workload": f"D3Q19 {n}^3 duct, 64-sphere cluster + PBB halo, tau 0.52, inlet 0.04, drag",
+           "cells": cells, "fluid_cells": fluid, "steps": steps, "horizon_specified": horizon,
+           "truncated": steps < horizon, "spheres": 64, "body_cells": int(body.sum()),
            "halo_cells": int(halo_cells.size)}
     grads = {}
-    for storage in storages:
-        log(f"leg config5: {storage} {n}^3, T {steps}")
+
+    def solver(storage):
         s = hip.EsotericPullHipSolver("d3q19", n, n, n, 0.52, storage=storage)
         s.set_flags(flags)
         s.set_drag_body(body, 0)
         s.set_inlet_outlet(0.04, 1.0)
         s.set_porosity(halo)
-        s.initialize_uniform()
+        s.initialize_uniform(1.0, 0.04, 0.0, 0.0)  # impulsive start: the inflow everywhere
+        return s
+
+    for storage in storages:
+        elem = BYTES_PER_POP[storage]
+        log(f"leg config5: {storage} {n}^3, T {steps}")
+        s = solver(storage)
         t0 = time.perf_counter()
         J, grad, du, fw = s.adjoint_run_binomial(steps, steps, 0, "drag")
         wall = time.perf_counter() - t0
         fwd_ms, rev_ms = s.adjoint_timing()
         s.close()
         grads[storage] = grad.reshape(-1)[halo_cells]
-        out[storage] = {"objective": J, "du_peak": du, "forward_steps": fw, "wall_seconds": wall,
-                        "forward_ms": fwd_ms, "reverse_ms": rev_ms,
-                        "forward_mlups": n ** 3 * steps / (fwd_ms * 1e3),
-                        "adjoint_mlups": n ** 3 * steps / (rev_ms * 1e3),
-                        "total_over_forward": (fwd_ms + rev_ms) / fwd_ms}
-        log(f"leg config5 {storage}: J {J:.6e}, du {du:.6e}, forward {fwd_ms:.0f} ms, reverse {rev_ms:.0f} ms "
-            f"({fw} forward steps)")
         del grad
+        # The same problem with a buffer for every state: the reverse phase is transposed steps only.
+        s = solver(storage)
+        s.adjoint_run_binomial(kernel_steps, kernel_steps, kernel_steps - 1, "drag", want_gamma=False)
+        kfwd_ms, krev_ms = s.adjoint_timing()
+        s.close()
+        step_ms = fwd_ms / steps
+        adj_ms = krev_ms / kernel_steps
+        # Buffers of the truncated run: solve the plan's closed form backwards for the snapshot
+        # count that gives the measured forward steps, then cost the specified horizon with it.
+        snaps = next((k for k in range(1, 4096) if hip.checkpoint_plan(steps, k)[0] == fw), None)
+        adj_bytes = fluid * (2 * 19 * 4 + 19 * elem + 4 + 16)  # lambda r/w (FP32), f read, gamma, FP64 grad r/w
+        res = {"objective": J, "du_peak": du, "forward_steps": fw, "wall_seconds": wall,
+               "forward_ms": fwd_ms, "reverse_ms": rev_ms, "forward_step_ms": step_ms,
+               "forward_mlups": cells * steps / (fwd_ms * 1e3), "adjoint_mlups": cells * steps / (rev_ms * 1e3),
+               "reverse_over_forward": rev_ms / fwd_ms, "total_over_forward": (fwd_ms + rev_ms) / fwd_ms,
+               "snapshots": snaps,
+               "transposed_step": {"ms": adj_ms, "over_forward_step": adj_ms / step_ms,
+                                   "algorithmic_bytes": adj_bytes,
+                                   "gbs": adj_bytes / (adj_ms * 1e-3) / 1e9,
+                                   "roofline_frac": adj_bytes / (adj_ms * 1e-3) / 1e9 / PEAK_GBS}}
+        if snaps is not None:
+            hf, hr, _ = hip.checkpoint_plan(horizon, snaps)
+            proj_rev = (hf - horizon) * step_ms + horizon * adj_ms
+            res["plan_at_horizon"] = {"steps": horizon, "snapshots": snaps, "forward_steps": hf, "repetitions": hr,
+                                      "projected_reverse_over_forward": proj_rev / (horizon * step_ms),
+                                      "projected_seconds": (hf * step_ms + horizon * adj_ms) * 1e-3}
+        out[storage] = res
+        log(f"leg config5 {storage}: J {J:.6e}, du {du:.6e}, forward {fwd_ms:.0f} ms, reverse {rev_ms:.0f} ms "
+            f"({rev_ms / fwd_ms:.2f}x, {fw} forward steps); transposed step {adj_ms:.2f} ms = "
+            f"{adj_ms / step_ms:.2f}x a forward step, {res['transposed_step']['gbs']:.0f} GB/s; "
+            f"T={horizon}: {res.get('plan_at_horizon')}")
     if "fp16s" in grads and "fp32" in grads:
         a, b = grads["fp16s"], grads["fp32"]
         out["fp16_vs_fp32"] = {
             "objective_rel": abs(out["fp16s"]["objective"] - out["fp32"]["objective"]) / abs(out["fp32"]["objective"]),
-            "du_peak_rel": abs(out["fp16s"]["du_peak"] - out["fp32"]["du_peak"]) / abs(out["fp32"]["du_peak"]),
+            "du_peak_rel": (abs(out["fp16s"]["du_peak"] - out["fp32"]["du_peak"]) / abs(out["fp32"]["du_peak"])
+                            if out["fp32"]["du_peak"] != 0 else None),
             "grad_rel_l2_halo": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
         log(f"leg config5: FP16 vs FP32 {out['fp16_vs_fp32']}")
     return out
@@ -387,18 +472,52 @@ def inplace_probe(n=256, repeats=20):
     return out
 
 
-def cpu_baseline(target_seconds=15.0, voxelize_mesh=None, resident=False):
+def host_cores():
+    """The host cores this process may use: its CPU affinity set, capped by a cgroup CPU quota
+    where one is set (cpu.max "quota period"), with what was read."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"usable": usable, "affinity": affinity, "cpu_count": os.cpu_count(), "cgroup_quota_cpus": quota,
+            "model": model, "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(target_seconds=15.0, voxelize_mesh=None, resident=False, n=256):
     """The reference's EsotericPullSolver<D3Q19> (oracle/_ref, built from the reference sources
-    like its Release build) on the host cores: 256^3 periodic Taylor-Green, cfd-bench protocol
-    (apps/cfd-bench.cpp:174-191) on a bounded number of steps.  Falls back to the CPU
-    restatement (kind "port") where the reference build is absent."""
-    threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16")))
+    like its Release build, run on this host) on every host core the process may use (one OpenMP
+    thread per core, OMP_PROC_BIND=close, OMP_PLACES=cores): 256^3 periodic Taylor-Green -- the
+    reference's cfd-bench workload and protocol (apps/cfd-bench.cpp:174-191), not the GPU
+    headline's cavity -- on a bounded number of steps.  Falls back to the CPU restatement (kind
+    "port") where the reference build is absent."""
+    cores = host_cores()
+    threads = cores["usable"]
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     import oracle
 
-    n = 256
+    # libgomp reads OMP_NUM_THREADS once, when it is first loaded; set the calling thread's team
+    # size directly as well, in the libgomp instance the checker libraries link.
+    try:
+        import ctypes as C
+
+        gomp = C.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(threads)
+        threads = int(gomp.omp_get_max_threads())
+    except OSError:
+        pass
     if oracle.ref_available(native=True):
         kind = "reference"
         s = oracle.RefEsoteric(19, n, n, n, 0.6, native=True)
@@ -417,16 +536,11 @@ def cpu_baseline(target_seconds=15.0, voxelize_mesh=None, resident=False):
     dt = time.perf_counter() - t0
     s.mass()
     mlups = n ** 3 * steps / dt / 1e6
-    cpu = ""
-    try:
-        with open("/proc/cpuinfo") as fh:
-            cpu = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
-    except OSError:
-        pass
     log(f"cpu baseline ({kind}): {mlups:.1f} MLUPS on {threads} threads, {steps} steps in {dt:.1f} s")
     out = {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": kind,
-           "sample": f"D3Q19 256^3 periodic Taylor-Green (A 0.02, tau 0.6), 1 warmup + {steps} timed steps, "
-                     f"FP32, {threads} OpenMP threads, {cpu}"}
+           "sample": f"D3Q19 {n}^3 periodic Taylor-Green (A 0.02, tau 0.6), 1 warmup + {steps} timed steps, "
+                     f"FP32, {threads} OpenMP threads on {cores['model']}",
+           "host": cores}
     if voxelize_mesh is not None and oracle.ref_available(native=True):
         # The reference's voxelize_surface (src/lbm/voxelize.cpp:5: every cell against every
         # triangle, one thread) on a small grid with the voxelize leg's mesh.
@@ -481,15 +595,16 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1")
-    ap.add_argument("--config5-steps", type=int, default=1000)
+    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1,config5")
+    ap.add_argument("--config5-steps", type=int, default=200)
     ap.add_argument("--config5-n", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the z-slab "
-              "sharding of the lattice over several GPUs is not implemented.", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the z-slab halo "
+              "messages exist (cfd_lbm_halo_pack / cfd_lbm_halo_unpack) but their transport between GPUs "
+              "is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import cfd_solvers_amd as hip

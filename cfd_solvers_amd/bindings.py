@@ -65,6 +65,14 @@ _SIGNATURES = {
     "cfd_lbm_transfer_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.c_int],
     "cfd_lbm_index_map": [C.c_void_p, C.c_int, _i64p, _i64p],
     "cfd_lbm_objective_pressure_drop": [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)],
+    "cfd_lbm_set_porosity_device": [C.c_void_p, C.c_void_p],
+    "cfd_lbm_adjoint_run_binomial_device": [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                                            C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_uint64)],
+    "cfd_lbm_halo_message_bytes": [C.c_void_p, C.POINTER(C.c_size_t)],
+    "cfd_lbm_halo_pack": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "cfd_lbm_halo_unpack": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "cfd_lbm_objective_run": [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double)],
     "cfd_lbm_adjoint_run": [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
                             C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_double)],
     "cfd_lbm_adjoint_timing": [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)],
@@ -402,6 +410,13 @@ class EsotericPullHipSolver:
         self.lib.call("cfd_lbm_objective_pressure_drop", self._h, x_in, x_out, C.byref(v))
         return v.value
 
+    def objective_run(self, steps, window, objective="pressure_drop", x_in=1, x_out=None):
+        """J of adjoint_run / adjoint_run_binomial from the current state, without the adjoint."""
+        J = C.c_double()
+        x_out = self.shape[-1] - 2 if x_out is None else x_out
+        self.lib.call("cfd_lbm_objective_run", self._h, OBJECTIVES[objective], steps, window, x_in, x_out, C.byref(J))
+        return J.value
+
     def adjoint_run(self, steps, window, checkpoint_every, x_in, x_out, want_gamma=True):
         """J, dJ/dgamma (shape (nz, ny, nx), FP64) and dJ/du_peak; the state advances by `steps`."""
         J, du = C.c_double(), C.c_double()
@@ -438,6 +453,35 @@ class EsotericPullHipSolver:
         self.lib.call("cfd_lbm_adjoint_run_binomial", self._h, OBJECTIVES[objective], steps, window, snapshots, x_in,
                       x_out, C.byref(J), grad.ctypes.data if grad is not None else None, C.byref(du), C.byref(fw))
         return J.value, (grad.reshape(self.shape) if grad is not None else None), du.value, fw.value
+
+    def halo_message_bytes(self):
+        """Bytes of one z-slab halo message (crossing pairs x nx x ny populations)."""
+        b = C.c_size_t()
+        self.lib.call("cfd_lbm_halo_message_bytes", self._h, C.byref(b))
+        return b.value
+
+    def halo_pack(self, up_device, down_device):
+        """After a step: this slab's up and down messages into two device buffers (addresses)."""
+        self.lib.call("cfd_lbm_halo_pack", self._h, C.c_void_p(int(up_device)), C.c_void_p(int(down_device)))
+
+    def halo_unpack(self, from_below_device, from_above_device):
+        """The up message of the slab below and the down message of the slab above, into this slab."""
+        self.lib.call("cfd_lbm_halo_unpack", self._h, C.c_void_p(int(from_below_device)),
+                      C.c_void_p(int(from_above_device)))
+
+    def set_porosity_device(self, gamma_device):
+        """Porosity from device memory: gamma_device is the device address (int) of cells floats."""
+        self.lib.call("cfd_lbm_set_porosity_device", self._h, C.c_void_p(int(gamma_device)))
+
+    def adjoint_run_binomial_device(self, steps, window, grad_device, snapshots=0, objective="pressure_drop", x_in=1,
+                                    x_out=None):
+        """adjoint_run_binomial with dJ/dgamma written to the device address grad_device (cells
+        doubles): J, dJ/du_peak, forward steps executed."""
+        J, du, fw = C.c_double(), C.c_double(), C.c_uint64()
+        x_out = self.shape[-1] - 2 if x_out is None else x_out
+        self.lib.call("cfd_lbm_adjoint_run_binomial_device", self._h, OBJECTIVES[objective], steps, window, snapshots,
+                      x_in, x_out, C.byref(J), C.c_void_p(int(grad_device)), C.byref(du), C.byref(fw))
+        return J.value, du.value, fw.value
 
     def adjoint_timing(self):
         fw, rv = C.c_float(), C.c_float()

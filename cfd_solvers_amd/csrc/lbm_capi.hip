@@ -75,6 +75,20 @@ __global__ void __launch_bounds__(256) zero_words_kernel(uint32_t* p, uint64_t w
 }
 }  // namespace
 
+namespace {
+// Porosity from device memory: gamma (cells floats) into the solver's plane of `real_bytes`-wide
+// reals; `bad` counts the values outside [0, 1] (NaN included).
+__global__ void __launch_bounds__(256) porosity_from_device(const float* __restrict__ src, void* __restrict__ dst,
+                                                            uint32_t real_bytes, uint64_t cells, uint32_t* bad) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const float g = src[i];
+    if (!(g >= 0.0F && g <= 1.0F)) atomicAdd(bad, 1U);
+    if (real_bytes == 4) static_cast<float*>(dst)[i] = g;
+    else static_cast<double*>(dst)[i] = static_cast<double>(g);
+}
+}  // namespace
+
 struct cfd_lbm_solver {
     int q = 19, storage = 0, device = 0;
     uint64_t pop_elems = 0;  // elements of the population array (layout of Geometry)
@@ -779,6 +793,32 @@ struct cfd_lbm_solver {
         return v;
     }
 
+    // J of adjoint_run without the adjoint: `steps` steps in place from the current state, with j
+    // accumulated in FP64 over the last `window` states by the adjoint run's own objective kernel in
+    // its order, so that J equals adjoint_run's bit for bit.  The state is left at time + steps.
+    double objective_run(int kind, uint64_t steps, uint64_t window, int x_in, int x_out) {
+        use_device();
+        if (steps < 1 || window < 1 || window > steps) throw std::invalid_argument("need 1 <= window <= steps");
+        const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage);
+        const cfd_hip::AdjointArgs args = objective_args(kind, x_in, x_out, window);
+        DeviceBuffer<unsigned char> acc;
+        acc.allocate(sizeof(double));
+        double* sum = reinterpret_cast<double*>(acc.ptr);
+        const LaunchState st = state();
+        adj->zero(st, sum, 1);
+        const bool b = bounded();
+        const bool forced = ax != 0.0F || ay != 0.0F || az != 0.0F || static_cast<bool>(lacc) || les;
+        for (uint64_t k = 0; k < steps; ++k) {
+            ops->step(st, odd(), b, forced, f.ptr, f.ptr);
+            ++t;
+            if (k + 1 >= steps - window + 1) adj->objective(st, odd(), b, f.ptr, args, 1.0, sum);
+        }
+        double J = 0.0;
+        d2h(&J, acc.ptr, sizeof(double));
+        sync();
+        return J;
+    }
+
     void adjoint_checks(uint64_t steps, uint64_t window) const {
         if (steps < 1 || window < 1 || window > steps) throw std::invalid_argument("need 1 <= window <= steps");
         if (ax != 0.0F || ay != 0.0F || az != 0.0F || lacc) throw std::invalid_argument("the adjoint supports unforced lattices only");
@@ -795,9 +835,14 @@ struct cfd_lbm_solver {
     //    steps run through them (each step recomputed once; memory steps/every + every - 1 states);
     //  * every == 0: binomial -- `snapshots` state buffers (0: as many as fit, keeping 1/16 of the
     //    free memory in reserve) plus the start state, optimal recomputation (checkpoint_schedule.hpp).
+    // grad_on_device: grad_gamma_out is device memory (cells doubles) and receives the gradient
+    // directly, with no host copy.
     double adjoint_run(int kind, uint64_t steps, uint64_t window, uint64_t every, uint64_t snapshots, int x_in,
-                       int x_out, double* grad_gamma_out, double* grad_peak_out, uint64_t* forward_steps_out) {
+                       int x_out, double* grad_gamma_out, double* grad_peak_out, uint64_t* forward_steps_out,
+                       bool grad_on_device = false) {
         use_device();
+        if (grad_on_device && (!grad_gamma_out || reinterpret_cast<uintptr_t>(grad_gamma_out) % 8 != 0))
+            throw std::invalid_argument("the device gradient needs an 8-byte aligned device pointer");
         adjoint_checks(steps, window);
         const cfd_hip::AdjointOps* adj = cfd_hip::adjoint_ops(q, storage);
         cfd_hip::AdjointArgs args = objective_args(kind, x_in, x_out, window);
@@ -839,10 +884,10 @@ struct cfd_lbm_solver {
         }
         DeviceBuffer<unsigned char> lam, gg, gp, acc;
         lam.allocate(pop_elems * real);
-        gg.allocate(cells * sizeof(double));
+        if (!grad_on_device) gg.allocate(cells * sizeof(double));
         gp.allocate(cells * sizeof(double));
         acc.allocate(sizeof(double));
-        args.grad_gamma = reinterpret_cast<double*>(gg.ptr);
+        args.grad_gamma = grad_on_device ? grad_gamma_out : reinterpret_cast<double*>(gg.ptr);
         args.grad_peak = reinterpret_cast<double*>(gp.ptr);
         const LaunchState st = state();
         adj->zero(st, args.grad_gamma, cells);
@@ -941,7 +986,7 @@ struct cfd_lbm_solver {
         d2h(&J, acc.ptr, sizeof(double));
         std::vector<double> peak(cells);
         d2h(peak.data(), gp.ptr, cells * sizeof(double));
-        if (grad_gamma_out) d2h(grad_gamma_out, gg.ptr, cells * sizeof(double));
+        if (grad_gamma_out && !grad_on_device) d2h(grad_gamma_out, gg.ptr, cells * sizeof(double));
         sync();
         (void)hipEventElapsedTime(&last_forward_ms, e0, e1);
         (void)hipEventElapsedTime(&last_reverse_ms, e1, e2);
@@ -975,6 +1020,59 @@ struct cfd_lbm_solver {
             if (!(g[i] >= 0.0F && g[i] <= 1.0F)) throw std::invalid_argument("porosity gamma must lie in [0, 1]");
         const float* parts[1] = {g};
         upload_real(gamma, parts, 1);
+    }
+
+    // set_porosity from device memory (cells floats on this solver's device): validated on the
+    // device; the field in use changes only when every value lies in [0, 1].
+    void set_porosity_device(const float* g) {
+        use_device();
+        if (!g) throw std::invalid_argument("set_porosity_device needs a device pointer (set_porosity(NULL) clears)");
+        const uint64_t blocks = (cells + 255) / 256;
+        DeviceBuffer<unsigned char> fresh, bad;
+        fresh.allocate(cells * real);
+        bad.allocate(sizeof(uint32_t));
+        zero_words(bad.ptr, 1);
+        porosity_from_device<<<static_cast<uint32_t>(blocks), 256, 0, stream>>>(
+            g, fresh.ptr, static_cast<uint32_t>(real), cells, reinterpret_cast<uint32_t*>(bad.ptr));
+        hip_check(hipGetLastError(), "porosity_from_device");
+        uint32_t n_bad = 0;
+        d2h(&n_bad, bad.ptr, sizeof(n_bad));
+        sync();
+        if (n_bad != 0) throw std::invalid_argument("porosity gamma must lie in [0, 1]");
+        std::swap(gamma.ptr, fresh.ptr);
+        std::swap(gamma.bytes, fresh.bytes);
+    }
+
+    // z-slab halo messages (the slab is this lattice, periodic in z; its neighbours are the slabs
+    // below and above).  Elements of one message, in population storage units.
+    size_t halo_message_bytes() {
+        if (q == 9) throw std::invalid_argument("halo messages cut a 3-D lattice in z (D3Q19 or D3Q27)");
+        return ops->halo_elems(state()) * elem;
+    }
+    void halo_checks() {
+        (void)halo_message_bytes();
+        if (bounded()) throw std::invalid_argument("halo messages need a lattice without flags (periodic slab)");
+        if (t == 0) throw std::invalid_argument("halo messages follow a step: no step has run");
+    }
+    // After a step: the up and down messages into device buffers of halo_message_bytes() each.
+    void halo_pack(void* up, void* down) {
+        use_device();
+        halo_checks();
+        if (!up || !down) throw std::invalid_argument("halo_pack needs two device buffers");
+        const LaunchState st = state();
+        const bool last_odd = ((t - 1) & 1U) != 0U;
+        ops->halo(st, last_odd, false, false, up);
+        ops->halo(st, last_odd, true, false, down);
+    }
+    // The up message of the slab below and the down message of the slab above, into this slab.
+    void halo_unpack(const void* from_below, const void* from_above) {
+        use_device();
+        halo_checks();
+        if (!from_below || !from_above) throw std::invalid_argument("halo_unpack needs two device buffers");
+        const LaunchState st = state();
+        const bool last_odd = ((t - 1) & 1U) != 0U;
+        ops->halo(st, last_odd, false, true, const_cast<void*>(from_below));
+        ops->halo(st, last_odd, true, true, const_cast<void*>(from_above));
     }
 
     void set_local_acceleration(const float* lx, const float* ly, const float* lz) {
@@ -1243,6 +1341,25 @@ int cfd_lbm_set_porosity(cfd_lbm_solver* s, const float* gamma, char* error, siz
     return guarded(error, cap, [&] { need(s).set_porosity(gamma); });
 }
 
+int cfd_lbm_set_porosity_device(cfd_lbm_solver* s, const float* gamma, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).set_porosity_device(gamma); });
+}
+
+int cfd_lbm_halo_message_bytes(cfd_lbm_solver* s, size_t* bytes, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        if (!bytes) throw std::invalid_argument("halo_message_bytes needs an output");
+        *bytes = need(s).halo_message_bytes();
+    });
+}
+
+int cfd_lbm_halo_pack(cfd_lbm_solver* s, void* up, void* down, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).halo_pack(up, down); });
+}
+
+int cfd_lbm_halo_unpack(cfd_lbm_solver* s, const void* from_below, const void* from_above, char* error, size_t cap) {
+    return guarded(error, cap, [&] { need(s).halo_unpack(from_below, from_above); });
+}
+
 int cfd_lbm_set_inlet_outlet(cfd_lbm_solver* s, float u_peak, float rho_out, char* error, size_t cap) {
     return guarded(error, cap, [&] {
         if (!std::isfinite(u_peak) || !(rho_out > 0.0F)) throw std::invalid_argument("need finite u_peak and rho_out > 0");
@@ -1399,6 +1516,14 @@ int cfd_lbm_objective_drag(cfd_lbm_solver* s, double* value, char* error, size_t
     });
 }
 
+int cfd_lbm_objective_run(cfd_lbm_solver* s, int objective_kind, uint64_t steps, uint64_t window, int x_in, int x_out,
+                          double* objective, char* error, size_t cap) {
+    return guarded(error, cap, [&] {
+        const double J = need(s).objective_run(objective_kind, steps, window, x_in, x_out);
+        if (objective) *objective = J;
+    });
+}
+
 int cfd_lbm_adjoint_run(cfd_lbm_solver* s, uint64_t steps, uint64_t window, uint64_t checkpoint_every, int x_in,
                         int x_out, double* objective, double* grad_gamma, double* grad_u_peak, char* error,
                         size_t cap) {
@@ -1406,6 +1531,17 @@ int cfd_lbm_adjoint_run(cfd_lbm_solver* s, uint64_t steps, uint64_t window, uint
         if (checkpoint_every < 1) throw std::invalid_argument("checkpoint interval must be >= 1");
         const double J = need(s).adjoint_run(cfd_hip::kPressureDrop, steps, window, checkpoint_every, 0, x_in, x_out,
                                              grad_gamma, grad_u_peak, nullptr);
+        if (objective) *objective = J;
+    });
+}
+
+int cfd_lbm_adjoint_run_binomial_device(cfd_lbm_solver* s, int objective_kind, uint64_t steps, uint64_t window,
+                                        uint64_t snapshots, int x_in, int x_out, double* objective,
+                                        double* grad_gamma, double* grad_u_peak, uint64_t* forward_steps, char* error,
+                                        size_t cap) {
+    return guarded(error, cap, [&] {
+        const double J = need(s).adjoint_run(objective_kind, steps, window, 0, snapshots, x_in, x_out, grad_gamma,
+                                             grad_u_peak, forward_steps, true);
         if (objective) *objective = J;
     });
 }

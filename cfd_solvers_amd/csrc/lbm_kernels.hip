@@ -534,6 +534,50 @@ __global__ __launch_bounds__(256) void probe_store(uint32_t* __restrict__ ids, c
     store_cell<Q, uint32_t, Odd, Bounded>(ids, g, p, n, x, y, z, 1U, fout, s);
 }
 
+// ------------------------------------------------------------------ z-slab halo messages
+// A lattice split into z-slabs runs the periodic kernel on each slab; the z wrap then puts the slots
+// a boundary cell shares with the neighbouring slab into the slab's own opposite face layer.  After
+// every step each slab sends 5 (D3Q19) / 9 (D3Q27) slot planes of those layers up and as many down,
+// and writes what it receives into the same slots (the receiver's slots sit where the sender's do):
+//   pair (i, i+1) with c_i,z = +1, top layer z = nz-1:    up: plane i+1 after an even step, i after an odd one
+//                                                        down: the other plane of the pair
+//   pair (i, i+1) with c_i,z = -1, bottom layer z = 0:     up: plane i after an even step, i+1 after an odd one
+//                                                        down: the other plane
+// (the slot ownership of esoteric_pull.hpp:349-382 across the cut; the crossing populations of
+// include/cfd/distributed/selective_halo.hpp:15-21).  Message element k*nx*ny + y*nx + x is crossing
+// pair k (in pair order) at (x, y).  With one slab that is its own neighbour, unpacking its own
+// messages changes nothing.
+template <int Q> __host__ __device__ constexpr int crossing_pairs() {
+    int n = 0;
+    for (int i = 1; i < Q; i += 2)
+        if (Lattice<Q>::cz[i] != 0) ++n;
+    return n;
+}
+
+template <int Q, class T>
+__global__ __launch_bounds__(256) void halo_planes(void* __restrict__ f, const Geometry g, uint32_t odd,
+                                                   uint32_t down, uint32_t unpack, T* __restrict__ msg) {
+    using L = Lattice<Q>;
+    constexpr uint32_t kPairs = static_cast<uint32_t>(crossing_pairs<Q>());
+    if constexpr (kPairs == 0) return;
+    const uint32_t layer = g.nx * g.ny;
+    const uint64_t id = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= static_cast<uint64_t>(kPairs) * layer) return;
+    const uint32_t k = static_cast<uint32_t>(id / layer), c = static_cast<uint32_t>(id % layer);
+    const uint32_t x = c % g.nx, y = c / g.nx;
+    int i = 1;
+    for (uint32_t seen = 0;; i += 2)
+        if (L::cz[i] != 0 && seen++ == k) break;
+    const bool top = L::cz[i] > 0;
+    // up, top layer: i+1 even / i odd; up, bottom layer: i even / i+1 odd; down: the other plane.
+    const bool second = top ? (odd == 0U) : (odd != 0U);
+    const int d = (second != (down != 0U)) ? i + 1 : i;
+    const uint32_t z = top ? g.nz - 1U : 0U;
+    T* slot = plane<T>(f, g, d) + row_base(g, z * g.ny + y) + x;
+    if (unpack) *slot = msg[id];
+    else msg[id] = *slot;
+}
+
 // ------------------------------------------------------------------ launchers
 inline void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -639,6 +683,17 @@ template <int Q, class T> struct Ops final : KernelOps {
         check(hipGetLastError(), "diagnostics_partial launch");
         hipLaunchKernelGGL(diagnostics_final, dim3(1), dim3(256), 0, st.stream, partial, st.geo.tiles, out);
         check(hipGetLastError(), "diagnostics_final launch");
+    }
+    size_t halo_elems(const LaunchState& st) const override {
+        return static_cast<size_t>(crossing_pairs<Q>()) * st.geo.nx * st.geo.ny;
+    }
+    void halo(const LaunchState& st, bool odd, bool down, bool unpack, void* msg) const override {
+        const uint64_t n = halo_elems(st);
+        if (n == 0) return;
+        const uint64_t blocks = (n + 255) / 256;
+        hipLaunchKernelGGL((halo_planes<Q, T>), dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, st.stream, st.f,
+                           st.geo, odd ? 1U : 0U, down ? 1U : 0U, unpack ? 1U : 0U, static_cast<T*>(msg));
+        check(hipGetLastError(), "halo_planes launch");
     }
     void links(const LaunchState& st, const uint8_t* flags, uint32_t* out, uint32_t* vel) const override {
         hipLaunchKernelGGL((build_links<Q>), grid_of(st.geo), block_of(st.geo), 0, st.stream, flags, st.geo, out,

@@ -10,6 +10,8 @@
 * ``rasterize``: cells whose centre lies within a sphere become walls.
 * ``porosity_halo``: a partial-bounce-back blend that ramps from ``gamma_max`` at a sphere's surface
   to zero ``width`` cells out (the halo of a porous coating around the solid cluster).
+* ``sphere_cluster_duct``: the flow-past-a-cluster case: an n^3 duct with inlet and outlet, 64
+  spheres of radius U[10, 30] (seed 5) centred in the middle 40 % of each axis, and their halo.
 
 Triangle surfaces (float64 arrays of shape (count, 3, 3): triangle, vertex a/b/c, x/y/z) feed the
 GPU voxeliser (``voxelize_surface`` / ``EsotericPullHipSolver.set_solid_surface``, the reference's
@@ -120,6 +122,22 @@ def porosity_halo(shape, centres, radii, width, gamma_max):
         sub = g[z0:z1 + 1, y0:y1 + 1, x0:x1 + 1]
         np.maximum(sub, val, out=sub)
     return g
+
+
+def sphere_cluster_duct(n, seed=5, count=64, r_min=10, r_max=30, halo_width=4.0, halo_max=0.5):
+    """Flat (cells,) arrays of the sphere-cluster duct: uint8 flags (walls, inlet x = 0, outlet
+    x = n-1, the spheres as walls), the bool body of the spheres (the drag body) and the float32
+    partial-bounce-back halo (zero outside fluid cells)."""
+    shape = (n, n, n)
+    lo, hi = 0.3 * n, 0.7 * n
+    centres, radii = random_spheres(shape, r_min, r_max, seed=seed, count=count, box=((lo, hi),) * 3,
+                                    periodic_x=False)
+    flags = channel_flags(n, n, n, inlet_outlet=True)
+    body = rasterize(shape, centres, radii, periodic_x=False)
+    flags[body] = WALL
+    halo = porosity_halo(shape, centres, radii, halo_width, halo_max)
+    halo[flags != 0] = 0.0
+    return flags.reshape(-1), body.reshape(-1), halo.reshape(-1)
 
 
 def read_ascii_stl(path):

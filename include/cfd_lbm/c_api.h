@@ -101,6 +101,11 @@ CFD_LBM_API int cfd_lbm_set_wall_velocity(cfd_lbm_solver* s, int index, float ux
                                           size_t error_capacity);
 /* Partial bounce-back porosity gamma in [0,1] per cell (cells values; NULL clears). */
 CFD_LBM_API int cfd_lbm_set_porosity(cfd_lbm_solver* s, const float* gamma, char* error, size_t error_capacity);
+/* set_porosity from DEVICE memory (cells floats on the solver's device), checked on the device: an
+ * error, and the field in use unchanged, unless every value lies in [0, 1].  No reference
+ * counterpart (the reference has no porosity field). */
+CFD_LBM_API int cfd_lbm_set_porosity_device(cfd_lbm_solver* s, const float* gamma, char* error,
+                                            size_t error_capacity);
 /* Inlet peak velocity and outlet density for CFD_LBM_VELOCITY_INLET / CFD_LBM_PRESSURE_OUTLET cells. */
 CFD_LBM_API int cfd_lbm_set_inlet_outlet(cfd_lbm_solver* s, float u_peak, float rho_out, char* error,
                                          size_t error_capacity);
@@ -172,6 +177,14 @@ CFD_LBM_API int cfd_lbm_set_drag_body(cfd_lbm_solver* s, const uint8_t* body, in
 /* Drag on the registered body at the current state (FP64): sum over fluid cells n and populations d
  * arriving from a body cell of -2 c_d[axis] f_d(n). */
 CFD_LBM_API int cfd_lbm_objective_drag(cfd_lbm_solver* s, double* value, char* error, size_t error_capacity);
+/* The objective of the adjoint entry points without the adjoint (no reference counterpart): runs
+ * `steps` steps in place from the current state and returns J, the mean of the objective over the
+ * last `window` states (objective_kind and x_in, x_out as for cfd_lbm_adjoint_run_binomial), equal
+ * bit for bit to the J those entry points return from the same state.  For callers that need J
+ * alone (line searches, finite differences, evaluation under torch.no_grad).  The state is left
+ * at time + steps. */
+CFD_LBM_API int cfd_lbm_objective_run(cfd_lbm_solver* s, int objective_kind, uint64_t steps, uint64_t window, int x_in,
+                                      int x_out, double* objective, char* error, size_t error_capacity);
 /* Discrete adjoint (no reference counterpart: the reference has no adjoint).  Runs `steps` steps
  * from the current state, J = mean of the pressure drop over the last `window` states, then the
  * transposed steps back with a snapshot every `checkpoint_every` steps (two-level: each step is
@@ -192,6 +205,13 @@ CFD_LBM_API int cfd_lbm_adjoint_run_binomial(cfd_lbm_solver* s, int objective_ki
                                              uint64_t snapshots, int x_in, int x_out, double* objective,
                                              double* grad_gamma, double* grad_u_peak, uint64_t* forward_steps,
                                              char* error, size_t error_capacity);
+/* cfd_lbm_adjoint_run_binomial with dJ/dgamma written into DEVICE memory (grad_gamma: cells
+ * doubles on the solver's device, 8-byte aligned, required) instead of host memory, so that a
+ * device-resident caller (torch tensors) makes no host round trip.  No reference counterpart. */
+CFD_LBM_API int cfd_lbm_adjoint_run_binomial_device(cfd_lbm_solver* s, int objective_kind, uint64_t steps,
+                                                    uint64_t window, uint64_t snapshots, int x_in, int x_out,
+                                                    double* objective, double* grad_gamma, double* grad_u_peak,
+                                                    uint64_t* forward_steps, char* error, size_t error_capacity);
 /* The binomial plan without a solver (host only, no device call): forward steps of a run of
  * `steps` steps with `snapshots` buffers from the closed form r*l - C(s+r, r-1) + 1, the
  * repetition number r, and the same count obtained by walking the plan. */
@@ -201,6 +221,22 @@ CFD_LBM_API int cfd_lbm_checkpoint_plan(uint64_t steps, uint64_t snapshots, uint
 /* GPU time of the last adjoint run's forward phase and reverse (recompute + transposed) phase. */
 CFD_LBM_API int cfd_lbm_adjoint_timing(cfd_lbm_solver* s, float* forward_ms, float* reverse_ms, char* error,
                                        size_t error_capacity);
+
+/* z-slab decomposition (include/cfd/solvers/lbm/distributed_pull.hpp:326-395 exchanges the
+ * crossing populations of include/cfd/distributed/selective_halo.hpp:15-21 between bricks).  A
+ * lattice cut into z-slabs runs one solver per slab, each created with its slab's nz and no flags
+ * (periodic in z: the z wrap puts the slots a face cell shares with the next slab into the slab's
+ * own opposite face layer).  After every step each slab packs an up message (to the slab above)
+ * and a down message (to the slab below), the messages are exchanged (RCCL send/recv between
+ * GPUs, or device copies on one GPU), and each slab unpacks the up message of the slab below and
+ * the down message of the slab above.  The slabs together then hold bit for bit the state of the
+ * whole periodic lattice.  A message is 5 (D3Q19) or 9 (D3Q27) planes of nx*ny populations in
+ * the storage's width; buffers are device memory on the solver's device.  A slab that is its own
+ * neighbour (one slab) receives its own messages, which changes nothing. */
+CFD_LBM_API int cfd_lbm_halo_message_bytes(cfd_lbm_solver* s, size_t* bytes, char* error, size_t error_capacity);
+CFD_LBM_API int cfd_lbm_halo_pack(cfd_lbm_solver* s, void* up, void* down, char* error, size_t error_capacity);
+CFD_LBM_API int cfd_lbm_halo_unpack(cfd_lbm_solver* s, const void* from_below, const void* from_above, char* error,
+                                    size_t error_capacity);
 
 /* Checkpoint / restart.  The file starts with the CFDCHK3 layout of DistributedPullBlock::
  * save_checkpoint (include/cfd/solvers/lbm/distributed_pull.hpp:176-203) for one brick covering

@@ -1,3 +1,4 @@
+import ctypes as C
 import os
 import sys
 from pathlib import Path
@@ -49,3 +50,48 @@ def hip():
 def report(name, **kv):
     """One flushed line per GPU case so a box that dies mid-run still names the case."""
     print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kv.items()), flush=True)
+
+
+class DeviceMemory:
+    """hipMalloc / hipMemcpy / hipFree of the HIP runtime the library itself loaded (resolved by
+    soname once the library is loaded), so that GPU tests need no second runtime (torch brings its
+    own).  Frees everything it allocated on free()."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self):
+        self.rt = C.CDLL("libamdhip64.so.7")
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.live = []
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        assert self.rt.hipMalloc(C.byref(p), max(int(nbytes), 1)) == 0
+        self.live.append(p.value)
+        return p.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        p = self.alloc(a.nbytes)
+        assert self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, self.H2D) == 0
+        return p
+
+    def download(self, p, shape, dtype):
+        out = np.empty(shape, dtype)
+        assert self.rt.hipMemcpy(out.ctypes.data, p, out.nbytes, self.D2H) == 0
+        return out
+
+    def free(self):
+        for p in self.live:
+            self.rt.hipFree(p)
+        self.live.clear()
+
+
+@pytest.fixture
+def dev(hip):
+    hip.load()
+    d = DeviceMemory()
+    yield d
+    d.free()

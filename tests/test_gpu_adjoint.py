@@ -232,6 +232,31 @@ def test_binomial_matches_two_level_bitwise(hip):
         assert g.time_step == T
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16s"])
+def test_objective_run_equals_adjoint_objective(hip, storage):
+    """cfd_lbm_objective_run (steps + objective, no adjoint) returns the adjoint runs' J bit for bit,
+    for the pressure drop and the drag, and leaves the same state."""
+    report("objective_run", storage=storage)
+    n = (16, 8, 8)
+    blend = 0.05 * design(duct(*n))
+    T, W = 40, 8
+    a = gpu_duct(hip, n, blend, storage=storage)
+    J, _, _, _ = a.adjoint_run_binomial(T, W, 3, "pressure_drop", 1, n[0] - 2, want_gamma=False)
+    b = gpu_duct(hip, n, blend, storage=storage)
+    assert b.objective_run(T, W, "pressure_drop", 1, n[0] - 2) == J
+    assert b.time_step == a.time_step == T
+    ma, mb = a.download_macroscopic(), b.download_macroscopic()
+    assert all(np.array_equal(getattr(ma, k), getattr(mb, k)) for k in ("rho", "ux", "uy", "uz"))
+    flags, body = cube_duct()
+    cb = 0.05 * design(flags)
+    c = gpu_cube(hip, cb, storage)
+    Jd, _, _, _ = c.adjoint_run_binomial(30, 6, 2, "drag", want_gamma=False)
+    d = gpu_cube(hip, cb, storage)
+    assert d.objective_run(30, 6, "drag") == Jd
+    with pytest.raises(RuntimeError, match="window"):
+        d.objective_run(5, 6, "drag")
+
+
 def test_drag_objective_matches_oracle(hip, oracle_mod):
     report("drag_objective")
     flags, _ = cube_duct()
@@ -317,19 +342,42 @@ gm = (blend - 1e-3 * 0.05 * v).astype(np.float32)
 with torch.no_grad():
     Jp = lbm_objective(torch.tensor(gp, device=dev), u_peak.detach(), prob).item()
     Jm = lbm_objective(torch.tensor(gm, device=dev), u_peak.detach(), prob).item()
+    J_nograd = lbm_objective(gamma, u_peak, prob).item()
+J_detached = lbm_objective(gamma.detach(), u_peak.detach(), prob)
+# A gamma outside [0, 1] is refused on the device and the field in use stays as it was.
+bad = torch.tensor(blend, device=dev)
+bad[5] = 1.5
+torch.cuda.synchronize()
+solver = prob.solver()
+try:
+    solver.set_porosity_device(bad.data_ptr())
+    refused = False
+except RuntimeError as e:
+    refused = "[0, 1]" in str(e)
+solver.initialize_uniform()
+J_after = solver.objective_run(prob.steps, prob.window, "pressure_drop", prob.x_in, 14)
+# The host path (a CPU gamma) gives the same J and gradient.
+gamma_cpu = torch.tensor(blend, requires_grad=True)
+J_cpu = lbm_objective(gamma_cpu, u_peak.detach().cpu(), prob)
+J_cpu.backward()
 print(json.dumps({
     "J": J.item(), "ref_J": ref_J, "grad_device": str(gamma.grad.device),
     "grad_err": float((gamma.grad.double().cpu().numpy() - ref_grad.reshape(-1).astype(np.float32)).__abs__().max()),
     "grad_scale": float(np.abs(ref_grad).max()), "du": u_peak.grad.item(), "ref_du": ref_du,
     "dJ": Jp - Jm, "directional": float(np.dot(ref_grad.reshape(-1), gp.astype(np.float64) - gm.astype(np.float64))),
+    "J_nograd": J_nograd, "J_detached": J_detached.item(), "detached_has_graph": J_detached.requires_grad,
+    "refused": refused, "J_after": J_after, "J_cpu": J_cpu.item(),
+    "cpu_grad_equal": bool(torch.equal(gamma_cpu.grad, gamma.grad.cpu())),
 }))
 """
 
 
 def test_torch_autograd_function():
-    """LbmObjective on CUDA tensors: J.backward() returns the adjoint gradients, and a directional
-    central difference of J along a random design perturbation matches <dJ/dgamma, v> (FP64
-    storage).  Runs in a child process that imports torch first, as a torch user would: the
+    """LbmObjective on CUDA tensors: J.backward() returns the adjoint gradients (computed into a device
+    tensor), a directional central difference of J along a random design perturbation matches
+    <dJ/dgamma, v> (FP64 storage), evaluations without a gradient run the objective alone and return
+    the same J, a gamma outside [0, 1] uploaded from device memory is refused, and CPU tensors give
+    the same J and gradient through the host path.  Runs in a child process that imports torch first, as a torch user would: the
     library then shares torch's HIP runtime (this process has already started the library's own)."""
     import json
     import subprocess
@@ -348,3 +396,7 @@ def test_torch_autograd_function():
     assert r["grad_err"] <= 1e-6 * r["grad_scale"]
     assert r["du"] == pytest.approx(r["ref_du"], rel=1e-6)
     assert r["dJ"] == pytest.approx(r["directional"], rel=1e-5)
+    # Objective-only evaluations (no grad mode, or no input requiring a gradient) return J bit for bit.
+    assert r["J_nograd"] == r["J"] and r["J_detached"] == r["J"] and not r["detached_has_graph"]
+    assert r["refused"] and r["J_after"] == r["J"]
+    assert r["J_cpu"] == r["J"] and r["cpu_grad_equal"]

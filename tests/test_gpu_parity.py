@@ -567,7 +567,7 @@ def test_smagorinsky_rejects_shifted_storage(hip):
 # ---------------------------------------------------------------------- set-up paths, invariants
 def test_initialize_fields_roundtrip_and_size_check(hip, oracle_mod):
     """initialize(initializer) with host fields (esoteric_pull.hpp:312-333): the fields read back
-    equal the oracle's bit for bit and the inputs to float rounding; wrong-sized inputs are
+    equal the oracle's bit for bit and the inputs to FP32 summation rounding; wrong-sized inputs are
     refused before anything crosses the ABI."""
     report("initialize_fields")
     n = (20, 14, 9)
@@ -581,7 +581,8 @@ def test_initialize_fields_roundtrip_and_size_check(hip, oracle_mod):
     gf = gpu_fields(g)
     assert_fields(gf, o.macroscopic())
     want = np.stack([rho, *u]).reshape(4, -1).astype(np.float64)
-    assert np.abs(gf - want).max() < 1e-7
+    # rho = sum of the 19 equilibrium populations in FP32: a few ulps of 1 away from the input
+    assert np.abs(gf - want).max() < 1e-6
     for bad in ((rho.reshape(-1)[:-1], *u), (1.0, *u), (rho, u[0], u[1], np.zeros(3, np.float32))):
         with pytest.raises(ValueError, match="size mismatch"):
             g.initialize_fields(*bad)
