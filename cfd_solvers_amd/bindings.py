@@ -85,6 +85,8 @@ _SIGNATURES = {
     "cfd_lbm_checkpoint_plan": [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_uint64)],
     "cfd_lbm_voxelize_surface": [_f64p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _f64p, _f64p, C.c_void_p],
+    "cfd_lbm_voxelize_surface_no_alloc": [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _f64p,
+                                          _f64p, C.c_void_p, C.c_void_p],
     "cfd_lbm_voxelize_timing": [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
     "cfd_lbm_transform_surface": [_f64p, C.c_size_t, _f64p, _f64p, C.c_double, _f64p],
     "cfd_lbm_set_solid_surface": [C.c_void_p, _f64p, C.c_size_t, _f64p, _f64p, C.c_int],
@@ -529,6 +531,18 @@ def voxelize_surface(triangles, shape, minimum=(0.0, 0.0, 0.0), maximum=(1.0, 1.
     load(library).call("cfd_lbm_voxelize_surface", t, len(t) // 9, nx, ny, nz, _vec3(minimum), _vec3(maximum),
                        mask.ctypes.data)
     return mask
+
+
+def voxelize_surface_no_alloc(triangles_device, count, shape, mask_device, minimum=(0.0, 0.0, 0.0),
+                              maximum=(1.0, 1.0, 1.0), stream=None, library=None):
+    """cfd_lbm_voxelize_surface_no_alloc: the mask of voxelize_surface written into caller-owned
+    device memory without device allocation.  triangles_device, mask_device: device addresses (ints)
+    of count x 9 float64 vertex coordinates and nx*ny*nz bytes; stream: a hipStream_t as an int
+    (None: the null stream).  Returns nothing; the mask is written when the call returns."""
+    nx, ny, nz = (int(n) for n in shape)
+    load(library).call("cfd_lbm_voxelize_surface_no_alloc", C.c_void_p(int(triangles_device)), int(count), nx, ny,
+                       nz, _vec3(minimum), _vec3(maximum), C.c_void_p(int(mask_device)),
+                       C.c_void_p(int(stream) if stream else None))
 
 
 def voxelize_timing(library=None):

@@ -1606,6 +1606,22 @@ int cfd_lbm_voxelize_surface(const double* triangles, size_t count, size_t nx, s
     });
 }
 
+int cfd_lbm_voxelize_surface_no_alloc(const double* triangles, size_t count, size_t nx, size_t ny, size_t nz,
+                                      const double* minimum, const double* maximum, uint8_t* mask, void* stream,
+                                      char* error, size_t cap) {
+    static_assert(CFD_LBM_VOXELIZE_MAX_TILES == cfd_hip::kVoxelizeMaxTiles, "tile limit");
+    static_assert(CFD_LBM_VOXELIZE_MAX_PAIRS == cfd_hip::kVoxelizeMaxPairs, "pair limit");
+    return guarded(error, cap, [&] {
+        if (!minimum || !maximum) throw std::invalid_argument("voxelization needs minimum and maximum");
+        cfd_hip::VoxelGridArgs g;
+        g.nx = nx;
+        g.ny = ny;
+        g.nz = nz;
+        for (int a = 0; a < 3; ++a) g.minimum[a] = minimum[a], g.maximum[a] = maximum[a];
+        cfd_hip::voxelize_surface_no_alloc(triangles, count, g, mask, static_cast<hipStream_t>(stream));
+    });
+}
+
 int cfd_lbm_voxelize_timing(double* bin_ms, double* lines_ms, uint64_t* pairs, char* error, size_t cap) {
     return guarded(error, cap, [&] {
         const cfd_hip::VoxelizeTiming t = cfd_hip::last_voxelize_timing();

@@ -25,9 +25,15 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+// Working memory of voxelize_surface_no_alloc, one copy per GPU for the life of the process:
+// header (pair total, non-finite flag), tile counts, tile offsets, tile cursors, pair list.
+// Host code never names it after set-up: its device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[cfd_hip::kVoxelizeScratchBytes];
 
 namespace cfd_hip {
 namespace {
@@ -269,6 +275,20 @@ __global__ void __launch_bounds__(kThreads) voxelize_lines(const Tri* tris, Line
     }
 }
 
+__global__ void __launch_bounds__(256) zero_words(uint32_t* p, uint32_t words) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) p[i] = 0U;
+}
+
+// flag = 1 when any of the `values` doubles is NaN or infinite.
+__global__ void __launch_bounds__(256) flag_non_finite(const double* values, uint64_t n, uint32_t* flag) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    bool bad = false;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        bad |= !isfinite(values[i]);
+    if (bad) atomicOr(flag, 1U);
+}
+
 __global__ void __launch_bounds__(256) solid_flags(uint8_t* mask, const uint8_t* base, size_t cells) {
     const size_t n = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= cells) return;
@@ -383,6 +403,95 @@ void voxelize_surface(const double* triangles, size_t count, const VoxelGridArgs
     t.pairs = total;
     g_last_timing = t;
     if (timing) *timing = t;
+}
+
+namespace {
+// Device address of `scratch` on each GPU, looked up once per device with that device current.
+std::mutex g_scratch_mutex;
+std::vector<unsigned char*> g_scratch_ptr;
+
+unsigned char* scratch_on_current_device() {
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr.size() <= static_cast<size_t>(dev)) g_scratch_ptr.resize(static_cast<size_t>(dev) + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p || reinterpret_cast<uintptr_t>(p) % 256 != 0) throw std::runtime_error("voxelize scratch is misaligned");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void voxelize_surface_no_alloc(const double* device_triangles, size_t count, const VoxelGridArgs& grid,
+                               uint8_t* device_mask, hipStream_t stream) {
+    if (!device_triangles || count == 0 || grid.nx == 0 || grid.ny == 0 || grid.nz == 0 ||
+        !(grid.maximum[0] > grid.minimum[0]) || !(grid.maximum[1] > grid.minimum[1]) ||
+        !(grid.maximum[2] > grid.minimum[2]))
+        throw std::invalid_argument("invalid voxelization inputs");
+    if (!device_mask) throw std::invalid_argument("voxelization needs an output mask");
+    if (reinterpret_cast<uintptr_t>(device_triangles) % alignof(double) != 0)
+        throw std::invalid_argument("device triangles must be 8-byte aligned doubles");
+    if (count > 0xFFFFFFFFull || grid.nx > 0xFFFFFFFFull || grid.ny > 0xFFFFFFFFull || grid.nz > 0xFFFFFFFFull)
+        throw std::invalid_argument("voxelization grid or surface too large");
+    const uint64_t tiles64 = ((grid.ny + kTileJ - 1) / kTileJ) * ((grid.nz + kTileK - 1) / kTileK);
+    if (tiles64 > kVoxelizeMaxTiles)
+        throw std::invalid_argument("voxelization grid has " + std::to_string(tiles64) +
+                                    " tiles of 8 x 8 x-lines; the no-allocation voxelizer takes at most " +
+                                    std::to_string(kVoxelizeMaxTiles));
+    LineGrid g{};
+    g.nx = static_cast<uint32_t>(grid.nx);
+    g.ny = static_cast<uint32_t>(grid.ny);
+    g.nz = static_cast<uint32_t>(grid.nz);
+    g.tiles_j = (g.ny + kTileJ - 1) / kTileJ;
+    g.tiles_k = (g.nz + kTileK - 1) / kTileK;
+    g.packed = (grid.nx % 8 == 0 && reinterpret_cast<uintptr_t>(device_mask) % 8 == 0) ? 1U : 0U;
+    g.minx = grid.minimum[0];
+    g.miny = grid.minimum[1];
+    g.minz = grid.minimum[2];
+    g.dx = (grid.maximum[0] - grid.minimum[0]) / static_cast<double>(grid.nx);
+    g.dy = (grid.maximum[1] - grid.minimum[1]) / static_cast<double>(grid.ny);
+    g.dz = (grid.maximum[2] - grid.minimum[2]) / static_cast<double>(grid.nz);
+    const uint32_t tiles = g.tiles_j * g.tiles_k;
+    const uint32_t n = static_cast<uint32_t>(count);
+
+    unsigned char* base = scratch_on_current_device();
+    auto* total = reinterpret_cast<unsigned long long*>(base);
+    auto* bad = reinterpret_cast<uint32_t*>(base + 8);
+    auto* counts = reinterpret_cast<uint32_t*>(base + kVoxelizeCountOffset);
+    auto* cursor = reinterpret_cast<uint32_t*>(base + kVoxelizeCursorOffset);
+    auto* offsets = reinterpret_cast<uint32_t*>(base + kVoxelizeTileOffset);
+    auto* list = reinterpret_cast<uint32_t*>(base + kVoxelizeListOffset);
+    const Tri* tris = reinterpret_cast<const Tri*>(device_triangles);
+
+    // The previous call's header, counts and cursors are cleared by a kernel (no memset node).
+    const uint32_t words = static_cast<uint32_t>(kVoxelizeTileOffset / 4);
+    zero_words<<<(words + 255) / 256, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(base), words);
+    check(hipGetLastError(), "zero_words");
+    const uint64_t values = static_cast<uint64_t>(count) * 9;
+    const uint32_t check_blocks = static_cast<uint32_t>(std::min<uint64_t>((values + 255) / 256, 1024));
+    flag_non_finite<<<check_blocks, 256, 0, stream>>>(device_triangles, values, bad);
+    check(hipGetLastError(), "flag_non_finite");
+    const uint32_t tri_blocks = (n + 255) / 256;
+    bin_count<<<tri_blocks, 256, 0, stream>>>(tris, n, g, counts);
+    check(hipGetLastError(), "bin_count");
+    bin_scan<<<1, kScanThreads, 0, stream>>>(counts, tiles, offsets, total);
+    check(hipGetLastError(), "bin_scan");
+    unsigned long long header[2] = {0, 0};  // pair total, non-finite flag (low word)
+    check(hipMemcpyAsync(header, base, sizeof(header), hipMemcpyDeviceToHost, stream), "copy pair total");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if ((header[1] & 0xFFFFFFFFull) != 0) throw std::invalid_argument("surface vertices must be finite");
+    if (header[0] > kVoxelizeMaxPairs)
+        throw std::invalid_argument("surface binned to " + std::to_string(header[0]) +
+                                    " (triangle, tile) pairs; the no-allocation voxelizer takes at most " +
+                                    std::to_string(kVoxelizeMaxPairs));
+    bin_fill<<<tri_blocks, 256, 0, stream>>>(tris, n, g, offsets, cursor, list);
+    check(hipGetLastError(), "bin_fill");
+    voxelize_lines<<<tiles, kThreads, 0, stream>>>(tris, g, offsets, list, device_mask);
+    check(hipGetLastError(), "voxelize_lines");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
 void solid_to_flags(uint8_t* mask, const uint8_t* base, size_t cells, hipStream_t stream) {

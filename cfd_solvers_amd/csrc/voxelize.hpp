@@ -15,6 +15,18 @@ struct VoxelGridArgs {
     double minimum[3] = {0, 0, 0}, maximum[3] = {0, 0, 0};
 };
 
+// voxelize_surface_no_alloc's working memory: a static device array per GPU (`scratch` in
+// voxelize.hip) holding a 256-byte header (pair total as 64 bits, non-finite flag), the tile
+// counts and cursors (kVoxelizeMaxTiles words each), the tile offsets (kVoxelizeMaxTiles + 1
+// words) and the pair list (kVoxelizeMaxPairs words): 17 MiB.
+constexpr uint32_t kVoxelizeMaxTiles = 1U << 16;           // tiles of 8 x 8 x-lines: (ny+7)/8 * (nz+7)/8
+constexpr uint64_t kVoxelizeMaxPairs = uint64_t{1} << 22;  // (triangle, tile) pairs
+constexpr size_t kVoxelizeCountOffset = 256;
+constexpr size_t kVoxelizeCursorOffset = kVoxelizeCountOffset + 4 * size_t{kVoxelizeMaxTiles};
+constexpr size_t kVoxelizeTileOffset = kVoxelizeCursorOffset + 4 * size_t{kVoxelizeMaxTiles};
+constexpr size_t kVoxelizeListOffset = kVoxelizeTileOffset + ((4 * (size_t{kVoxelizeMaxTiles} + 1) + 255) / 256) * 256;
+constexpr size_t kVoxelizeScratchBytes = kVoxelizeListOffset + 4 * kVoxelizeMaxPairs;
+
 // Per-stage device times of the last voxelize_surface call on this thread (HIP events, ms):
 // binning (count + scan + fill) and the line kernel.
 struct VoxelizeTiming {
@@ -32,6 +44,15 @@ void validate_voxelization(const double* triangles, size_t count, const VoxelGri
 // inputs voxelize.cpp:5 rejects and for non-finite vertex coordinates.
 void voxelize_surface(const double* triangles, size_t count, const VoxelGridArgs& grid, uint8_t* device_mask,
                       hipStream_t stream, VoxelizeTiming* timing = nullptr);
+
+// voxelize_surface without device allocation: `device_triangles` (count x 9 doubles) and
+// `device_mask` are caller-owned device memory, the working memory is the static per-GPU array
+// above, and the work runs on `stream` (the call returns when the mask is written).  Throws
+// std::invalid_argument for what voxelize_surface rejects (non-finite vertices are found on the
+// device) and for grids of more than kVoxelizeMaxTiles tiles or surfaces of more than
+// kVoxelizeMaxPairs (triangle, tile) pairs.  Calls that overlap on one GPU race on the array.
+void voxelize_surface_no_alloc(const double* device_triangles, size_t count, const VoxelGridArgs& grid,
+                               uint8_t* device_mask, hipStream_t stream);
 
 // Timing of the calling thread's last voxelize_surface call.
 VoxelizeTiming last_voxelize_timing();

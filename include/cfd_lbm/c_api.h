@@ -263,6 +263,21 @@ CFD_LBM_API int cfd_lbm_load_checkpoint(cfd_lbm_solver* s, const char* path, cha
 CFD_LBM_API int cfd_lbm_voxelize_surface(const double* triangles, size_t count, size_t nx, size_t ny, size_t nz,
                                          const double* minimum, const double* maximum, uint8_t* mask, char* error,
                                          size_t error_capacity);
+/* cfd_lbm_voxelize_surface without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  triangles (count x 9 doubles, 8-byte aligned) and mask
+ * (nx*ny*nz bytes) are caller-owned DEVICE memory; stream is a hipStream_t (NULL: the null stream).
+ * The working memory is a static device array of 17 MiB per GPU, held until the process ends; it
+ * limits the grid to CFD_LBM_VOXELIZE_MAX_TILES tiles of 8 x 8 x-lines, ((ny+7)/8)*((nz+7)/8), and
+ * the surface to CFD_LBM_VOXELIZE_MAX_PAIRS (triangle, tile) pairs; larger inputs are rejected with
+ * an error.  The call returns when the mask is written.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other threads, other streams) race.  The mask equals
+ * cfd_lbm_voxelize_surface's bit for bit; it records no timing. */
+#define CFD_LBM_VOXELIZE_MAX_TILES 65536u
+#define CFD_LBM_VOXELIZE_MAX_PAIRS 4194304u
+CFD_LBM_API int cfd_lbm_voxelize_surface_no_alloc(const double* triangles, size_t count, size_t nx, size_t ny,
+                                                  size_t nz, const double* minimum, const double* maximum,
+                                                  uint8_t* mask, void* stream, char* error, size_t error_capacity);
 /* GPU time of the calling thread's last voxelisation: binning of triangles to line tiles, the
  * line kernel, and the number of (triangle, tile) pairs binned. */
 CFD_LBM_API int cfd_lbm_voxelize_timing(double* bin_ms, double* lines_ms, uint64_t* pairs, char* error,
