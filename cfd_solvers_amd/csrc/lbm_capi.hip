@@ -1061,8 +1061,7 @@ struct cfd_lbm_solver {
         if (!up || !down) throw std::invalid_argument("halo_pack needs two device buffers");
         const LaunchState st = state();
         const bool last_odd = ((t - 1) & 1U) != 0U;
-        ops->halo(st, last_odd, false, false, up);
-        ops->halo(st, last_odd, true, false, down);
+        ops->halo(st, last_odd, false, up, down);
     }
     // The up message of the slab below and the down message of the slab above, into this slab.
     void halo_unpack(const void* from_below, const void* from_above) {
@@ -1071,8 +1070,7 @@ struct cfd_lbm_solver {
         if (!from_below || !from_above) throw std::invalid_argument("halo_unpack needs two device buffers");
         const LaunchState st = state();
         const bool last_odd = ((t - 1) & 1U) != 0U;
-        ops->halo(st, last_odd, false, true, const_cast<void*>(from_below));
-        ops->halo(st, last_odd, true, true, const_cast<void*>(from_above));
+        ops->halo(st, last_odd, true, const_cast<void*>(from_below), const_cast<void*>(from_above));
     }
 
     void set_local_acceleration(const float* lx, const float* ly, const float* lz) {

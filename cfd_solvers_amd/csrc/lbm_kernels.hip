@@ -554,15 +554,22 @@ template <int Q> __host__ __device__ constexpr int crossing_pairs() {
     return n;
 }
 
-template <int Q, class T>
+// One launch covers both messages: elements [0, E) of the up message, then [E, 2E) of the down one.
+// Each thread moves V consecutive elements of a row (V = 16 bytes' worth when nx allows 16-byte
+// aligned chunks, else 1): rows are contiguous in the population planes and in the message alike.
+template <int Q, class T, int V>
 __global__ __launch_bounds__(256) void halo_planes(void* __restrict__ f, const Geometry g, uint32_t odd,
-                                                   uint32_t down, uint32_t unpack, T* __restrict__ msg) {
+                                                   uint32_t unpack, T* __restrict__ up_msg, T* __restrict__ down_msg) {
     using L = Lattice<Q>;
     constexpr uint32_t kPairs = static_cast<uint32_t>(crossing_pairs<Q>());
     if constexpr (kPairs == 0) return;
     const uint32_t layer = g.nx * g.ny;
-    const uint64_t id = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (id >= static_cast<uint64_t>(kPairs) * layer) return;
+    const uint64_t per = static_cast<uint64_t>(kPairs) * layer / V;  // chunks per message
+    const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= 2U * per) return;
+    const uint32_t down = gid >= per ? 1U : 0U;
+    const uint64_t id = (down ? gid - per : gid) * V;  // first element of the chunk
+    T* __restrict__ msg = down ? down_msg : up_msg;
     const uint32_t k = static_cast<uint32_t>(id / layer), c = static_cast<uint32_t>(id % layer);
     const uint32_t x = c % g.nx, y = c / g.nx;
     int i = 1;
@@ -574,8 +581,17 @@ __global__ __launch_bounds__(256) void halo_planes(void* __restrict__ f, const G
     const int d = (second != (down != 0U)) ? i + 1 : i;
     const uint32_t z = top ? g.nz - 1U : 0U;
     T* slot = plane<T>(f, g, d) + row_base(g, z * g.ny + y) + x;
-    if (unpack) *slot = msg[id];
-    else msg[id] = *slot;
+    if constexpr (V == 1) {
+        if (unpack) *slot = msg[id];
+        else msg[id] = *slot;
+    } else {
+        using Chunk = unsigned int __attribute__((ext_vector_type(4)));
+        static_assert(V * sizeof(T) == sizeof(Chunk), "16-byte chunks");
+        Chunk* a = reinterpret_cast<Chunk*>(slot);
+        Chunk* b = reinterpret_cast<Chunk*>(msg + id);
+        if (unpack) *a = *b;
+        else *b = *a;
+    }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -687,12 +703,25 @@ template <int Q, class T> struct Ops final : KernelOps {
     size_t halo_elems(const LaunchState& st) const override {
         return static_cast<size_t>(crossing_pairs<Q>()) * st.geo.nx * st.geo.ny;
     }
-    void halo(const LaunchState& st, bool odd, bool down, bool unpack, void* msg) const override {
-        const uint64_t n = halo_elems(st);
+    void halo(const LaunchState& st, bool odd, bool unpack, void* up, void* down) const override {
+        const uint64_t n = 2U * halo_elems(st);
         if (n == 0) return;
-        const uint64_t blocks = (n + 255) / 256;
-        hipLaunchKernelGGL((halo_planes<Q, T>), dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, st.stream, st.f,
-                           st.geo, odd ? 1U : 0U, down ? 1U : 0U, unpack ? 1U : 0U, static_cast<T*>(msg));
+        // 16-byte chunks when every row starts 16-byte aligned in the planes (the plane bases are
+        // 256-byte aligned and a row starts nx elements after the previous one) and in the
+        // messages (both 16-byte aligned).
+        constexpr uint32_t kV = 16U / static_cast<uint32_t>(sizeof(T));
+        const bool wide = kV > 1 && (st.geo.nx % kV) == 0 && (reinterpret_cast<uintptr_t>(st.f) % 16U) == 0 &&
+                          (reinterpret_cast<uintptr_t>(up) % 16U) == 0 && (reinterpret_cast<uintptr_t>(down) % 16U) == 0 &&
+                          (st.geo.stride % kV) == 0 && (st.geo.row_shift == 31 || (st.geo.block_elems % kV) == 0);
+        const uint64_t threads = wide ? n / kV : n;
+        const uint64_t blocks = (threads + 255) / 256;
+        const dim3 grid(static_cast<uint32_t>(blocks));
+        if (wide)
+            hipLaunchKernelGGL((halo_planes<Q, T, kV>), grid, dim3(256), 0, st.stream, st.f, st.geo, odd ? 1U : 0U,
+                               unpack ? 1U : 0U, static_cast<T*>(up), static_cast<T*>(down));
+        else
+            hipLaunchKernelGGL((halo_planes<Q, T, 1>), grid, dim3(256), 0, st.stream, st.f, st.geo, odd ? 1U : 0U,
+                               unpack ? 1U : 0U, static_cast<T*>(up), static_cast<T*>(down));
         check(hipGetLastError(), "halo_planes launch");
     }
     void links(const LaunchState& st, const uint8_t* flags, uint32_t* out, uint32_t* vel) const override {

@@ -43,9 +43,9 @@ struct KernelOps {
     virtual void index_map(const LaunchState& st, bool odd, bool bounded, const uint8_t* flags,
                            const uint32_t* links, uint32_t* ids, int64_t* load) const = 0;
     // z-slab halo messages: elements of one message (crossing pairs x nx x ny), and the pack
-    // (unpack = false) or unpack of the up (down = false) or down message after a step of parity odd.
+    // (unpack = false) or unpack of the up and down messages after a step of parity odd.
     virtual size_t halo_elems(const LaunchState& st) const = 0;
-    virtual void halo(const LaunchState& st, bool odd, bool down, bool unpack, void* msg) const = 0;
+    virtual void halo(const LaunchState& st, bool odd, bool unpack, void* up, void* down) const = 0;
     // out: per-cell link descriptor; vel: (q-1+9)/10 words per cell of wall-velocity indices
     virtual void links(const LaunchState& st, const uint8_t* flags, uint32_t* out, uint32_t* vel) const = 0;
 };
