@@ -29,12 +29,26 @@ _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 
 
+REF_CFD_BENCH = HERE / "_ref" / "cfd-bench-hip"
+
+
 def build(reference: str | None = None, quiet: bool = True) -> None:
-    """Run oracle/Makefile (the restatement always; _ref only where the reference tree exists)."""
+    """Run oracle/Makefile (the restatement always; _ref only where the reference tree exists).
+    With the reference tree and the HIP library present, also builds _ref/cfd-bench-hip: the
+    reference's apps/cfd-bench.cpp with INTEGRATION.md's --backend hip insertions, linked to the
+    library (oracle/integration.py)."""
     cmd = ["make", "-C", str(HERE), "all"]
     if reference:
         cmd.append(f"REF={reference}")
     subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL if quiet else None)
+    if reference and (Path(reference) / "apps" / "cfd-bench.cpp").exists():
+        from . import integration
+
+        if (integration.HIP_LIB_DIR / "libcfd_lbm_hip.so").exists():
+            (HERE / "_ref").mkdir(exist_ok=True)
+            r = integration.compile_cfd_bench(reference, REF_CFD_BENCH)
+            if r.returncode != 0:
+                raise RuntimeError("building the reference's cfd-bench with --backend hip failed:\n" + r.stderr[-2000:])
 
 
 _lib = None

@@ -41,6 +41,33 @@ def test_cfd_bench_json(lattice, storage):
     assert r["mass_error_rel"] < 1e-5
 
 
+REF_BENCH = ROOT / "oracle" / "_ref" / "cfd-bench-hip"
+
+
+@pytest.mark.skipif(not REF_BENCH.exists(), reason="the reference's patched cfd-bench was not built (no reference tree)")
+@pytest.mark.parametrize("lattice", ["d3q19", "d3q27", "d2q9"])
+def test_reference_cfd_bench_drives_the_hip_backend(lattice):
+    """The reference's own apps/cfd-bench.cpp with INTEGRATION.md's three insertions (built by
+    oracle.build into oracle/_ref/) runs --backend hip on this GPU: the reference's output schema,
+    backend "hip", and the same mass error, bit for bit, as this repository's cfd-bench on the same
+    run (both step the FP32 lattice through the same library from Taylor-Green 0.02)."""
+    report("reference_cfd_bench", lattice=lattice)
+    nz = 1 if lattice == "d2q9" else 32
+    args = ["--lattice", lattice, "--backend", "hip", "--nx", 32, "--ny", 32, "--nz", nz, "--warmup", 2,
+            "--steps", 10, "--json"]
+    ref = run([REF_BENCH, *args])
+    assert ref.returncode == 0, ref.stderr
+    r = json.loads(ref.stdout.strip().splitlines()[-1])
+    ours = run([BENCH, *args])
+    assert ours.returncode == 0, ours.stderr
+    o = json.loads(ours.stdout.strip().splitlines()[-1])
+    report("reference_cfd_bench_result", device=r["device"], mass_error_rel=r["mass_error_rel"])
+    assert r["schema"] == "cfd_solvers.benchmark.v1" and r["backend"] == "hip" and r["streaming"] == "in-place"
+    assert r["q"] == {"d3q19": 19, "d3q27": 27, "d2q9": 9}[lattice] and r["steps"] == 10
+    assert "gfx950" in r["device"]
+    assert r["mass_error_abs"] == o["mass_error_abs"] and r["mass_error_rel"] < 1e-5
+
+
 def test_cfd_bench_csv_and_rejections():
     out = run([BENCH, "--lattice", "d3q19", "--backend", "hip", "--nx", 32, "--ny", 32, "--nz", 32, "--warmup", 1,
                "--steps", 3, "--csv"])
