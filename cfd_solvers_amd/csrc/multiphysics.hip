@@ -28,6 +28,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -47,6 +48,16 @@ constexpr int kReduceBlocks = 1024;
 
 __device__ __forceinline__ uint32_t at(uint32_t x, uint32_t y, uint32_t z, const Dims& d) {
     return (z * d.ny + y) * d.nx + x;
+}
+
+// Element i of a field plane, addressed as the plane's uniform base plus the 32-bit byte offset
+// 4i: the global_load/store "saddr" form, one VGPR per address instead of a 64-bit pair.  Planes
+// hold at most 2^30 cells (checked when a module is created), so the offset fits.
+__device__ __forceinline__ float ld(const float* base, uint32_t i) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(base) + (i << 2));
+}
+__device__ __forceinline__ float& el(float* base, uint32_t i) {
+    return *reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(base) + (i << 2));
 }
 
 // A cell of the stencil kernels and its periodic neighbours along +-x, +-y, +-z (linear indices).
@@ -97,21 +108,233 @@ __device__ __forceinline__ void for_stencil_cells(const Dims& d, F&& f) {
     }
 }
 
-// Rows per thread of each stencil kernel, the fastest of 1, 2, 4 and 8 at 256^3 (kernel times
-// with 1 / 2 / 4 / 8 rows: thermal step 198 / 207 / 215 / 238 us, fill advection 88 / 80 / 79 /
-// 88 us, interface geometry 125 / 124 / 120 / 134 us, Q-criterion 89 / 70 / 60 / 60 us).  The
-// D3Q7 step streams 14 planes of its own and gains nothing from sharing rows.
+// Rows per thread of the kernels on the 3-D launch, the fastest of 1, 2, 4 and 8 at 256^3 (1 / 2 /
+// 4 / 8 rows: thermal step 198 / 207 / 215 / 238 us, interface geometry 125 / 124 / 120 / 134 us,
+// surface tension with the geometry 241 / 231 / 232 / 252 us).  The D3Q7 step reads each of its 7
+// planes once, from a different neighbour per plane, so it has no neighbour reuse to exploit.
 constexpr uint32_t kRowsThermal = 1;
-constexpr uint32_t kRowsAdvect = 4;
 constexpr uint32_t kRowsGeometry = 4;
 constexpr uint32_t kRowsTension = 2;
-constexpr uint32_t kRowsQ = 4;
 // The per-XCD row-block remap, where it measured faster at 256^3 (off / on: interface geometry
-// 123.8 / 118.3 us, Q-criterion 61.2 / 58.7 us; thermal step 195.1 / 207.6 us, fill advection
-// 79.5 / 81.9 us, surface tension 233 / 245 us).  It removes the re-fetched halo rows (Q-criterion
-// reads 18.1 B per cell without it, 12 algorithmic) but spreads the XCDs' streams over the plane.
+// 123.8 / 118.3 us; thermal step 195.1 / 207.6 us, surface tension 233 / 245 us).
 constexpr bool kRemapGeometry = true;
-constexpr bool kRemapQ = true;
+
+// ------------------------------------------------------------------ z-marching stencil tiles
+// Fill advection and the Q-criterion read three or four fields at their neighbours.  They march
+// along z through a tile of kTileX x (kTileWaves * R) cells: each thread keeps the values of its R
+// cells at z-1, z and z+1 in registers, and the block stages plane z with a one-cell halo in x and
+// y in LDS for the x and y neighbours.  Every input value is then read from memory about once
+// (the tile's halo and the plane below and above each z chunk come on top) instead of once per
+// neighbour that uses it.  The loads of plane z+1+kPrefetch are issued before the arithmetic of
+// plane z, into a ring of register sets that is never copied (a copy would wait for the loads in
+// flight).  The LDS tile is double-buffered, so one barrier per plane suffices: a thread writes
+// buffer b again only after the barrier of the plane in between, which every thread reaches after
+// its reads of buffer b.  At 256^3 the tiles take fill advection from 82.2 to 75.9 us and the
+// Q-criterion from 59.7 to 50.9 us (rows per thread 1 of 1, 2, 4; 2048 of 512-4096 blocks;
+// prefetch 1 of 1, 2).  The normals (one field; three IEEE divisions and a square root per cell),
+// the curvature (already at the copy rate of its 18 B per cell) and the surface tension measured
+// 5-24 % slower on tiles and keep the 3-D launch.
+constexpr uint32_t kTileX = 64;     // one wavefront per tile row
+constexpr uint32_t kTileWaves = 4;  // block = kTileX x kTileWaves threads
+// Blocks a launch aims for (tiles per plane x z chunks): eight 256-thread blocks on each CU.
+constexpr uint32_t kZBlocks = 2048;
+// Planes a thread's loads run ahead of the plane it computes (1 or 2).
+constexpr uint32_t kPrefetch = 1;
+constexpr uint32_t kRowsAdvect = 1;
+constexpr uint32_t kRowsQ = 1;
+
+template <int NF> struct Inputs {
+    const float* p[NF];
+};
+// The values a 7-point stencil reads around one cell, per input field.
+template <int NF> struct Nbr {
+    float c[NF], xm[NF], xp[NF], ym[NF], yp[NF], zm[NF], zp[NF];
+};
+
+constexpr int bits_below(uint32_t mask, int f) {
+    int k = 0;
+    for (int i = 0; i < f; ++i) k += (mask >> i) & 1U;
+    return k;
+}
+
+// ZM: fields whose z neighbours the stencil reads; XYM: fields whose x or y neighbours it reads
+// (staged in LDS).  Every field's own value at the cell is available.  A block also reads the
+// planes just below and above its z chunk, so no kernel may write its own inputs.  op(nbr, n) runs once per
+// cell n of the tile.  Launch: block (kTileX, kTileWaves), grid (tiles in x, tiles in y, z chunks
+// of zchunk planes).  Consecutive blocks of the flattened grid go to one XCD (when the block count
+// is a multiple of 8), so a tile's halo rows mostly come from its own XCD's L2.
+template <int NF, uint32_t ZM, uint32_t XYM, uint32_t R, class Op>
+__device__ __forceinline__ void zmarch(const Inputs<NF>& in, const Dims& d, uint32_t zchunk, Op&& op) {
+    constexpr uint32_t TY = kTileWaves * R;
+    constexpr uint32_t W = kTileX + 2;
+    constexpr int NXY = bits_below(XYM, NF);
+    static_assert(2 * kTileX + 2 * TY <= kTileX * kTileWaves, "one halo value per thread");
+    __shared__ float tile[2][NXY > 0 ? NXY : 1][(TY + 2) * W];
+
+    uint32_t b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const uint32_t total = gridDim.x * gridDim.y * gridDim.z;
+    if ((total & 7U) == 0U) b = (b & 7U) * (total >> 3) + (b >> 3);
+    const uint32_t bx = b % gridDim.x;
+    b /= gridDim.x;
+    const uint32_t by = b % gridDim.y;
+    const uint32_t bz = b / gridDim.y;
+    const uint32_t z0 = bz * zchunk;
+    if (z0 >= d.nz) return;  // the whole block
+    const uint32_t z1 = min(z0 + zchunk, d.nz);
+    const uint32_t tx = threadIdx.x, ty = threadIdx.y;
+    const uint32_t x0 = bx * kTileX, y0 = by * TY;
+    const uint32_t w = min(kTileX, d.nx - x0), h = min(TY, d.ny - y0);
+    const uint32_t plane = d.nx * d.ny;
+
+    uint32_t own[R];
+    bool rin[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        rin[r] = tx < w && ty + kTileWaves * r < h;
+        own[r] = (y0 + ty + kTileWaves * r) * d.nx + x0 + tx;
+    }
+    // Halo duty: threads 0..63 the row below the tile, 64..127 the row above, then the column left
+    // and the column right of the tile (all wrapped).  The row above sits right after the tile's
+    // last row and the right column right after its last column, also in partial tiles.
+    int hslot = -1;
+    uint32_t hoff = 0;
+    if constexpr (NXY > 0) {
+        const uint32_t t = ty * kTileX + tx;
+        if (t < kTileX) {
+            if (t < w) {
+                hoff = (y0 == 0U ? d.ny - 1U : y0 - 1U) * d.nx + x0 + t;
+                hslot = static_cast<int>(1U + t);
+            }
+        } else if (t < 2U * kTileX) {
+            const uint32_t u = t - kTileX;
+            if (u < w) {
+                hoff = (y0 + h == d.ny ? 0U : y0 + h) * d.nx + x0 + u;
+                hslot = static_cast<int>((h + 1U) * W + 1U + u);
+            }
+        } else if (t < 2U * kTileX + TY) {
+            const uint32_t u = t - 2U * kTileX;
+            if (u < h) {
+                hoff = (y0 + u) * d.nx + (x0 == 0U ? d.nx - 1U : x0 - 1U);
+                hslot = static_cast<int>((1U + u) * W);
+            }
+        } else if (t < 2U * kTileX + 2U * TY) {
+            const uint32_t u = t - 2U * kTileX - TY;
+            if (u < h) {
+                hoff = (y0 + u) * d.nx + (x0 + w == d.nx ? 0U : x0 + w);
+                hslot = static_cast<int>((1U + u) * W + w + 1U);
+            }
+        }
+    }
+
+    // Ring of S register sets: plane z0 + j lives in set j % S.  Computing plane z reads sets
+    // z-1, z, z+1 and loads plane z + D + 1 into the set of plane z - 2; the ring is never copied,
+    // so loads stay in flight until the plane they fetch is used, D steps later.
+    constexpr uint32_t S = kPrefetch + 3;
+    static_assert(S == 4 || S == 5, "the z loop below is unrolled for rings of 4 or 5 sets");
+    float v[S][NF][R];
+    float hv[S][NXY > 0 ? NXY : 1];
+    auto load_plane = [&](uint32_t set, uint32_t zz) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r)
+                if (rin[r]) v[set][f][r] = ld(in.p[f], zz * plane + own[r]);
+    };
+    auto load_halo = [&](uint32_t set, uint32_t zz) {
+        if constexpr (NXY > 0) {
+            if (hslot < 0) return;
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if ((XYM >> f) & 1U) hv[set][bits_below(XYM, f)] = ld(in.p[f], zz * plane + hoff);
+        }
+    };
+#pragma unroll
+    for (uint32_t s = 0; s < S; ++s) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) v[s][f][r] = 0.0F;
+#pragma unroll
+        for (int k = 0; k < (NXY > 0 ? NXY : 1); ++k) hv[s][k] = 0.0F;
+    }
+    // Prologue: planes z0 - 1 .. z0 + D and the halos of planes z0 .. z0 + D - 1 (those up to z1).
+    load_plane(S - 1, z0 == 0U ? d.nz - 1U : z0 - 1U);
+    uint32_t za = z0;  // wrapped plane index of the next load
+#pragma unroll
+    for (uint32_t j = 0; j <= kPrefetch; ++j) {
+        if (z0 + j <= z1) {
+            load_plane(j, za);
+            if (j < kPrefetch && z0 + j < z1) load_halo(j, za);
+        }
+        za = za + 1U == d.nz ? 0U : za + 1U;
+    }
+    // za is now plane z0 + D + 1 (wrapped); zh the plane of the next halo load, z0 + D.
+    uint32_t zh = za == 0U ? d.nz - 1U : za - 1U;
+
+    auto step = [&](auto K, uint32_t z) {
+        constexpr uint32_t k = decltype(K)::value;
+        constexpr uint32_t sp = (k + S - 1) % S, sc = k, sn = (k + 1) % S, sl = (k + kPrefetch + 1) % S;
+        constexpr uint32_t hs = (k + kPrefetch) % S;
+        const uint32_t lb = (z - z0) & 1U;  // LDS buffer
+        if constexpr (NXY > 0) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                if (!((XYM >> f) & 1U)) continue;
+                const int kk = bits_below(XYM, f);
+                float* t = tile[lb][kk];
+#pragma unroll
+                for (uint32_t r = 0; r < R; ++r)
+                    if (rin[r]) t[(1U + ty + kTileWaves * r) * W + 1U + tx] = v[sc][f][r];
+                if (hslot >= 0) t[hslot] = hv[sc][kk];
+            }
+        }
+        if (z + kPrefetch < z1) {
+            load_plane(sl, za);
+            load_halo(hs, zh);
+        }
+        za = za + 1U == d.nz ? 0U : za + 1U;
+        zh = zh + 1U == d.nz ? 0U : zh + 1U;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            if (!rin[r]) continue;
+            Nbr<NF> s{};
+            const uint32_t row = 1U + ty + kTileWaves * r;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                s.c[f] = v[sc][f][r];
+                if ((ZM >> f) & 1U) {
+                    s.zm[f] = v[sp][f][r];
+                    s.zp[f] = v[sn][f][r];
+                }
+                if constexpr (NXY > 0) {
+                    if ((XYM >> f) & 1U) {
+                        const float* t = tile[lb][bits_below(XYM, f)];
+                        s.xm[f] = t[row * W + tx];
+                        s.xp[f] = t[row * W + tx + 2U];
+                        s.ym[f] = t[(row - 1U) * W + 1U + tx];
+                        s.yp[f] = t[(row + 1U) * W + 1U + tx];
+                    }
+                }
+            }
+            op(s, z * plane + own[r]);
+        }
+    };
+    for (uint32_t z = z0;; z += S) {
+        step(std::integral_constant<uint32_t, 0>{}, z);
+        if (z + 1U >= z1) break;
+        step(std::integral_constant<uint32_t, 1>{}, z + 1U);
+        if (z + 2U >= z1) break;
+        step(std::integral_constant<uint32_t, 2>{}, z + 2U);
+        if (z + 3U >= z1) break;
+        step(std::integral_constant<uint32_t, 3>{}, z + 3U);
+        if (z + 4U >= z1) break;
+        if constexpr (S > 4) {
+            step(std::integral_constant<uint32_t, 4>{}, z + 4U);
+            if (z + 5U >= z1) break;
+        }
+    }
+}
 
 // ------------------------------------------------------------------ D3Q7 thermal lattice
 // Directions 0, +x, -x, +y, -y, +z, -z; weights 1/4 and 1/8 (resident_multiphysics_sycl.hpp:188-191).
@@ -140,15 +363,15 @@ __global__ void __launch_bounds__(kBlock) thermal_step(const float* __restrict__
         float temperature = 0.0F;
     #pragma unroll
         for (int q = 0; q < 7; ++q) {
-            local[q] = in[static_cast<size_t>(q) * d.cells + src[q]];
+            local[q] = ld(in + static_cast<size_t>(q) * d.cells, src[q]);
             temperature += local[q];
         }
-        const float u = ux[n], v = uy[n], w = uz[n];
+        const float u = ld(ux, n), v = ld(uy, n), w = ld(uz, n);
     #pragma unroll
         for (int q = 0; q < 7; ++q) {
             const float cu = q7_cx(q) * u + q7_cy(q) * v + q7_cz(q) * w;
             const float geq = q7_weight(q) * temperature * (1.0F + 4.0F * cu);
-            out[static_cast<size_t>(q) * d.cells + n] = local[q] - omega * (local[q] - geq);
+            el(out + static_cast<size_t>(q) * d.cells, n) = local[q] - omega * (local[q] - geq);
         }
     });
 }
@@ -224,35 +447,31 @@ __global__ void __launch_bounds__(kBlock) vof_fill(float* phi, const float* fiel
 }
 
 // Upwind fluxes through the six faces with face velocities averaged from the two cells, then
-// clamped to [0, 1] (resident_multiphysics_sycl.hpp:238-263).
-__global__ void __launch_bounds__(kBlock) vof_advect(const float* __restrict__ phi, float* __restrict__ out,
-                                                     const float* __restrict__ ux, const float* __restrict__ uy,
-                                                     const float* __restrict__ uz, Dims d, float dt) {
-    for_stencil_cells<kRowsAdvect, false>(d, [&](const Around& a) {
-        const uint32_t n = a.n;
-        const float uxp = 0.5F * (ux[n] + ux[a.xp]), uxm = 0.5F * (ux[a.xm] + ux[n]);
-        const float uyp = 0.5F * (uy[n] + uy[a.yp]), uym = 0.5F * (uy[a.ym] + uy[n]);
-        const float uzp = 0.5F * (uz[n] + uz[a.zp]), uzm = 0.5F * (uz[a.zm] + uz[n]);
-        // All six neighbour fills are loaded unconditionally (one memory round trip), then selected.
-        const float p = phi[n];
-        const float pxp = phi[a.xp], pxm = phi[a.xm], pyp = phi[a.yp], pym = phi[a.ym], pzp = phi[a.zp],
-                    pzm = phi[a.zm];
-        const float fxp = uxp >= 0.0F ? uxp * p : uxp * pxp;
-        const float fxm = uxm >= 0.0F ? uxm * pxm : uxm * p;
-        const float fyp = uyp >= 0.0F ? uyp * p : uyp * pyp;
-        const float fym = uym >= 0.0F ? uym * pym : uym * p;
-        const float fzp = uzp >= 0.0F ? uzp * p : uzp * pzp;
-        const float fzm = uzm >= 0.0F ? uzm * pzm : uzm * p;
+// clamped to [0, 1] (resident_multiphysics_sycl.hpp:238-263).  Fields: 0 fill (all neighbours),
+// 1 ux (x neighbours), 2 uy (y neighbours), 3 uz (z neighbours).
+__global__ void __launch_bounds__(kBlock) vof_advect(const Inputs<4> in, float* __restrict__ out, Dims d, float dt,
+                                                     uint32_t zchunk) {
+    zmarch<4, 0b1001U, 0b0111U, kRowsAdvect>(in, d, zchunk, [&](const Nbr<4>& s, uint32_t n) {
+        const float uxp = 0.5F * (s.c[1] + s.xp[1]), uxm = 0.5F * (s.xm[1] + s.c[1]);
+        const float uyp = 0.5F * (s.c[2] + s.yp[2]), uym = 0.5F * (s.ym[2] + s.c[2]);
+        const float uzp = 0.5F * (s.c[3] + s.zp[3]), uzm = 0.5F * (s.zm[3] + s.c[3]);
+        const float p = s.c[0];
+        const float fxp = uxp >= 0.0F ? uxp * p : uxp * s.xp[0];
+        const float fxm = uxm >= 0.0F ? uxm * s.xm[0] : uxm * p;
+        const float fyp = uyp >= 0.0F ? uyp * p : uyp * s.yp[0];
+        const float fym = uym >= 0.0F ? uym * s.ym[0] : uym * p;
+        const float fzp = uzp >= 0.0F ? uzp * p : uzp * s.zp[0];
+        const float fzm = uzm >= 0.0F ? uzm * s.zm[0] : uzm * p;
         float value = p - dt * ((fxp - fxm) + (fyp - fym) + (fzp - fzm));
         value = value < 0.0F ? 0.0F : value > 1.0F ? 1.0F : value;
-        out[n] = value;
+        el(out, n) = value;
     });
 }
 
 __device__ __forceinline__ void fill_gradient(const float* phi, const Around& a, float& gx, float& gy, float& gz) {
-    gx = 0.5F * (phi[a.xp] - phi[a.xm]);
-    gy = 0.5F * (phi[a.yp] - phi[a.ym]);
-    gz = 0.5F * (phi[a.zp] - phi[a.zm]);
+    gx = 0.5F * (ld(phi, a.xp) - ld(phi, a.xm));
+    gy = 0.5F * (ld(phi, a.yp) - ld(phi, a.ym));
+    gz = 0.5F * (ld(phi, a.zp) - ld(phi, a.zm));
 }
 
 // Unit interface normal grad(phi)/|grad(phi)| (zero where the gradient vanishes).
@@ -266,13 +485,13 @@ __global__ void __launch_bounds__(kBlock) vof_normals(const float* __restrict__ 
         float* nyv = geo + d.cells;
         float* nzv = geo + 2 * static_cast<size_t>(d.cells);
         if (mag > 1.0e-12F) {
-            nxv[n] = gx / mag;
-            nyv[n] = gy / mag;
-            nzv[n] = gz / mag;
+            el(nxv, n) = gx / mag;
+            el(nyv, n) = gy / mag;
+            el(nzv, n) = gz / mag;
         } else {
-            nxv[n] = 0.0F;
-            nyv[n] = 0.0F;
-            nzv[n] = 0.0F;
+            el(nxv, n) = 0.0F;
+            el(nyv, n) = 0.0F;
+            el(nzv, n) = 0.0F;
         }
     });
 }
@@ -284,8 +503,8 @@ __global__ void __launch_bounds__(kBlock) vof_curvature(float* geo, Dims d) {
         const float* nxv = geo;
         const float* nyv = geo + d.cells;
         const float* nzv = geo + 2 * static_cast<size_t>(d.cells);
-        geo[3 * static_cast<size_t>(d.cells) + n] =
-            -0.5F * ((nxv[a.xp] - nxv[a.xm]) + (nyv[a.yp] - nyv[a.ym]) + (nzv[a.zp] - nzv[a.zm]));
+        el(geo + 3 * static_cast<size_t>(d.cells), n) =
+            -0.5F * ((ld(nxv, a.xp) - ld(nxv, a.xm)) + (ld(nyv, a.yp) - ld(nyv, a.ym)) + (ld(nzv, a.zp) - ld(nzv, a.zm)));
     });
 }
 
@@ -297,10 +516,10 @@ __global__ void __launch_bounds__(kBlock) vof_tension(const float* __restrict__ 
         const uint32_t n = a.n;
         float gx, gy, gz;
         fill_gradient(phi, a, gx, gy, gz);
-        const float c = sigma * curvature[n] / density;
-        ax[n] += c * gx;
-        ay[n] += c * gy;
-        az[n] += c * gz;
+        const float c = sigma * ld(curvature, n) / density;
+        el(ax, n) += c * gx;
+        el(ay, n) += c * gy;
+        el(az, n) += c * gz;
     });
 }
 
@@ -385,17 +604,14 @@ __global__ void __launch_bounds__(kBlock) particles_advance(float* s, uint32_t c
 
 // ------------------------------------------------------------------ Q-criterion
 // Q = (|Omega|^2 - |S|^2) / 2 from central differences (resident_multiphysics_sycl.hpp:394-404).
-__global__ void __launch_bounds__(kBlock) q_criterion(const float* __restrict__ ux, const float* __restrict__ uy,
-                                                      const float* __restrict__ uz, float* __restrict__ q, Dims d) {
-    for_stencil_cells<kRowsQ, kRemapQ>(d, [&](const Around& a) {
-        const uint32_t n = a.n;
-        const uint32_t p[3] = {a.xp, a.yp, a.zp}, m[3] = {a.xm, a.ym, a.zm};
+__global__ void __launch_bounds__(kBlock) q_criterion(const Inputs<3> u, float* __restrict__ q, Dims d, uint32_t zchunk) {
+    zmarch<3, 0b111U, 0b111U, kRowsQ>(u, d, zchunk, [&](const Nbr<3>& s, uint32_t n) {
         float g[3][3];
     #pragma unroll
-        for (int axis = 0; axis < 3; ++axis) {
-            g[0][axis] = 0.5F * (ux[p[axis]] - ux[m[axis]]);
-            g[1][axis] = 0.5F * (uy[p[axis]] - uy[m[axis]]);
-            g[2][axis] = 0.5F * (uz[p[axis]] - uz[m[axis]]);
+        for (int c = 0; c < 3; ++c) {
+            g[c][0] = 0.5F * (s.xp[c] - s.xm[c]);
+            g[c][1] = 0.5F * (s.yp[c] - s.ym[c]);
+            g[c][2] = 0.5F * (s.zp[c] - s.zm[c]);
         }
         float s2 = 0.0F, o2 = 0.0F;
     #pragma unroll
@@ -406,7 +622,7 @@ __global__ void __launch_bounds__(kBlock) q_criterion(const float* __restrict__ 
                 s2 += sij * sij;
                 o2 += oij * oij;
             }
-        q[n] = 0.5F * (o2 - s2);
+        el(q, n) = 0.5F * (o2 - s2);
     });
 }
 
@@ -426,6 +642,23 @@ StencilShape stencil_shape(const Dims& d, uint32_t rows) {
     return {dim3((d.nx + bx - 1) / bx, (d.ny + by * rows - 1) / (by * rows), d.nz), dim3(bx, by, 1)};
 }
 
+// Launch shape of the z-marching kernels: kTileX x (kTileWaves * rows) tiles, and z chunks sized so
+// that the launch has about kZBlocks blocks.
+struct ZShape {
+    dim3 grid, block;
+    uint32_t zchunk;
+};
+ZShape zmarch_shape(const Dims& d, uint32_t rows) {
+    const uint32_t ty = kTileWaves * rows;
+    const uint32_t tiles_x = (d.nx + kTileX - 1) / kTileX, tiles_y = (d.ny + ty - 1) / ty;
+    const uint64_t tiles = static_cast<uint64_t>(tiles_x) * tiles_y;
+    uint32_t chunks = tiles >= kZBlocks ? 1U : static_cast<uint32_t>((kZBlocks + tiles - 1) / tiles);
+    chunks = std::min(chunks, d.nz);
+    const uint32_t zchunk = (d.nz + chunks - 1) / chunks;
+    chunks = (d.nz + zchunk - 1) / zchunk;
+    return {dim3(tiles_x, tiles_y, chunks), dim3(kTileX, kTileWaves, 1), zchunk};
+}
+
 struct Stats {
     uint64_t h2d = 0, d2h = 0, d2d = 0, syncs = 0;
 };
@@ -440,6 +673,8 @@ struct Module {
                                         " couples to FP32 device views: the flow needs FP32, FP16S or FP32S storage");
         if (flow.nx < min_extent || flow.ny < min_extent || flow.nz < min_extent)
             throw std::invalid_argument(std::string("invalid resident ") + what + " lattice");
+        if (flow.cells > (1ull << 30))  // field planes are addressed with 32-bit byte offsets
+            throw std::invalid_argument(std::string("resident ") + what + " lattice too large: at most 2^30 cells");
         dims = {static_cast<uint32_t>(flow.nx), static_cast<uint32_t>(flow.ny), static_cast<uint32_t>(flow.nz),
                 static_cast<uint32_t>(flow.cells)};
         const StencilShape widest = stencil_shape(dims, 1);
@@ -448,6 +683,7 @@ struct Module {
                                         std::to_string(65535U * widest.block.y) + " rows per plane");
     }
     StencilShape shape(uint32_t rows) const { return stencil_shape(dims, rows); }
+    ZShape zshape(uint32_t rows) const { return zmarch_shape(dims, rows); }
     void use() const { check(hipSetDevice(flow.device), "hipSetDevice"); }
     float* alloc(uint64_t floats) const {
         void* p = nullptr;
@@ -775,8 +1011,9 @@ int cfd_lbm_surface_advect(cfd_lbm_surface* s, const float* ux, const float* uy,
         need_obj(s).use();
         need_views({ux, uy, uz}, "free-surface advection");
         if (!(dt > 0.0F)) throw std::invalid_argument("invalid resident free-surface advection");
-        const StencilShape sh = s->shape(kRowsAdvect);
-        vof_advect<<<sh.grid, sh.block, 0, s->flow.stream>>>(s->fill, s->next, ux, uy, uz, s->dims, dt);
+        const ZShape sh = s->zshape(kRowsAdvect);
+        vof_advect<<<sh.grid, sh.block, 0, s->flow.stream>>>(Inputs<4>{{s->fill, ux, uy, uz}}, s->next, s->dims, dt,
+                                                             sh.zchunk);
         check(hipGetLastError(), "vof_advect");
         std::swap(s->fill, s->next);
     });
@@ -915,8 +1152,8 @@ int cfd_lbm_q_criterion(cfd_lbm_flow_diagnostics* d, const float* ux, const floa
     return guarded(error, cap, [&] {
         need_obj(d).use();
         need_views({ux, uy, uz}, "Q-criterion");
-        const StencilShape sh = d->shape(kRowsQ);
-        q_criterion<<<sh.grid, sh.block, 0, d->flow.stream>>>(ux, uy, uz, d->q, d->dims);
+        const ZShape sh = d->zshape(kRowsQ);
+        q_criterion<<<sh.grid, sh.block, 0, d->flow.stream>>>(Inputs<3>{{ux, uy, uz}}, d->q, d->dims, sh.zchunk);
         check(hipGetLastError(), "q_criterion");
         if (q) *q = d->q;
     });

@@ -163,11 +163,16 @@ def test_q_criterion_matches_oracle(hip, oracle_mod):
     flow.close()
 
 
-@pytest.mark.parametrize("shape", [(300, 37, 11), (65, 5, 3), (3, 129, 4), (64, 128, 6)])
+@pytest.mark.parametrize("shape", [(300, 37, 11), (65, 5, 3), (3, 129, 4), (64, 128, 6), (256, 128, 71),
+                                   (200, 100, 130)])
 def test_stencils_on_ragged_shapes(hip, oracle_mod, shape):
-    """The stencil kernels' 3-D launch (64-256-wide blocks, several rows per thread) on extents that
-    leave partial blocks in x and y, and on one whose planes hold a multiple of 8 blocks for every
-    kernel (the per-XCD block remap): every kernel bit for bit against the oracle."""
+    """Every stencil kernel bit for bit against the oracle on extents that leave partial blocks
+    and tiles in x and y.  The 3-D launch (normals, curvature, tension, thermal step) takes 64-256
+    wide blocks of several rows per thread; 64 x 128 x 6 holds a multiple of 8 of them per plane
+    (the per-XCD remap).  The z-marching tiles (fill advection, Q-criterion; 64 x 4 cells) march
+    several planes per block in the last two shapes: 256 x 128 x 71 in 15 chunks of 5 planes (the
+    last of 1), 1920 blocks, a multiple of 8 (the per-XCD block placement); 200 x 100 x 130 in 19
+    chunks of 7 planes (the last of 4), 1900 blocks."""
     nx, ny, nz = shape
     report("resident_ragged", shape=shape)
     flow = hip.EsotericPullHipSolver("d3q19", nx, ny, nz, 0.7)
