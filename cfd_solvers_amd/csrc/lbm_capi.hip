@@ -136,12 +136,14 @@ struct cfd_lbm_solver {
         elem = storage == 1 ? 2 : (storage == 2 ? 8 : 4);
         real = storage == 2 ? 8 : 4;
 
-        // Launch geometry: 256-thread tiles, bx a power of two in [32, 256] covering x.
-        uint32_t bx = 32;
+        // Launch geometry: 256-thread tiles, bx a power of two in [64, 256] covering x, so that a
+        // wavefront's 64 lanes lie in one row segment (the bounded step reads the segment's span
+        // once per wavefront).
+        uint32_t bx = 64;
         while (bx < nx && bx < 256) bx <<= 1;
-        if (const char* v = std::getenv("CFD_LBM_BX")) {  // tuning: tile width 32, 64, 128 or 256
+        if (const char* v = std::getenv("CFD_LBM_BX")) {  // tuning: tile width 64, 128 or 256
             const long w = std::atol(v);
-            if (w == 32 || w == 64 || w == 128 || w == 256) bx = static_cast<uint32_t>(w);
+            if (w == 64 || w == 128 || w == 256) bx = static_cast<uint32_t>(w);
         }
         // Resident wavefronts per SIMD of the step kernel, capped by reserving LDS per block (160 KiB
         // per CU; a 256-thread block puts one wave on each of the 4 SIMDs).  Measured on MI355X at
@@ -253,7 +255,7 @@ struct cfd_lbm_solver {
         p.solid_rho = reinterpret_cast<const Real*>(solid_rho.ptr);
         p.links = reinterpret_cast<const uint32_t*>(links.ptr);
         p.link_vel = reinterpret_cast<const uint32_t*>(link_vel.ptr);
-        p.segments = reinterpret_cast<const uint16_t*>(segments.ptr);
+        p.segments = reinterpret_cast<const uint32_t*>(segments.ptr);
         p.seg_x = seg_x();
         p.cells = static_cast<uint32_t>(cells);
         std::memcpy(physics, &p, sizeof(p));
@@ -465,8 +467,8 @@ struct cfd_lbm_solver {
             solid_rho.reset();
             return;
         }
-        if (!segments) segments.allocate(static_cast<size_t>(seg_x()) * ny * nz * sizeof(uint16_t));
-        ops->segments(st, fresh.ptr, seg_x(), reinterpret_cast<uint16_t*>(segments.ptr));
+        if (!segments) segments.allocate(static_cast<size_t>(seg_x()) * ny * nz * sizeof(uint32_t));
+        ops->segments(st, fresh.ptr, seg_x(), reinterpret_cast<uint32_t*>(segments.ptr));
         if (!links) links.allocate(cells * sizeof(uint32_t));
         if (!link_vel) link_vel.allocate(cells * vel_words() * sizeof(uint32_t));
         ops->links(st, fresh.ptr, reinterpret_cast<uint32_t*>(links.ptr), reinterpret_cast<uint32_t*>(link_vel.ptr));

@@ -153,7 +153,8 @@ template <class S, int Q, int D, class Real> __device__ __forceinline__ Real sta
 }
 
 // ------------------------------------------------------------------ launch geometry
-// A block is a bx*by = 256-thread tile of one z plane (bx spans x, multiple of 32).  Tiles are
+// A block is a bx*by = 256-thread tile of one z plane (bx spans x: 64, 128 or 256, so that the 64
+// lanes of a wavefront lie in one row and one 64-cell row segment).  Tiles are
 // numbered x-fastest.  The hardware deals consecutive blocks round-robin to the 8 XCDs, so the
 // chip streams through neighbouring rows together; xcd_remap (off by default: it measured slower
 // on MI355X) instead gives each XCD one contiguous slab of tiles.
@@ -351,7 +352,10 @@ template <class Real> struct Physics {
     const Real* solid_rho;      // density frozen in wall cells
     const uint32_t* links;      // per-cell link descriptor (bounded lattices)
     const uint32_t* link_vel;   // per cell, (q-1+9)/10 words of 3-bit wall-velocity indices per link
-    const uint16_t* segments;   // per 64-cell row segment: span lo | hi << 8 of its non-plain cells, 0 if none
+    // Per 64-cell row segment of a bounded lattice: the span lo | hi << 8 of the cells [lo, hi) that
+    // are not plain fluid with a plain-fluid 3x3x3 neighbourhood, 0 if none.  A 32-bit word, read
+    // through the scalar cache by the wavefront that updates the segment.
+    const uint32_t* segments;
     uint32_t seg_x;             // segments per row, ceil(nx / 64)
     uint32_t cells;
 };

@@ -48,11 +48,14 @@ __global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const vo
     const Neighbourhood h = neighbourhood(g, x, y, z);
     const uint32_t n = h.lin(0, 0, 0);
     if constexpr (Bounded) {
-        // A 64-cell row segment whose cells are all plain fluid (span 0) takes the periodic path:
-        // no link loads, no bounce-back selects.  A wavefront covers one segment when bx >= 64,
-        // so the branch is uniform there.  Otherwise only the cells inside the segment's span
-        // [lo, hi) can have a nonzero link descriptor; the others skip its load.
-        const uint32_t span = p.segments[(z * g.ny + y) * p.seg_x + (x >> 6)];
+        // The 64 lanes of a wavefront lie in one 64-cell row segment (bx >= 64), so its span is
+        // wavefront-uniform: one scalar load, and a branch on it costs no lane masking.  A segment
+        // whose cells are all plain fluid (span 0) takes the periodic path: no link loads, no
+        // bounce-back selects.  Otherwise only the cells inside the span [lo, hi) can have a nonzero
+        // link descriptor; the others skip its load.
+        const uint32_t k = __builtin_amdgcn_readfirstlane((z * g.ny + y) * p.seg_x + (x >> 6));
+        using ConstantWords = const __attribute__((address_space(4))) uint32_t*;  // scalar loads
+        const uint32_t span = ((ConstantWords)p.segments)[k];
         if (span == 0U) {
             update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, 0U, n, x, y, z);
             return;
@@ -71,7 +74,7 @@ __global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const vo
 // with a plain-fluid 3x3x3 neighbourhood (periodic), stored as lo | hi << 8; 0 when every cell is
 // (the interior segments).  Cells outside the span have link descriptor 0.
 __global__ __launch_bounds__(256) void mark_segments(const uint8_t* __restrict__ flags, const Geometry g,
-                                                     uint32_t seg_x, uint16_t* __restrict__ segments) {
+                                                     uint32_t seg_x, uint32_t* __restrict__ segments) {
     const uint32_t total = seg_x * g.ny * g.nz;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
@@ -104,7 +107,7 @@ __global__ __launch_bounds__(256) void mark_segments(const uint8_t* __restrict__
         left = mid;
         mid = right;
     }
-    segments[k] = hi == 0U ? uint16_t{0} : static_cast<uint16_t>(lo | (hi << 8));
+    segments[k] = hi == 0U ? 0U : lo | (hi << 8);
 }
 
 // Link descriptor of every cell from the flags (layout in lbm_device.hpp).
@@ -649,7 +652,7 @@ template <int Q, class T> struct Ops final : KernelOps {
                                st.f, st.geo, flags, old_flags, static_cast<Real*>(solid_rho));
         check(hipGetLastError(), "freeze_wall_density launch");
     }
-    void segments(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint16_t* out) const override {
+    void segments(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint32_t* out) const override {
         const uint32_t total = seg_x * st.geo.ny * st.geo.nz;
         hipLaunchKernelGGL(mark_segments, dim3((total + 255U) / 256U), dim3(256), 0, st.stream, flags, st.geo, seg_x,
                            out);

@@ -30,7 +30,7 @@ struct KernelOps {
     virtual void bounce_back(const LaunchState& st, bool odd, bool to_bounce_back, const uint8_t* flags) const = 0;
     virtual void freeze(const LaunchState& st, bool odd, const uint8_t* flags, const uint8_t* old_flags,
                         void* solid_rho) const = 0;
-    virtual void segments(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint16_t* out) const = 0;
+    virtual void segments(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint32_t* out) const = 0;
     virtual void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
                       const void* solid_rho) const = 0;
     virtual void macroscopic(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;

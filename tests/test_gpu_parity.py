@@ -609,7 +609,7 @@ def test_local_acceleration_matches_reference_fixture(hip, golden, q):
     assert np.array_equal(g.raw_storage(), d[f"q{q}_raw9"])
 
 
-LAUNCH_SHAPES = [{}, {"CFD_LBM_BX": "32"}, {"CFD_LBM_BX": "64"}, {"CFD_LBM_BX": "128"}, {"CFD_LBM_BX": "256"},
+LAUNCH_SHAPES = [{}, {"CFD_LBM_BX": "64"}, {"CFD_LBM_BX": "128"}, {"CFD_LBM_BX": "256"},
                  {"CFD_LBM_XCD_REMAP": "1"}, {"CFD_LBM_WAVES": "8"}, {"CFD_LBM_BLOCK_CELLS": "4096"},
                  {"CFD_LBM_LAYOUT": "planes"}]
 
