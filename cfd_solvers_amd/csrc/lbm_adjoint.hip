@@ -45,6 +45,22 @@ __device__ __forceinline__ void feq_grad(Real rho, Real ux, Real uy, Real uz, Re
     gu[2] = wr * (Real(3) * static_cast<Real>(L::cz[D]) + Real(9) * cu * static_cast<Real>(L::cz[D]) - Real(3) * uz);
 }
 
+template <class Real> __device__ __forceinline__ Real fma_real(Real a, Real b, Real c) {
+    if constexpr (sizeof(Real) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
+// s += c_D v, component by component (c_D in {-1, 0, 1}^3).
+template <int Q, int D, class Real> __device__ __forceinline__ void add_c(Real (&s)[3], Real v) {
+    using L = Lattice<Q>;
+    if constexpr (L::cx[D] > 0) s[0] += v;
+    if constexpr (L::cx[D] < 0) s[0] -= v;
+    if constexpr (L::cy[D] > 0) s[1] += v;
+    if constexpr (L::cy[D] < 0) s[1] -= v;
+    if constexpr (L::cz[D] > 0) s[2] += v;
+    if constexpr (L::cz[D] < 0) s[2] -= v;
+}
+
 // mu = (d fout / d fin)^T lam for one cell; also d/dgamma and d/du_peak of lam . fout.
 template <int Q, class Real>
 __device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
@@ -97,39 +113,72 @@ __device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geom
         });
         return;
     }
+    // BGK with partial bounce-back: fout_d = (1 - gamma) (f_d - omega (f_d - feq_d)) + gamma f_opp(d).
+    // With a_d = (1 - gamma) lam_d, b_d = omega w_d a_d and cu_d = c_d . u, the contraction of lam
+    // with the gradients of feq_d(rho, u) needs only three sums, S0 = sum_d b_d, S1 = sum_d b_d c_d
+    // and S2 = sum_d b_d cu_d c_d:
+    //   g_rho = sum_d omega a_d dfeq_d/drho      = S0 (1 - 1.5 |u|^2) + 3 u.S1 + 4.5 u.S2
+    //   G     = sum_d omega a_d dfeq_d/du / rho  = 3 S1 + 9 S2 - 3 u S0
+    // and mu_j = (1 - omega) a_j + gamma lam_opp(j) + g_rho - G.u + c_j.G, since drho/df_j = 1 and
+    // du/df_j = (c_j - u) / rho.  Opposite directions share w_d and flip c_d and cu_d, so each pair
+    // adds the sum and the difference of its two multipliers.  The sums associate differently from
+    // the per-direction form (feq_grad, kept for the inlet and outlet), which changes results only
+    // by rounding.
     Real rho, mx, my, mz;
     moments<Q>(f, rho, mx, my, mz);
     const Real ux = mx / rho, uy = my / rho, uz = mz / rho;
     const Real uu = ux * ux + uy * uy + uz * uz;
     const Real omega = p.omega;
-    const Real gm = p.gamma ? p.gamma[n] : Real(0);
+    const bool porous = p.gamma != nullptr;
+    const Real gm = porous ? p.gamma[n] : Real(0);
     const Real keep = Real(1) - gm;
-    Real g_rho = Real(0), G0 = Real(0), G1 = Real(0), G2 = Real(0), dg = Real(0);
-    Real a[Q];
-    static_for<Q>([&](auto DD) {
-        constexpr int d = decltype(DD)::value;
-        Real e, gu[3];
-        feq_grad<Q, d>(rho, ux, uy, uz, uu, e, gu);
-        a[d] = keep * lam[d];
-        const Real oa = omega * a[d];
-        g_rho += oa * e;
-        G0 += oa * gu[0];
-        G1 += oa * gu[1];
-        G2 += oa * gu[2];
-        const Real fstar = f[d] - omega * (f[d] - rho * e);
-        dg += lam[d] * (f[opposite(d)] - fstar);
+    const Real ok = omega * keep;                  // b_d = ok w_d lam_d
+    const Real om1k = (Real(1) - omega) * keep;    // (1 - omega) a_d = om1k lam_d
+    const Real quad = Real(1) - Real(1.5) * uu;
+    const Real w0 = static_cast<Real>(L::w[0]);
+    Real S0 = ok * w0 * lam[0];
+    Real S1[3] = {Real(0), Real(0), Real(0)}, S2[3] = {Real(0), Real(0), Real(0)};
+    // dJ/dgamma = sum_d lam_d (f_opp(d) - fstar_d), fstar_d = f_d - omega (f_d - feq_d), for porous
+    // lattices only.
+    Real dg = Real(0);
+    if (porous) {
+        const Real fs = f[0] - omega * (f[0] - rho * w0 * quad);
+        dg = lam[0] * (f[0] - fs);
+    }
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const Real w = static_cast<Real>(L::w[i]);
+        const Real cu = cdot<Q, i>(ux, uy, uz);
+        const Real kw = ok * w;
+        const Real bs = kw * (lam[i] + lam[i + 1]);
+        const Real bd = kw * (lam[i] - lam[i + 1]);
+        S0 += bs;
+        const Real bscu = bs * cu;
+        add_c<Q, i>(S1, bd);
+        add_c<Q, i>(S2, bscu);
+        if (porous) {
+            const Real rw = rho * w;
+            const Real even = rw * fma_real(Real(4.5) * cu, cu, quad);  // feq of the pair without the odd term
+            const Real odd = rw * (Real(3) * cu);
+            const Real fs_a = f[i] - omega * (f[i] - (even + odd));
+            const Real fs_b = f[i + 1] - omega * (f[i + 1] - (even - odd));
+            dg += lam[i] * (f[i + 1] - fs_a) + lam[i + 1] * (f[i] - fs_b);
+        }
     });
-    G0 = G0 / rho;
-    G1 = G1 / rho;
-    G2 = G2 / rho;
-    const Real Gu = G0 * ux + G1 * uy + G2 * uz;
-    const Real base = g_rho - Gu;
-    static_for<Q>([&](auto DD) {
-        constexpr int d = decltype(DD)::value;
-        const Real Gc = cdot<Q, d>(G0, G1, G2);
-        mu[d] = (Real(1) - omega) * a[d] + gm * lam[opposite(d)] + base + Gc;
+    const Real G0 = Real(3) * S1[0] + Real(9) * S2[0] - Real(3) * ux * S0;
+    const Real G1 = Real(3) * S1[1] + Real(9) * S2[1] - Real(3) * uy * S0;
+    const Real G2 = Real(3) * S1[2] + Real(9) * S2[2] - Real(3) * uz * S0;
+    const Real g_rho = S0 * quad + Real(3) * (ux * S1[0] + uy * S1[1] + uz * S1[2]) +
+                       Real(4.5) * (ux * S2[0] + uy * S2[1] + uz * S2[2]);
+    const Real base = g_rho - (G0 * ux + G1 * uy + G2 * uz);
+    mu[0] = om1k * lam[0] + gm * lam[0] + base;
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1;
+        const Real Gc = cdot<Q, i>(G0, G1, G2);
+        mu[i] = om1k * lam[i] + gm * lam[i + 1] + (base + Gc);
+        mu[i + 1] = om1k * lam[i + 1] + gm * lam[i] + (base - Gc);
     });
-    dgamma = p.gamma ? static_cast<double>(dg) : 0.0;
+    dgamma = porous ? static_cast<double>(dg) : 0.0;
 }
 
 // Objective plane weight of a cell: c_in on plane x_in, c_out on plane x_out (fluid cells only).
@@ -146,9 +195,7 @@ template <int Q, class T, bool Odd, bool Bounded>
 __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fstate, void* __restrict__ lam_buf,
                                                     const Geometry g, const Physics<typename Storage<T>::Real> p,
                                                     const AdjointArgs a) {
-    using L = Lattice<Q>;
     using Real = typename Storage<T>::Real;
-    using S = Storage<Real>;
     uint32_t x, y, z;
     if (!cell_coords(g, x, y, z)) return;
     const Neighbourhood h = neighbourhood(g, x, y, z);
@@ -398,7 +445,7 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
               const AdjointArgs& a) const override {
         per_parity(odd, bounded, [&](auto O, auto B) {
             hipLaunchKernelGGL((adjoint_step<Q, T, decltype(O)::value, decltype(B)::value>), grid(st.geo),
-                               block(st.geo), 0, st.stream, fstate, lam, st.geo,
+                               block(st.geo), st.adjoint_lds, st.stream, fstate, lam, st.geo,
                                *static_cast<const Physics<Real>*>(st.physics), a);
         });
         check(hipGetLastError(), "adjoint_step launch");

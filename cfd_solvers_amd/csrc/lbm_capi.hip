@@ -103,7 +103,7 @@ struct cfd_lbm_solver {
     float u_peak = 0.0F, rho_out = 1.0F;
     bool les = false;  // Smagorinsky LES (esoteric_pull.hpp:72-97, InPlaceLbmConfig::smagorinsky_les)
     uint64_t t = 0;
-    uint32_t step_lds = 0;
+    uint32_t step_lds = 0, adjoint_lds = 0;
     std::vector<uint8_t> host_flags;  // copy of the flags (empty: periodic)
     float last_forward_ms = 0.0F, last_reverse_ms = 0.0F;
     uint64_t last_forward_steps = 0;
@@ -153,6 +153,12 @@ struct cfd_lbm_solver {
         if (storage == 0 || storage == 3) waves = lattice == 27 ? 4 : 5;
         if (const char* v = std::getenv("CFD_LBM_WAVES")) waves = std::atol(v);
         if (waves >= 3 && waves < 8) step_lds = static_cast<uint32_t>(163840 / waves - 1024);
+        // The transposed step (adjoint) with FP16 storage runs 6 waves per SIMD by its registers; on
+        // HBM-bound lattices 5 measured faster (768^3 sphere cluster: 20.6 -> 19.6 ms), on 128^3,
+        // which the caches partly hold, slower (76 -> 79 us).  CFD_LBM_ADJOINT_WAVES (3..8) overrides.
+        long adjoint_waves = (storage == 1 && cells >= (size_t{1} << 24)) ? 5 : 8;
+        if (const char* v = std::getenv("CFD_LBM_ADJOINT_WAVES")) adjoint_waves = std::atol(v);
+        if (adjoint_waves >= 3 && adjoint_waves < 8) adjoint_lds = static_cast<uint32_t>(163840 / adjoint_waves - 1024);
         geo.nx = static_cast<uint32_t>(nx);
         geo.ny = static_cast<uint32_t>(ny);
         geo.nz = static_cast<uint32_t>(nz);
@@ -264,7 +270,7 @@ struct cfd_lbm_solver {
     LaunchState state() {
         if (storage == 2) build_physics<double>();
         else build_physics<float>();
-        return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells), step_lds};
+        return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells), step_lds, adjoint_lds};
     }
 
     void sync() const {

@@ -17,6 +17,7 @@ struct LaunchState {
     hipStream_t stream;
     uint32_t cells;
     uint32_t step_lds;       // dynamic LDS bytes per step-kernel block (occupancy cap; 0 = none)
+    uint32_t adjoint_lds;    // the same for the transposed step
 };
 
 // One instance per (lattice, storage); every launch is asynchronous on st.stream.
