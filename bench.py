@@ -307,6 +307,8 @@ def config5_leg(hip, n=768, steps=200, storages=("fp16s", "fp32"), kernel_steps=
         s.initialize_uniform(1.0, 0.04, 0.0, 0.0)  # impulsive start: the inflow everywhere
         return s
 
+    if steps >= 2000:  # minutes per run: the library reports its progress every 1000 steps
+        os.environ.setdefault("CFD_LBM_PROGRESS_STEPS", "1000")
     for storage in storages:
         elem = BYTES_PER_POP[storage]
         log(f"leg config5: {storage} {n}^3, T {steps}")
@@ -612,6 +614,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1,config5")
     ap.add_argument("--config5-steps", type=int, default=200)
+    ap.add_argument("--config5-storages", default="fp16s,fp32")
     ap.add_argument("--config5-n", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-probe", action="store_true")
@@ -638,7 +641,8 @@ def main():
             legs[name] = adjoint_leg(hip)
             continue
         if name == "config5":
-            legs[name] = config5_leg(hip, n=args.config5_n, steps=args.config5_steps)
+            legs[name] = config5_leg(hip, n=args.config5_n, steps=args.config5_steps,
+                                     storages=tuple(args.config5_storages.split(",")))
             continue
         if name == "voxelize":
             legs[name] = voxelize_leg(hip)

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -905,6 +906,17 @@ struct cfd_lbm_solver {
         const uint64_t t0 = t;
         const auto par = [t0](uint64_t k) { return ((t0 + k) & 1U) != 0U; };
         uint64_t forward_steps = 0;
+        // Progress lines on stderr every CFD_LBM_PROGRESS_STEPS forward or transposed steps (0 or
+        // unset: none), each after the stream has finished those steps, for runs of many minutes.
+        uint64_t progress = 0, transposed_done = 0;
+        if (const char* v = std::getenv("CFD_LBM_PROGRESS_STEPS")) progress = std::strtoull(v, nullptr, 10);
+        const auto report = [&](const char* phase, uint64_t done, uint64_t total) {
+            sync();
+            std::fprintf(stderr, "[adjoint] %s: %llu of %llu steps, %llu forward steps so far\n", phase,
+                         static_cast<unsigned long long>(done), static_cast<unsigned long long>(total),
+                         static_cast<unsigned long long>(forward_steps));
+            std::fflush(stderr);
+        };
         // Snapshots taken on the way forward: (step index, buffer).
         std::vector<std::pair<uint64_t, void*>> shots;
         std::vector<void*> free_list;
@@ -941,12 +953,15 @@ struct cfd_lbm_solver {
             cur = dst;
             ++forward_steps;
             if (k + 1 >= steps - window + 1) adj->objective(st, par(k + 1), b, cur, args, 1.0, reinterpret_cast<double*>(acc.ptr));
+            if (progress > 0 && (k + 1) % progress == 0) report("forward sweep", k + 1, steps);
         }
         t = t0 + steps;
         hip_check(hipEventRecord(e1, stream), "hipEventRecord");
         const auto transposed = [&](uint64_t tt, const void* state) {
             args.add_objective = tt + window >= steps + 1 ? 1 : 0;
             adj->step(st, par(tt), b, state, lam.ptr, args);
+            ++transposed_done;
+            if (progress > 0 && transposed_done % progress == 0) report("reverse", transposed_done, steps);
         };
         // States a+1 .. m of the chain into dst, starting from state a in src (out of place first).
         const auto advance = [&](const void* src, void* dst, uint64_t a0, uint64_t m) {
