@@ -146,14 +146,6 @@ struct cfd_lbm_solver {
             const long w = std::atol(v);
             if (w == 64 || w == 128 || w == 256) bx = static_cast<uint32_t>(w);
         }
-        // Resident wavefronts per SIMD of the step kernel, capped by reserving LDS per block (160 KiB
-        // per CU; a 256-thread block puts one wave on each of the 4 SIMDs).  Measured on MI355X at
-        // 256^3: FP32 D3Q19 is fastest at 5 waves (4 % over 8), FP32 D3Q27 at 4; FP16 wants all 8.
-        // CFD_LBM_WAVES (3..8) overrides.
-        long waves = 8;
-        if (storage == 0 || storage == 3) waves = lattice == 27 ? 4 : 5;
-        if (const char* v = std::getenv("CFD_LBM_WAVES")) waves = std::atol(v);
-        if (waves >= 3 && waves < 8) step_lds = static_cast<uint32_t>(163840 / waves - 1024);
         // The transposed step (adjoint) with FP16 storage runs 6 waves per SIMD by its registers; on
         // HBM-bound lattices 5 measured faster (768^3 sphere cluster: 20.6 -> 19.6 ms), on 128^3,
         // which the caches partly hold, slower (76 -> 79 us).  CFD_LBM_ADJOINT_WAVES (3..8) overrides.
@@ -209,6 +201,16 @@ struct cfd_lbm_solver {
             if (2U * geo.stride * elem > (uint64_t{1} << 32))
                 throw std::invalid_argument("lattice too large: two population planes must fit in 4 GiB");
         }
+        // Resident wavefronts per SIMD of the step kernel, capped by reserving LDS per block (160 KiB
+        // per CU; a 256-thread block puts one wave on each of the 4 SIMDs).  Measured on MI355X:
+        // FP32 D3Q19 is fastest at 5 waves in row blocks (256^3: 4 % over 8) and at 4 in plain
+        // planes (768^3 sphere cluster: 14.3-14.7 ms against 15.2 at 5 and 15.4 at 8); FP32 D3Q27
+        // at 4; FP16 wants all 8 (768^3: 7.07-7.12 ms at 8 or 6, 7.29-7.38 at 5).  CFD_LBM_WAVES
+        // (3..8) overrides.
+        long waves = 8;
+        if (storage == 0 || storage == 3) waves = (lattice == 27 || !blocked) ? 4 : 5;
+        if (const char* v = std::getenv("CFD_LBM_WAVES")) waves = std::atol(v);
+        if (waves >= 3 && waves < 8) step_lds = static_cast<uint32_t>(163840 / waves - 1024);
 
         ops = cfd_hip::kernel_ops(q, storage);
         int count = 0;
