@@ -111,6 +111,28 @@ def test_adjoint_periodic_porous_matches_oracle(hip, oracle_mod):
     assert np.abs(grad - grado).max() <= 1e-9 * np.abs(grado).max()
 
 
+def test_progress_lines_leave_the_gradient_unchanged(hip, capfd, monkeypatch):
+    """CFD_LBM_PROGRESS_STEPS=k makes an adjoint run print a line on stderr after every k forward
+    sweep steps and every k transposed steps, each after the stream has finished them.  The run's
+    objective and gradients are the same bits as without the lines."""
+    report("adjoint_progress")
+    n = (16, 8, 8)
+    blend = 0.05 * design(duct(*n))
+    monkeypatch.delenv("CFD_LBM_PROGRESS_STEPS", raising=False)
+    quiet = gpu_duct(hip, n, blend, storage="fp32").adjoint_run_binomial(12, 4, 3)
+    capfd.readouterr()
+    monkeypatch.setenv("CFD_LBM_PROGRESS_STEPS", "5")
+    loud = gpu_duct(hip, n, blend, storage="fp32").adjoint_run_binomial(12, 4, 3)
+    err = capfd.readouterr().err
+    monkeypatch.delenv("CFD_LBM_PROGRESS_STEPS")
+    lines = [l for l in err.splitlines() if l.startswith("[adjoint]")]
+    assert [l.split(":")[0] for l in lines] == ["[adjoint] forward sweep"] * 2 + ["[adjoint] reverse"] * 2
+    assert lines[0].startswith("[adjoint] forward sweep: 5 of 12 steps")
+    assert lines[-1].startswith("[adjoint] reverse: 10 of 12 steps")
+    assert quiet[0] == loud[0] and quiet[2] == loud[2] and quiet[3] == loud[3]
+    assert np.array_equal(quiet[1], loud[1])
+
+
 def test_adjoint_checkpoint_interval_is_exact(hip):
     """Recomputed states are bit-identical to the original ones, so the gradient does not depend on
     the snapshot interval (including no recomputation at all)."""
