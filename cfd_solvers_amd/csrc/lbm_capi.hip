@@ -44,6 +44,15 @@ struct TransferStats {
     uint64_t h2d = 0, d2h = 0, d2d = 0, syncs = 0;
 };
 
+// A timing event, destroyed on every way out of the scope that made it.
+struct TimingEvent {
+    hipEvent_t e = nullptr;
+    TimingEvent() { hip_check(hipEventCreate(&e), "hipEventCreate"); }
+    ~TimingEvent() { (void)hipEventDestroy(e); }
+    TimingEvent(const TimingEvent&) = delete;
+    TimingEvent& operator=(const TimingEvent&) = delete;
+};
+
 template <class T> struct DeviceBuffer {
     T* ptr = nullptr;
     size_t bytes = 0;
@@ -936,10 +945,8 @@ struct cfd_lbm_solver {
                 a0 = m;
             }
         }
-        hipEvent_t e0, e1, e2;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        hip_check(hipEventCreate(&e1), "hipEventCreate");
-        hip_check(hipEventCreate(&e2), "hipEventCreate");
+        TimingEvent ev0, ev1, ev2;
+        hipEvent_t e0 = ev0.e, e1 = ev1.e, e2 = ev2.e;
         hip_check(hipEventRecord(e0, stream), "hipEventRecord");
         // Forward sweep.  A step whose result is a checkpoint writes it out of place into the
         // checkpoint's buffer, and the step after it reads from there: storing costs no extra
@@ -1015,9 +1022,6 @@ struct cfd_lbm_solver {
         sync();
         (void)hipEventElapsedTime(&last_forward_ms, e0, e1);
         (void)hipEventElapsedTime(&last_reverse_ms, e1, e2);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipEventDestroy(e2);
         last_forward_steps = forward_steps;
         if (grad_peak_out) {
             double sum = 0.0;
@@ -1141,24 +1145,14 @@ struct cfd_lbm_solver {
 
     float step_timed(uint64_t count) {
         use_device();
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        TimingEvent e0, e1;
         float ms = 0.0F;
-        try {
-            hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-            step(count);
-            hip_check(hipEventRecord(e1, stream), "hipEventRecord");
-            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-            ++stats.syncs;
-            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            throw;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        hip_check(hipEventRecord(e0.e, stream), "hipEventRecord");
+        step(count);
+        hip_check(hipEventRecord(e1.e, stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1.e), "hipEventSynchronize");
+        ++stats.syncs;
+        hip_check(hipEventElapsedTime(&ms, e0.e, e1.e), "hipEventElapsedTime");
         return ms;
     }
 
