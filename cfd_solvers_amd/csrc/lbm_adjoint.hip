@@ -413,6 +413,23 @@ __global__ __launch_bounds__(256) void plane_partial(const void* __restrict__ f,
     if (threadIdx.x == 0) a.scratch[blockIdx.x] = sm[0];
 }
 
+// partial[b] = sum of block b's contiguous share of v[0..n), in a fixed tree order.
+__global__ __launch_bounds__(256) void sum_chunks_f64(const double* __restrict__ v, uint64_t n,
+                                                      double* __restrict__ partial) {
+    __shared__ double sm[256];
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t begin = chunk * blockIdx.x, end = begin + chunk < n ? begin + chunk : n;
+    double s = 0.0;
+    for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) s += v[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
 __global__ __launch_bounds__(256) void fill_zero_f64(double* __restrict__ v, uint64_t n) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) v[i] = 0.0;
@@ -490,6 +507,12 @@ template <int Q, class T> struct AdjOps final : AdjointOps {
         hipLaunchKernelGGL(fill_zero_f64, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st.stream, v,
                            n);
         check(hipGetLastError(), "fill_zero launch");
+    }
+    void sum(const LaunchState& st, const double* v, uint64_t n, double* partial, double* out) const override {
+        zero(st, out, 1);
+        hipLaunchKernelGGL(sum_chunks_f64, dim3(kSumBlocks), dim3(256), 0, st.stream, v, n, partial);
+        hipLaunchKernelGGL(sum_partials, dim3(1), dim3(1024), 0, st.stream, partial, kSumBlocks, 1.0, out);
+        check(hipGetLastError(), "sum launch");
     }
 };
 

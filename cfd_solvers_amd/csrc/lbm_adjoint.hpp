@@ -39,7 +39,12 @@ struct AdjointOps {
     virtual void objective(const LaunchState& st, bool odd, bool bounded, const void* f, const AdjointArgs& a,
                            double weight, double* acc) const = 0;
     virtual void zero(const LaunchState& st, double* v, uint64_t n) const = 0;
+    // *out = sum of v[0..n) in a fixed order: kSumBlocks per-block partials (in `partial`, which
+    // holds kSumBlocks doubles), then one block over them.
+    virtual void sum(const LaunchState& st, const double* v, uint64_t n, double* partial, double* out) const = 0;
 };
+
+constexpr uint32_t kSumBlocks = 1024;
 
 const AdjointOps* adjoint_ops(int q, int storage);
 

@@ -1014,20 +1014,20 @@ struct cfd_lbm_solver {
             reverse(0, steps, shots[0].second, true);
         }
         hip_check(hipEventRecord(e2, stream), "hipEventRecord");
-        double J = 0.0;
+        // dJ/du_peak: the inlet cells' contributions summed on the device (fixed order).
+        DeviceBuffer<unsigned char> peak_sums;
+        peak_sums.allocate((cfd_hip::kSumBlocks + 1) * sizeof(double));
+        double* peak_out = reinterpret_cast<double*>(peak_sums.ptr) + cfd_hip::kSumBlocks;
+        adj->sum(st, args.grad_peak, cells, reinterpret_cast<double*>(peak_sums.ptr), peak_out);
+        double J = 0.0, du = 0.0;
         d2h(&J, acc.ptr, sizeof(double));
-        std::vector<double> peak(cells);
-        d2h(peak.data(), gp.ptr, cells * sizeof(double));
+        d2h(&du, peak_out, sizeof(double));
         if (grad_gamma_out && !grad_on_device) d2h(grad_gamma_out, gg.ptr, cells * sizeof(double));
         sync();
         (void)hipEventElapsedTime(&last_forward_ms, e0, e1);
         (void)hipEventElapsedTime(&last_reverse_ms, e1, e2);
         last_forward_steps = forward_steps;
-        if (grad_peak_out) {
-            double sum = 0.0;
-            for (double v : peak) sum += v;
-            *grad_peak_out = sum;
-        }
+        if (grad_peak_out) *grad_peak_out = du;
         if (forward_steps_out) *forward_steps_out = forward_steps;
         return J;
     }
