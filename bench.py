@@ -3,7 +3,7 @@
 
     python bench.py [--steps K] [--warmup W]
                     [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1,config5]
-                    [--config5-steps T] [--no-cpu]
+                    [--config5-steps T] [--config5-storages fp16s,fp32] [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
@@ -18,11 +18,12 @@ z-slabs, and config1 the
 cache-resident 64^3 Taylor-Green correctness case, and the config5
 leg forward + adjoint of a 768^3 duct past a sphere cluster (drag objective, binomial
 checkpointing) with FP16 and FP32 storage, on a horizon truncated to 200 steps by default, with the
-plan's cost at the specified 10^4 steps projected from the measured step times.  Prints progress on stderr and ONE JSON
+plan's cost at the specified 10^4 steps projected from the measured step times
+(`--config5-steps 10000` runs it: ~10 min with FP16 storage).  Prints progress on stderr and ONE JSON
 line on stdout.  MLUPS counts every lattice cell per step, as the reference's cfd-bench does
 (apps/cfd-bench.cpp:111-117), and `fluid_mlups` beside it counts only the cells a step updates
 (wall cells return at once); the roofline uses algorithmic HBM bytes per launch (each fluid
-population read and written once; the bounded kernel's metadata -- one 2-byte span per 64-cell row
+population read and written once; the bounded kernel's metadata -- one 4-byte span per 64-cell row
 segment and a 4-byte link descriptor per cell inside a span -- is overhead, not algorithmic, and
 shows in the measured `traffic`) over the kernel time measured with HIP events on the solver's
 stream.  Two probes run in the same process: a plain 2 GiB copy
