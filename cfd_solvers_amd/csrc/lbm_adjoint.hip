@@ -234,7 +234,9 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
         if (c != Real(0)) static_for<Q>([&](auto DD) { mu[decltype(DD)::value] += c; });
     }
     // lambda_t at the slots this cell read; zero on bounce-back store slots that were not read.
-    store_at<Real>(plane<Real>(lam_buf, g, 0), off_n, mu[0]);
+    // (Each store offset passes its own barrier: one barrier per distinct offset measured one
+    // VGPR more here.)
+    store_at<Real>(plane<Real>(lam_buf, g, 0), opaque_offset(off_n), mu[0]);
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
         const uint32_t off_up = s.up[i] / static_cast<uint32_t>(sizeof(T)) * rb;
@@ -245,10 +247,10 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
         }
         const uint32_t sel_a = (Odd != wup) ? plane_bytes : 0U;
         const uint32_t sel_b = (Odd != wdn) ? 0U : plane_bytes;
-        store_at<Real>(plane<Real>(lam_buf, g, i), off_up + sel_a, mu[i]);
-        store_at<Real>(plane<Real>(lam_buf, g, i), off_n + sel_b, mu[i + 1]);
-        if (wup) store_at<Real>(plane<Real>(lam_buf, g, Odd ? i + 1 : i), off_up, Real(0));
-        if (wdn) store_at<Real>(plane<Real>(lam_buf, g, Odd ? i : i + 1), off_n, Real(0));
+        store_at<Real>(plane<Real>(lam_buf, g, i), opaque_offset(off_up + sel_a), mu[i]);
+        store_at<Real>(plane<Real>(lam_buf, g, i), opaque_offset(off_n + sel_b), mu[i + 1]);
+        if (wup) store_at<Real>(plane<Real>(lam_buf, g, Odd ? i + 1 : i), opaque_offset(off_up), Real(0));
+        if (wdn) store_at<Real>(plane<Real>(lam_buf, g, Odd ? i : i + 1), opaque_offset(off_n), Real(0));
     });
     if (p.gamma && flag == kFluid) a.grad_gamma[n] += dgamma;
     if (flag == kInlet) a.grad_peak[n] += dpeak;

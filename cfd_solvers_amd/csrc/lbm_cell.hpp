@@ -27,15 +27,22 @@ template <class T> __device__ __forceinline__ T load_at(const void* base, uint32
     return *p;
 #endif
 }
+// byte_off should be opaque to the optimiser at the store (store_cell passes it through
+// opaque_offset): otherwise it materialises all 64-bit store addresses before the collision and
+// keeps them live (2 VGPRs each), which halves the occupancy.
 template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t byte_off, T v) {
-    // Opaque to the optimiser: otherwise it materialises all 64-bit store addresses before the
-    // collision and keeps them live (2 VGPRs each), which halves the occupancy.
-    asm volatile("" : "+v"(byte_off));
     T* p = reinterpret_cast<T*>(static_cast<unsigned char*>(base) + byte_off);
     // Streaming stores: no slot written in a step is read again within that step, and the whole
     // lattice passes through the caches every step (measured ~1 % faster than plain stores).
     if constexpr (__is_same(T, F32Dev)) __builtin_nontemporal_store(v.v, reinterpret_cast<float*>(p));
     else __builtin_nontemporal_store(v, p);
+}
+
+// An offset the optimiser cannot look through, once per distinct offset: a barrier at every store
+// would copy an offset that several stores share.
+__device__ __forceinline__ uint32_t opaque_offset(uint32_t byte_off) {
+    asm volatile("" : "+v"(byte_off));
+    return byte_off;
 }
 
 // The link descriptor of cell n on a bounded lattice (0 on a periodic one).
@@ -121,16 +128,17 @@ __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Phy
             });
         }
     }
-    const uint32_t off_n = s.self;
+    const uint32_t off_n = opaque_offset(s.self);
     store_at<T>(plane<T>(f, g, 0), off_n, S::template encode<Q, 0>(fout[0]));
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
+        const uint32_t off_up = opaque_offset(s.up[i]);
         if constexpr (Odd) {
             store_at<T>(plane<T>(f, g, i), off_n, S::template encode<Q, i>(fout[i]));
-            store_at<T>(plane<T>(f, g, i + 1), s.up[i], S::template encode<Q, i + 1>(fout[i + 1]));
+            store_at<T>(plane<T>(f, g, i + 1), off_up, S::template encode<Q, i + 1>(fout[i + 1]));
         } else {
             store_at<T>(plane<T>(f, g, i + 1), off_n, S::template encode<Q, i>(fout[i]));
-            store_at<T>(plane<T>(f, g, i), s.up[i], S::template encode<Q, i + 1>(fout[i + 1]));
+            store_at<T>(plane<T>(f, g, i), off_up, S::template encode<Q, i + 1>(fout[i + 1]));
         }
     });
 }
