@@ -2,14 +2,14 @@
 """Benchmark of the gfx950 esoteric-pull stream-collide (one GPU).
 
     python bench.py [--steps K] [--warmup W]
-                    [--legs fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1,config5]
+                    [--legs fp32,fp16,fp32s,fp64,d3q27,d2q9,adjoint,voxelize,resident,shard,config1,config5]
                     [--config5-steps T] [--config5-storages fp16s,fp32] [--no-cpu]
 
 A step is one fused in-place stream-collide launch over the whole lattice, inputs resident in
 HBM.  The headline workload is a D3Q19 256^3 lid-driven cavity (walls on five faces, lid
 u = (0.1, 0, 0), tau 0.55, uint8 flags) with FP32 population storage; further legs run the same
 geometry with FP16 and FP32 shifted-population storage, FP64 (the reference for the velocity
-error of the others) and D3Q27, and the adjoint leg runs the
+error of the others) and D3Q27, the d2q9 leg a 4096^2 D2Q9 cavity (FP32 and FP16), and the adjoint leg runs the
 checkpointed discrete adjoint of a 128^3 duct (pressure-drop objective, 2000 steps), and the
 voxelize leg the GPU STL voxelisation of a 16128-triangle sphere on 512^3 cells, and the resident
 leg the device-resident coupling modules (thermal, fill, particles, Q-criterion) at 256^3, and the shard
@@ -114,6 +114,43 @@ def run_leg(hip, name, lattice, storage, n, tau, steps, warmup):
         f"over {steady_steps} more steps {steady_ms:.3f} ms ({100 * res['steady_state']['roofline_frac']:.1f}%)")
     s.close()
     return res, velocity
+
+
+def d2q9_leg(hip, n=4096, steps=20, warmup=5, storages=("fp32", "fp16s")):
+    """D2Q9 lid-driven cavity on n x n cells (16.8 M at 4096^2, as many as the 256^3 legs): lid
+    y = n-1 moving at (0.1, 0), walls x = 0, n-1 and y = 0 at rest, tau 0.55, FP32 and FP16S
+    storage.  Algorithmic bytes: each fluid cell's 9 populations read and written once."""
+    out = {"workload": f"D2Q9 {n}^2 lid-driven cavity, lid u=(0.1,0), tau 0.55"}
+    f = np.zeros((n, n), dtype=np.uint8)
+    f[-1, :] = 0x81
+    f[:, 0] = f[:, -1] = 0x80
+    f[0, :] = 0x80
+    flags = f.reshape(-1)
+    fluid = int(np.count_nonzero(flags == 0))
+    for storage in storages:
+        s = hip.EsotericPullHipSolver("d2q9", n, n, 1, 0.55, storage=storage)
+        s.set_flags(flags)
+        s.set_wall_velocity(1, (0.1, 0.0, 0.0))
+        s.initialize_uniform()
+        s.step(warmup)
+        s.wait()
+        gpu_ms = s.step_timed(steps)
+        steady_ms = s.step_timed(200) / 200
+        mass, _, vmax = s.diagnostics()
+        s.close()
+        bytes_launch = fluid * 2 * 9 * BYTES_PER_POP[storage]
+        kernel_ms = gpu_ms / steps
+        res = {"kernel_ms": kernel_ms, "kernel_mlups": n * n / (kernel_ms * 1e3),
+               "algorithmic_bytes_per_launch": bytes_launch, "gbs": bytes_launch / (kernel_ms * 1e-3) / 1e9,
+               "roofline_frac": bytes_launch / (kernel_ms * 1e-3) / 1e9 / PEAK_GBS,
+               "steady_state": {"steps": 200, "kernel_ms": steady_ms,
+                                "roofline_frac": bytes_launch / (steady_ms * 1e-3) / 1e9 / PEAK_GBS},
+               "finite": bool(np.isfinite(mass) and np.isfinite(vmax))}
+        out[storage] = res
+        log(f"d2q9 {storage}: {res['kernel_mlups']:.0f} MLUPS, kernel {kernel_ms:.3f} ms, {res['gbs']:.0f} GB/s "
+            f"({100 * res['roofline_frac']:.1f}% of 8 TB/s); over 200 more steps {steady_ms:.3f} ms "
+            f"({100 * res['steady_state']['roofline_frac']:.1f}%)")
+    return out
 
 
 def config1_leg(hip, n=64, steps=100, warmup=10):
@@ -613,7 +650,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,adjoint,voxelize,resident,shard,config1,config5")
+    ap.add_argument("--legs", default="fp32,fp16,fp32s,fp64,d3q27,d2q9,adjoint,voxelize,resident,shard,config1,config5")
     ap.add_argument("--config5-steps", type=int, default=200)
     ap.add_argument("--config5-storages", default="fp16s,fp32")
     ap.add_argument("--config5-n", type=int, default=768)
@@ -656,6 +693,9 @@ def main():
             continue
         if name == "config1":
             legs[name] = config1_leg(hip)
+            continue
+        if name == "d2q9":
+            legs[name] = d2q9_leg(hip, steps=args.steps, warmup=args.warmup)
             continue
         lattice, storage = table[name]
         legs[name], vel = run_leg(hip, name, lattice, storage, args.n, 0.55, args.steps, args.warmup)
