@@ -395,13 +395,15 @@ def test_shifted_boundaries_parity(hip, oracle_mod, storage):
 
 RAGGED = [(0, 19, "fp32", (67, 9, 7)), (1, 27, "fp32", (130, 5, 6)), (2, 19, "fp16s", (300, 4, 3)),
           (3, 19, "fp32s", (3, 40, 9)), (4, 27, "fp64", (65, 10, 5)), (5, 19, "fp64", (96, 7, 2)),
-          (6, 27, "fp16s", (129, 6, 4)), (7, 19, "fp32", (257, 3, 3))]
+          (6, 27, "fp16s", (129, 6, 4)), (7, 19, "fp32", (257, 3, 3)),
+          # rows wider than the default population block (2048 cells FP32, 8192 FP16): one row per block
+          (8, 19, "fp32", (2100, 4, 3)), (9, 27, "fp32", (2049, 3, 2)), (10, 19, "fp16s", (8230, 3, 2))]
 
 
 @pytest.mark.parametrize("seed,q,storage,n", RAGGED)
 def test_random_walls_on_ragged_lattices(hip, oracle_mod, seed, q, storage, n):
     """Seeded random geometry on extents that leave partial tiles and 64-cell segments (nx above
-    256, not a multiple of 64, 3 cells wide; nz = 2): solid boxes, scattered wall cells and a moving
+    256, not a multiple of 64, 3 cells wide, wider than a population block; nz = 2): solid boxes, scattered wall cells and a moving
     y = ny-1 plane with one of three wall velocities, uniform forcing, Taylor-Green start with noise.
     Every storage and both 3-D lattices, bit for bit against the oracle in the fluid cells after an
     odd number of steps (both parities)."""
