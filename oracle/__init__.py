@@ -7,6 +7,7 @@ built from the reference's own sources by ``oracle/Makefile``.  Only ``tests/``,
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -230,6 +231,19 @@ class Oracle:
     @time_step.setter
     def time_step(self, t):
         self.L.orc_set_time_step(self.h, t)
+
+
+@contextlib.contextmanager
+def exact_fp64_weights():
+    """FP64 oracles created and stepped inside use the lattice weights' exact values instead of the
+    reference's FP32 constants (a precision-study switch with no reference counterpart)."""
+    L = lib()
+    L.orc_set_exact_fp64_weights.argtypes = [C.c_int]
+    L.orc_set_exact_fp64_weights(1)
+    try:
+        yield
+    finally:
+        L.orc_set_exact_fp64_weights(0)
 
 
 def f32_to_f16_bits(values):

@@ -54,6 +54,8 @@ template <> struct Lat<9> {
     static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     static constexpr float w[9] = {4.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F, 1.0F / 9.0F,
                                    1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+    static constexpr double w_exact[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
+                                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 };
 template <> struct Lat<19> {
     static constexpr int dims = 3;
@@ -64,6 +66,10 @@ template <> struct Lat<19> {
                                     1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
                                     1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
                                     1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+    static constexpr double w_exact[19] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                                           1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                           1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                           1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 };
 template <> struct Lat<27> {
     static constexpr int dims = 3;
@@ -78,7 +84,18 @@ template <> struct Lat<27> {
         1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,
         1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 216.0F, 1.0F / 216.0F,
         1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F};
+    static constexpr double w_exact[27] = {
+        8.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,
+        1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,
+        1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 216.0, 1.0 / 216.0,
+        1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0};
 };
+
+// FP64 arithmetic with the weights' exact values instead of the reference's FP32 constants, which
+// sum to 1 + 1.49e-8 (D3Q19) and 1 + 7.45e-9 (D2Q9, D3Q27): with those, every BGK collision of an
+// FP64 run adds omega times the excess to the cell's density.  No reference counterpart: a switch
+// for precision studies, off by default and process-wide (orc_set_exact_fp64_weights).
+inline bool g_exact_fp64_weights = false;
 
 inline int opposite(int i) { return i == 0 ? 0 : ((i & 1) != 0 ? i + 1 : i - 1); }
 
@@ -537,7 +554,12 @@ private:
         return (shift(z, L::cz[d], nz_) * ny_ + shift(y, L::cy[d], ny_)) * nx_ + shift(x, L::cx[d], nx_);
     }
 
-    static Real w(int d) { return static_cast<Real>(L::w[d]); }
+    static Real w(int d) {
+        if constexpr (std::is_same_v<Real, double>) {
+            if (g_exact_fp64_weights) return L::w_exact[d];
+        }
+        return static_cast<Real>(L::w[d]);
+    }
     static Real feq(int d, Real rho, Real ux, Real uy, Real uz) {
         const Real cu = Real(3) * (static_cast<Real>(L::cx[d]) * ux + static_cast<Real>(L::cy[d]) * uy +
                                    static_cast<Real>(L::cz[d]) * uz);
@@ -1040,7 +1062,8 @@ private:
 
     // d feq_d / d(rho, u) for the second-order equilibrium, in double.
     static void feq_grad(int d, double rho, double ux, double uy, double uz, double& e, double (&gu)[3]) {
-        const double cx = L::cx[d], cy = L::cy[d], cz = L::cz[d], w = static_cast<double>(L::w[d]);
+        const double cx = L::cx[d], cy = L::cy[d], cz = L::cz[d];
+        const double w = g_exact_fp64_weights ? L::w_exact[d] : static_cast<double>(L::w[d]);
         const double cu = cx * ux + cy * uy + cz * uz;
         const double uu = ux * ux + uy * uy + uz * uz;
         e = w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
@@ -1590,6 +1613,8 @@ ORC_API size_t orc_cells(void* h) { return H(h)->cells(); }
 ORC_API int orc_objective(void* h, int kind, int x_in, int x_out, double* out) {
     return guarded([&] { *out = H(h)->objective(kind, x_in, x_out); });
 }
+ORC_API void orc_set_exact_fp64_weights(int enabled) { orc::g_exact_fp64_weights = enabled != 0; }
+
 ORC_API int orc_set_smagorinsky(void* h, int enabled) {
     return guarded([&] { H(h)->set_smagorinsky(enabled != 0); });
 }

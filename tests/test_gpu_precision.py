@@ -1,9 +1,31 @@
-"""Storage precisions against the FP64 build of the same kernel.
+"""Storage precisions against the FP64 build of the same kernel, per workload: density, velocity
+and (where the workload has one) the objective and its gradient.
 
-* D3Q19 256^3 lid-driven cavity (the bench workload): relative L2 error of the velocity field.
+* Taylor-Green vortex, D3Q19 64^3 periodic, tau 0.6, amplitude 0.05 + noise 1e-4, 110 steps.
+* D3Q19 256^3 lid-driven cavity (the bench workload), 1000 steps.
+* D3Q27 256^3 lid-driven cavity, lid (0.1, 0, 0), tau 0.55, 5000 steps.
 * Porous channel (D3Q19 512 x 256 x 256, periodic in x, walls in y/z, non-overlapping spheres of
   radius U[4, 12] (seed 3) packed to porosity 0.8, Guo forcing g_x = 1e-6, tau 0.6, 20 000 steps):
-  error of the mean fluid velocity and of the permeability k = nu <u_x>_superficial / g_x.
+  also the mean fluid velocity and the permeability k = nu <u_x>_superficial / g_x.
+* Porous duct, D3Q19 128^3, velocity inlet (peak 0.05), pressure outlet, design field
+  0.01 * U[0.3, 0.7] (seed 4), T = 2000: the pressure drop over the last 200 states, dJ/dgamma and
+  dJ/du_peak from the binomial adjoint.
+* Sphere-cluster duct (64 spheres of radius U[10, 30], seed 5, partial-bounce-back halo, tau 0.52,
+  inlet 0.04, impulsive start), T = 200, drag over all states: at 768^3 against FP32 storage (an
+  FP64 lattice of 768^3 exceeds the FP64 limit of 2^28 cells), and the same generator at 512^3
+  against FP64, with the adjoint gradient.
+
+Density error: max |rho - rho_64| / rho_64, and the relative L2 error of the density fluctuation
+rho - mean(rho) (the pressure field) with each run's own mean removed.  Velocity error: relative L2
+over fluid cells of the vector field.  Fields come from the FP64 moment download, so the comparison
+adds no rounding of its own.  Closed lattices also report each run's relative mass drift.
+
+The FP64 build keeps the reference's lattice weights, which are FP32 constants (as in its double
+PrecisionPullSolver): they sum to 1 + 1.49e-8 (D3Q19) or 1 + 7.45e-9 (D3Q27, D2Q9), and every BGK
+collision adds omega times that excess to a cell's density, so the FP64 and FP32 builds gain mass at
+~2.5e-8 of the total per step (tau 0.6, D3Q19).  The shifted storages apply the weights to
+deviations only and conserve mass to rounding: their max density error against FP64 is the FP64
+build's drift, and the fluctuation error removes it.
 
 Collision arithmetic is FP32 except in the FP64 build: the reference's arithmetic on full
 populations for fp32, the same arithmetic on deviations f - w_d for the shifted storages fp32s
@@ -17,6 +39,8 @@ import pytest
 from conftest import report
 
 pytestmark = pytest.mark.gpu
+
+SHIFTED_AND_FP32 = ("fp32", "fp32s", "fp16s")
 
 
 def cavity_flags(n):
@@ -32,26 +56,112 @@ def rel_l2(a, b):
     return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b ** 2)))
 
 
+def field_errors(m, ref, fluid, slab=16):
+    """(max relative density error, relative L2 error of the density fluctuation rho - mean(rho),
+    relative L2 error of u) of the moment fields m against ref over the fluid cells (bool, shape
+    (nz, ny, nx)), accumulated in FP64 over slabs of z planes so that lattices of 10^8 cells need
+    no full-size temporaries.  With d = rho - rho_ref and e = rho_ref - 1 over N cells,
+    sum((d - mean d)^2) = sum(d^2) - N mean(d)^2, and likewise for e."""
+    rho_max = sd = sd2 = se = se2 = du2 = u2 = 0.0
+    count = 0
+    for z0 in range(0, fluid.shape[0], slab):
+        sl = slice(z0, z0 + slab)
+        f = fluid[sl]
+        r = m.rho[sl][f].astype(np.float64)
+        r0 = ref.rho[sl][f].astype(np.float64)
+        if r0.size:
+            rho_max = max(rho_max, float(np.max(np.abs(r - r0) / r0)))
+        d, e = r - r0, r0 - 1.0
+        count += d.size
+        sd += float(np.sum(d))
+        sd2 += float(np.sum(d * d))
+        se += float(np.sum(e))
+        se2 += float(np.sum(e * e))
+        for k in ("ux", "uy", "uz"):
+            a = getattr(m, k)[sl][f].astype(np.float64)
+            b = getattr(ref, k)[sl][f].astype(np.float64)
+            du2 += float(np.sum((a - b) ** 2))
+            u2 += float(np.sum(b ** 2))
+    fluct = np.sqrt(max(sd2 - sd * sd / count, 0.0) / (se2 - se * se / count))
+    return rho_max, float(fluct), float(np.sqrt(du2 / u2))
+
+
+def mass_drift(solver, steps):
+    m0 = solver.total_mass()
+    solver.step(steps)
+    return (solver.total_mass() - m0) / m0
+
+
+def report_errors(case, storage, e, **extra):
+    report(case, storage=storage, rho_max_rel=f"{e[0]:.3e}", rho_fluct_rel_l2=f"{e[1]:.3e}", u_rel_l2=f"{e[2]:.3e}",
+           **{k: (f"{v:.3e}" if isinstance(v, float) else v) for k, v in extra.items()})
+
+
+def test_taylor_green_64_errors_vs_fp64(hip):
+    n, steps = 64, 110
+    fields, drift = {}, {}
+    for storage in ("fp64",) + SHIFTED_AND_FP32:
+        report("taylor_green_precision", storage=storage, n=n, steps=steps)
+        s = hip.EsotericPullHipSolver(19, n, n, n, 0.6, storage=storage)
+        s.initialize_taylor_green(0.05, 1e-4, 11)
+        drift[storage] = mass_drift(s, steps)
+        fields[storage] = s.download_macroscopic(f64=True)
+        s.close()
+    fluid = np.ones((n, n, n), bool)
+    e = {k: field_errors(fields[k], fields["fp64"], fluid) for k in SHIFTED_AND_FP32}
+    for k in SHIFTED_AND_FP32:
+        report_errors("taylor_green_precision_result", k, e[k], mass_drift=drift[k])
+    report("taylor_green_precision_fp64", mass_drift=drift["fp64"])
+    for k in ("fp32", "fp32s"):
+        assert e[k][0] <= 1e-5 and e[k][1] <= 2e-3 and e[k][2] <= 1e-5
+    assert e["fp16s"][0] <= 1e-4 and e["fp16s"][1] <= 3e-2 and e["fp16s"][2] <= 1e-3
+    assert abs(drift["fp32s"]) <= 1e-7 and abs(drift["fp16s"]) <= 1e-6
+
+
 def test_cavity_256_velocity_error_vs_fp64(hip):
     n, steps = 256, 1000
     flags = cavity_flags(n)
-    fluid = flags.reshape(n, n, n) == 0
-    u = {}
-    for storage in ("fp64", "fp32", "fp32s", "fp16s"):
+    fluid = (flags == 0).reshape(n, n, n)
+    fields, drift = {}, {}
+    for storage in ("fp64",) + SHIFTED_AND_FP32:
         report("cavity_precision", storage=storage, n=n, steps=steps)
         s = hip.EsotericPullHipSolver(19, n, n, n, 0.55, storage=storage)
         s.set_flags(flags)
         s.set_wall_velocity(1, (0.1, 0.0, 0.0))
         s.initialize_uniform()
-        s.step(steps)
-        m = s.download_macroscopic()
-        u[storage] = np.stack([m.ux[fluid], m.uy[fluid], m.uz[fluid]]).astype(np.float64)
+        drift[storage] = mass_drift(s, steps)
+        fields[storage] = s.download_macroscopic(f64=True)
         s.close()
-    e = {k: rel_l2(u[k], u["fp64"]) for k in ("fp32", "fp32s", "fp16s")}
-    report("cavity_precision_result", rel_l2_fp32=e["fp32"], rel_l2_fp32s=e["fp32s"], rel_l2_fp16s=e["fp16s"],
-           u_max=np.abs(u["fp64"]).max())
-    assert e["fp32"] <= 5e-4 and e["fp32s"] <= 5e-4
-    assert e["fp16s"] <= 2e-3
+    e = {k: field_errors(fields[k], fields["fp64"], fluid) for k in SHIFTED_AND_FP32}
+    for k in SHIFTED_AND_FP32:
+        report_errors("cavity_precision_result", k, e[k], mass_drift=drift[k])
+    report("cavity_precision_fp64", mass_drift=drift["fp64"])
+    assert e["fp32"][2] <= 5e-4 and e["fp32s"][2] <= 5e-4
+    assert e["fp16s"][2] <= 1e-3
+    assert max(e[k][0] for k in SHIFTED_AND_FP32) <= 5e-4
+
+
+def test_cavity_d3q27_256_errors_vs_fp64(hip):
+    n, steps = 256, 5000
+    flags = cavity_flags(n)
+    fluid = (flags == 0).reshape(n, n, n)
+    fields, drift = {}, {}
+    for storage in ("fp64",) + SHIFTED_AND_FP32:
+        report("cavity27_precision", storage=storage, n=n, steps=steps)
+        s = hip.EsotericPullHipSolver(27, n, n, n, 0.55, storage=storage)
+        s.set_flags(flags)
+        s.set_wall_velocity(1, (0.1, 0.0, 0.0))
+        s.initialize_uniform()
+        drift[storage] = mass_drift(s, steps)
+        fields[storage] = s.download_macroscopic(f64=True)
+        s.close()
+    e = {k: field_errors(fields[k], fields["fp64"], fluid) for k in SHIFTED_AND_FP32}
+    for k in SHIFTED_AND_FP32:
+        report_errors("cavity27_precision_result", k, e[k], mass_drift=drift[k])
+    report("cavity27_precision_fp64", mass_drift=drift["fp64"])
+    assert e["fp32"][2] <= 1e-3 and e["fp32s"][2] <= 1e-3
+    assert e["fp16s"][2] <= 2e-3
+    assert max(e[k][0] for k in SHIFTED_AND_FP32) <= 1e-3
 
 
 def test_porous_channel_permeability_vs_fp64(hip):
@@ -69,23 +179,132 @@ def test_porous_channel_permeability_vs_fp64(hip):
     porosity = fluid.sum() / interior
     nu = (tau - 0.5) / 3.0
     res = {}
-    for storage in ("fp64", "fp32", "fp32s", "fp16s"):
+    for storage in ("fp64",) + SHIFTED_AND_FP32:
         report("porous_precision", storage=storage, spheres=len(radii), porosity=porosity, steps=steps)
         s = hip.EsotericPullHipSolver(19, nx, ny, nz, tau, acceleration=(g, 0.0, 0.0), storage=storage)
         s.set_flags(flags.reshape(-1))
         s.initialize_uniform()
-        s.step(steps)
-        m = s.download_macroscopic()
-        ux = m.ux[fluid].astype(np.float64)
-        res[storage] = (ux.mean(), nu * ux.sum() / interior / g, ux)
+        drift = mass_drift(s, steps)
+        m = s.download_macroscopic(f64=True)
+        ux = m.ux[fluid]
+        res[storage] = (ux.mean(), nu * ux.sum() / interior / g, m, drift)
         s.close()
-    u64, k64, f64 = res["fp64"]
+    u64, k64, m64, drift64 = res["fp64"]
     out = {}
-    for storage in ("fp32", "fp32s", "fp16s"):
-        u, k, f = res[storage]
-        out[storage] = (abs(u - u64) / abs(u64), abs(k - k64) / abs(k64), rel_l2(f, f64))
-    report("porous_precision_result", mean_u_fp64=u64, k_fp64=k64,
-           **{f"{s}_{q}": out[s][i] for s in out for i, q in enumerate(("mean_u_err", "k_err", "rel_l2"))})
+    for storage in SHIFTED_AND_FP32:
+        u, k, m, drift = res[storage]
+        out[storage] = (abs(u - u64) / abs(u64), abs(k - k64) / abs(k64)) + field_errors(m, m64, fluid)
+        report_errors("porous_precision_result", storage, out[storage][2:], mean_u_err=out[storage][0],
+                      k_err=out[storage][1], mass_drift=drift)
+    report("porous_precision_fp64", mean_u=u64, k=k64, mass_drift=drift64)
     assert u64 > 0 and np.isfinite(u64)
     assert out["fp32s"][1] <= 1e-4
     assert out["fp16s"][1] <= 2e-3
+
+
+def porous_duct(n):
+    nx, ny, nz = n
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, 0, :] = f[:, -1, :] = 0x80
+    f[0, :, :] = f[-1, :, :] = 0x80
+    f[1:-1, 1:-1, 0] = 0x01
+    f[1:-1, 1:-1, -1] = 0x02
+    f = f.reshape(-1)
+    gamma = np.where(f == 0, np.random.default_rng(4).uniform(0.3, 0.7, f.size), 0.0)
+    return f, (0.01 * gamma).astype(np.float32)
+
+
+def adjoint_errors(case, runs, fluid, extra_keys=()):
+    """Relative errors of J, dJ/dgamma (L2 over fluid cells), dJ/du_peak and the fields of each run
+    against the first (reference) run; runs = {storage: (J, grad, du, fields)}."""
+    ref_name = next(iter(runs))
+    J0, g0, du0, m0 = runs[ref_name]
+    out = {}
+    for k, (J, g, du, m) in runs.items():
+        if k == ref_name:
+            continue
+        ej = abs(J - J0) / abs(J0)
+        eg = rel_l2(g[fluid], g0[fluid])
+        edu = abs(du - du0) / abs(du0) if du0 != 0.0 else abs(du)
+        out[k] = (ej, eg, edu) + field_errors(m, m0, fluid)
+        report_errors(case, k, out[k][3:], vs=ref_name, J_rel=ej, grad_rel_l2=eg, du_peak_rel=edu)
+    return out
+
+
+def test_porous_duct_128_objective_and_gradient_vs_fp64(hip):
+    n = (128, 128, 128)
+    T, W = 2000, 200
+    flags, blend = porous_duct(n)
+    fluid = (flags == 0).reshape(n[::-1])
+    runs = {}
+    for storage in ("fp64",) + SHIFTED_AND_FP32:
+        report("duct_precision", storage=storage, n=n, steps=T, window=W)
+        s = hip.EsotericPullHipSolver(19, *n, 0.6, storage=storage)
+        s.set_flags(flags)
+        s.set_inlet_outlet(0.05, 1.0)
+        s.set_porosity(blend)
+        s.initialize_uniform()
+        J, grad, du, _ = s.adjoint_run_binomial(T, W, 0, "pressure_drop", 1, n[0] - 2)
+        runs[storage] = (J, grad, du, s.download_macroscopic(f64=True))
+        s.close()
+    out = adjoint_errors("duct_precision_result", runs, fluid)
+    # fp32 shares the FP64 build's weight constants; fp32s differs from it by their mass source.
+    assert max(out["fp32"][i] for i in (0, 1, 2, 5)) <= 2e-5
+    assert max(out["fp32s"][i] for i in (0, 1, 2, 5)) <= 3e-4
+    assert out["fp16s"][0] <= 1e-3 and out["fp16s"][1] <= 2e-3 and out["fp16s"][2] <= 1e-3
+    assert out["fp16s"][5] <= 2e-3
+
+
+def cluster_run(hip, cluster, n, storage, steps, adjoint):
+    flags, body, halo = cluster
+    s = hip.EsotericPullHipSolver(19, n, n, n, 0.52, storage=storage)
+    try:
+        s.set_flags(flags)
+        s.set_drag_body(body, 0)
+        s.set_inlet_outlet(0.04, 1.0)
+        s.set_porosity(halo)
+        s.initialize_uniform(1.0, 0.04, 0.0, 0.0)  # impulsive start: the inflow everywhere
+        if adjoint:
+            J, grad, du, _ = s.adjoint_run_binomial(steps, steps, 0, "drag")
+        else:
+            J, grad, du = s.objective_run(steps, steps, "drag"), None, 0.0
+        return J, grad, du, s.download_macroscopic(f64=True)
+    finally:
+        s.close()
+
+
+def test_sphere_cluster_512_drag_and_gradient_vs_fp64(hip):
+    from cfd_solvers_amd import geometry as G
+
+    n, T = 512, 200
+    cluster = G.sphere_cluster_duct(n, seed=5)
+    fluid = (cluster[0] == 0).reshape(n, n, n)
+    runs = {}
+    for storage in ("fp64",) + SHIFTED_AND_FP32:
+        report("cluster512_precision", storage=storage, n=n, steps=T)
+        runs[storage] = cluster_run(hip, cluster, n, storage, T, adjoint=True)
+    out = adjoint_errors("cluster512_precision_result", runs, fluid)
+    for k in SHIFTED_AND_FP32:
+        assert np.isfinite(runs[k][0]) and runs[k][0] > 0
+        assert out[k][0] <= 1e-3 and out[k][1] <= 2e-3 and out[k][5] <= 2e-3
+
+
+def test_sphere_cluster_768_drag_vs_fp32(hip):
+    from cfd_solvers_amd import geometry as G
+
+    n, T = 768, 200
+    cluster = G.sphere_cluster_duct(n, seed=5)
+    fluid = (cluster[0] == 0).reshape(n, n, n)
+    runs = {}
+    for storage in ("fp32", "fp32s", "fp16s"):
+        report("cluster768_precision", storage=storage, n=n, steps=T)
+        J, _, _, m = cluster_run(hip, cluster, n, storage, T, adjoint=False)
+        runs[storage] = (J, np.zeros(1), 0.0, m)
+    out = {}
+    J0, m0 = runs["fp32"][0], runs["fp32"][3]
+    for k in ("fp32s", "fp16s"):
+        ej = abs(runs[k][0] - J0) / abs(J0)
+        out[k] = (ej,) + field_errors(runs[k][3], m0, fluid)
+        report_errors("cluster768_precision_result", k, out[k][1:], vs="fp32", J_rel=ej)
+    assert out["fp16s"][0] <= 1e-3 and out["fp16s"][3] <= 2e-3
+    assert out["fp32s"][0] <= 1e-5 and out["fp32s"][3] <= 2e-4

@@ -308,6 +308,72 @@ def test_fp32_vs_fp64():
     assert np.abs(fields(a) - fields(b)).max() < 1e-6
 
 
+@pytest.mark.parametrize("q,expected_excess", [(9, 7.450580596923828e-09), (19, 1.4901161193847656e-08),
+                                               (27, 7.450580596923828e-09)])
+def test_fp32_weight_constants_make_fp64_gain_mass(q, expected_excess):
+    """The reference's lattice weights are FP32 constants (descriptors.hpp:42-53, 84-95, 129-142)
+    whose sum exceeds 1.  Each BGK collision moves the density by omega (sum_d f_eq - rho) =
+    omega rho (excess + O(|u|^2 excess)), so a periodic FP64 run gains mass as
+    (1 + omega excess)^steps - 1; with the exact weights it conserves mass to rounding, and so do
+    the shifted storages, which multiply the weights into deviations only."""
+    excess = sum(float(np.float32(w)) for w in {
+        9: [4 / 9] + [1 / 9] * 4 + [1 / 36] * 4,
+        19: [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12,
+        27: [8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8}[q]) - 1.0
+    assert excess == expected_excess
+    tau, steps = F32(0.6), 40
+    n = (16, 16, 1) if q == 9 else (16, 16, 16)
+
+    def drift(storage):
+        o = O.Oracle(q, *n, tau, storage=storage)
+        o.init_taylor_green(F32(0.02))
+        m0 = o.mass()
+        o.step(steps)
+        return (o.mass() - m0) / m0
+
+    predicted = (1.0 + excess / tau) ** steps - 1.0
+    assert drift("fp64") == pytest.approx(predicted, rel=2e-3)
+    with O.exact_fp64_weights():
+        assert abs(drift("fp64")) < 1e-13
+    assert abs(drift("fp32s")) < 2e-3 * predicted
+
+
+def test_fp32_weight_constants_bias_the_duct_objective():
+    """Pressure drop of a porous duct (24 x 10 x 10, inlet peak 0.05, design field 0.01 U[0.3, 0.7],
+    600 steps, mean over the last 100) against an FP64 run with the exact weights.  The FP64 run with
+    the reference's FP32 weight constants, and the reference's FP32 arithmetic, carry the weights'
+    mass source (~1.5e-5 of J here); the shifted FP32 storage does not and lands within 1e-6."""
+    nx, ny, nz = 24, 10, 10
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, 0, :] = f[:, -1, :] = O.WALL
+    f[0, :, :] = f[-1, :, :] = O.WALL
+    f[1:-1, 1:-1, 0] = O.INLET
+    f[1:-1, 1:-1, -1] = O.OUTLET
+    f = f.reshape(-1)
+    blend = np.where(f == 0, 0.01 * np.random.default_rng(4).uniform(0.3, 0.7, f.size), 0.0)
+    blend = blend.astype(np.float32).astype(np.float64)
+
+    def objective(storage):
+        o = O.Oracle(19, nx, ny, nz, F32(0.6), storage=storage)
+        o.set_flags(f)
+        o.set_inlet_outlet(F32(0.05), 1.0)
+        o.set_gamma(blend)
+        o.init_uniform()
+        o.step(500)
+        J = 0.0
+        for _ in range(100):
+            o.step(1)
+            J += o.objective(1, nx - 2)
+        return J / 100
+
+    with O.exact_fp64_weights():
+        exact = objective("fp64")
+    err = {s: abs(objective(s) - exact) / abs(exact) for s in ("fp64", "fp32", "fp32s", "fp16s")}
+    assert err["fp64"] > 5e-6 and err["fp32"] > 5e-6
+    assert err["fp32s"] < 1e-6
+    assert err["fp16s"] < 1e-3
+
+
 # ---------------------------------------------------------------- FP16 shifted storage
 def test_f16_codec_matches_ieee_rne():
     rng = np.random.default_rng(0)
