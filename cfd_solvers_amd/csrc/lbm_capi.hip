@@ -113,7 +113,7 @@ struct cfd_lbm_solver {
     float u_peak = 0.0F, rho_out = 1.0F;
     bool les = false;  // Smagorinsky LES (esoteric_pull.hpp:72-97, InPlaceLbmConfig::smagorinsky_les)
     uint64_t t = 0;
-    uint32_t step_lds = 0, adjoint_lds = 0;
+    uint32_t step_lds = 0, step_lds_periodic = 0, adjoint_lds = 0;
     std::vector<uint8_t> host_flags;  // copy of the flags (empty: periodic)
     float last_forward_ms = 0.0F, last_reverse_ms = 0.0F;
     uint64_t last_forward_steps = 0;
@@ -155,12 +155,6 @@ struct cfd_lbm_solver {
             const long w = std::atol(v);
             if (w == 64 || w == 128 || w == 256) bx = static_cast<uint32_t>(w);
         }
-        // The transposed step (adjoint) with FP16 storage runs 6 waves per SIMD by its registers; on
-        // HBM-bound lattices 5 measured faster (768^3 sphere cluster: 20.6 -> 19.6 ms), on 128^3,
-        // which the caches partly hold, slower (76 -> 79 us).  CFD_LBM_ADJOINT_WAVES (3..8) overrides.
-        long adjoint_waves = (storage == 1 && cells >= (size_t{1} << 24)) ? 5 : 8;
-        if (const char* v = std::getenv("CFD_LBM_ADJOINT_WAVES")) adjoint_waves = std::atol(v);
-        if (adjoint_waves >= 3 && adjoint_waves < 8) adjoint_lds = static_cast<uint32_t>(163840 / adjoint_waves - 1024);
         geo.nx = static_cast<uint32_t>(nx);
         geo.ny = static_cast<uint32_t>(ny);
         geo.nz = static_cast<uint32_t>(nz);
@@ -213,16 +207,40 @@ struct cfd_lbm_solver {
             if (2U * geo.stride * elem > (uint64_t{1} << 32))
                 throw std::invalid_argument("lattice too large: two population planes must fit in 4 GiB");
         }
-        // Resident wavefronts per SIMD of the step kernel, capped by reserving LDS per block (160 KiB
-        // per CU; a 256-thread block puts one wave on each of the 4 SIMDs).  Measured on MI355X:
-        // FP32 D3Q19 is fastest at 5 waves in row blocks (256^3: 4 % over 8) and at 4 in plain
-        // planes (768^3 sphere cluster: 14.3-14.7 ms against 15.2 at 5 and 15.4 at 8); FP32 D3Q27
-        // at 4; FP16 wants all 8 (768^3: 7.07-7.12 ms at 8 or 6, 7.29-7.38 at 5).  CFD_LBM_WAVES
-        // (3..8) overrides.
-        long waves = 8;
-        if (storage == 0 || storage == 3) waves = (lattice == 27 || !blocked) ? 4 : 5;
-        if (const char* v = std::getenv("CFD_LBM_WAVES")) waves = std::atol(v);
-        if (waves >= 3 && waves < 8) step_lds = static_cast<uint32_t>(163840 / waves - 1024);
+        // Resident wavefronts per SIMD of the step kernel, bounded and periodic, capped by reserving
+        // LDS per block (160 KiB per CU; a 256-thread block puts one wave on each of the 4 SIMDs).
+        // Measured on MI355X:
+        //  * row blocks: FP32 D3Q19 5 (256^3: 4 % over 8), FP32 D3Q27 4, FP16 uncapped;
+        //  * plain planes, bounded: FP32 3 and FP16 5 (768^3 cavity: 12.4 ms against 14.1 at 4,
+        //    FP16 6.66 against 7.24 uncapped; 768^3 sphere cluster: 12.6 against 13.4, FP16 6.78
+        //    against 7.24);
+        //  * plain planes, periodic: FP32 D3Q19 4 (512^3: 3.85 ms against 4.01 at 3), FP32 D3Q27 3
+        //    (5.15 against 5.77 at 4), FP16 5 (2.06 against 2.10 uncapped).
+        // CFD_LBM_WAVES (3..8) overrides both.
+        const bool fp32_arith = storage == 0 || storage == 3;
+        long waves_bounded = 8, waves_periodic = 8;
+        if (blocked) {
+            if (fp32_arith) waves_bounded = waves_periodic = lattice == 27 ? 4 : 5;
+        } else if (fp32_arith) {
+            waves_bounded = 3;
+            waves_periodic = lattice == 27 ? 3 : 4;
+        } else if (storage == 1) {
+            waves_bounded = waves_periodic = 5;
+        }
+        if (const char* v = std::getenv("CFD_LBM_WAVES")) waves_bounded = waves_periodic = std::atol(v);
+        auto lds_cap = [](long w) { return w >= 3 && w < 8 ? static_cast<uint32_t>(163840 / w - 1024) : 0U; };
+        step_lds = lds_cap(waves_bounded);
+        step_lds_periodic = lds_cap(waves_periodic);
+        // The transposed step (adjoint) runs 6 (FP16) or 5 (FP32) waves per SIMD by its registers.
+        // On HBM-bound lattices fewer measured faster: FP16 at 5 (768^3 sphere cluster: 20.6 ->
+        // 19.6 ms; on 128^3, which the caches partly hold, slower: 76 -> 79 us), FP32 in plain planes
+        // at 3 (768^3 sphere cluster, reverse phase of 200 steps: 14.00 / 13.25 s against 14.07 /
+        // 13.45 s in two runs).  CFD_LBM_ADJOINT_WAVES (3..8) overrides.
+        long adjoint_waves = 8;
+        if (storage == 1 && cells >= (size_t{1} << 24)) adjoint_waves = 5;
+        else if (fp32_arith && !blocked) adjoint_waves = 3;
+        if (const char* v = std::getenv("CFD_LBM_ADJOINT_WAVES")) adjoint_waves = std::atol(v);
+        adjoint_lds = lds_cap(adjoint_waves);
 
         ops = cfd_hip::kernel_ops(q, storage);
         int count = 0;
@@ -285,7 +303,8 @@ struct cfd_lbm_solver {
     LaunchState state() {
         if (storage == 2) build_physics<double>();
         else build_physics<float>();
-        return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells), step_lds, adjoint_lds};
+        return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells), step_lds, step_lds_periodic,
+                           adjoint_lds};
     }
 
     void sync() const {
@@ -490,6 +509,7 @@ struct cfd_lbm_solver {
         }
         if (!segments) segments.allocate(static_cast<size_t>(seg_x()) * ny * nz * sizeof(uint32_t));
         ops->segments(st, fresh.ptr, seg_x(), reinterpret_cast<uint32_t*>(segments.ptr));
+        if (gamma) ops->porous_segments(st, gamma.ptr, seg_x(), reinterpret_cast<uint32_t*>(segments.ptr));
         if (!links) links.allocate(cells * sizeof(uint32_t));
         if (!link_vel) link_vel.allocate(cells * vel_words() * sizeof(uint32_t));
         ops->links(st, fresh.ptr, reinterpret_cast<uint32_t*>(links.ptr), reinterpret_cast<uint32_t*>(link_vel.ptr));
@@ -1041,17 +1061,27 @@ struct cfd_lbm_solver {
         les = enabled;
     }
 
+    // The segment words of a bounded lattice: spans from the flags, then the porous marks of gamma.
+    void mark_segments() {
+        if (!segments) return;
+        const LaunchState st = state();
+        ops->segments(st, flags.ptr, seg_x(), reinterpret_cast<uint32_t*>(segments.ptr));
+        if (gamma) ops->porous_segments(st, gamma.ptr, seg_x(), reinterpret_cast<uint32_t*>(segments.ptr));
+    }
+
     void set_porosity(const float* g) {
         use_device();
         if (!g) {
             sync();
             gamma.reset();
+            mark_segments();
             return;
         }
         for (size_t i = 0; i < cells; ++i)
             if (!(g[i] >= 0.0F && g[i] <= 1.0F)) throw std::invalid_argument("porosity gamma must lie in [0, 1]");
         const float* parts[1] = {g};
         upload_real(gamma, parts, 1);
+        mark_segments();
     }
 
     // set_porosity from device memory (cells floats on this solver's device): validated on the
@@ -1073,6 +1103,7 @@ struct cfd_lbm_solver {
         if (n_bad != 0) throw std::invalid_argument("porosity gamma must lie in [0, 1]");
         std::swap(gamma.ptr, fresh.ptr);
         std::swap(gamma.bytes, fresh.bytes);
+        mark_segments();
     }
 
     // z-slab halo messages (the slab is this lattice, periodic in z; its neighbours are the slabs

@@ -339,6 +339,9 @@ __device__ __forceinline__ void moments(const Real (&f)[Q], Real& rho, Real& mx,
     });
 }
 
+constexpr uint32_t kSpanBits = 0xFFFFU;        // lo | hi << 8 of a segment word
+constexpr uint32_t kPorousSegment = 1U << 16;  // the segment holds a cell with gamma != 0
+
 template <class Real> struct Physics {
     Real omega;
     Real tau;
@@ -353,8 +356,9 @@ template <class Real> struct Physics {
     const uint32_t* links;      // per-cell link descriptor (bounded lattices)
     const uint32_t* link_vel;   // per cell, (q-1+9)/10 words of 3-bit wall-velocity indices per link
     // Per 64-cell row segment of a bounded lattice: the span lo | hi << 8 of the cells [lo, hi) that
-    // are not plain fluid with a plain-fluid 3x3x3 neighbourhood, 0 if none.  A 32-bit word, read
-    // through the scalar cache by the wavefront that updates the segment.
+    // are not plain fluid with a plain-fluid 3x3x3 neighbourhood, 0 if none, and kPorousSegment when
+    // a cell of the segment has a nonzero partial bounce-back fraction.  A 32-bit word, read through
+    // the scalar cache by the wavefront that updates the segment.
     const uint32_t* segments;
     uint32_t seg_x;             // segments per row, ceil(nx / 64)
     uint32_t cells;
@@ -549,9 +553,11 @@ __device__ __forceinline__ void relax(Real (&f)[Q], Real rp, Real rho, Real ux, 
 // Forced selects the Guo-forcing variant at compile time: the unforced variant carries no force
 // temporaries and fits twice the waves per SIMD.  With Forced, a zero acceleration still takes the
 // unforced arithmetic (the reference's guo_force returns 0 then), so both variants agree bitwise.
+// gamma: the partial bounce-back fractions, or null where the cell's segment holds none (the
+// blend with gamma = 0 returns the BGK result unchanged, up to the sign of an exact zero).
 template <int Q, bool Forced, class S, class Real>
-__device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
-                                        uint32_t y, uint32_t z, Real (&f)[Q]) {
+__device__ __forceinline__ Real collide(const Physics<Real>& p, const Real* gamma, const Geometry& g, uint8_t flag,
+                                        uint32_t n, uint32_t y, uint32_t z, Real (&f)[Q]) {
     if (flag == kInlet || flag == kOutlet) {
         Real rp, rho, ux;
         if (flag == kInlet) {
@@ -594,7 +600,7 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
     if constexpr (Forced && !S::kShifted)
         if (p.les) omega = Real(1) / smagorinsky_tau<Q>(f, rho, ux, uy, uz, uu, p.tau);
     Real fin_copy[Q];
-    if (p.gamma) {
+    if (gamma) {
         static_for<Q>([&](auto DD) { fin_copy[decltype(DD)::value] = f[decltype(DD)::value]; });
     }
     if (!Forced || (ax == Real(0) && ay == Real(0) && az == Real(0))) {
@@ -611,11 +617,11 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Geometry& 
             f[d] = f[d] + src;
         });
     }
-    if (p.gamma) {
+    if (gamma) {
         // Partial bounce-back (Walsh, Burwinkle & Saar 2009): blend the BGK result with the
         // pre-collision population of the opposite direction (w_d = w_opp(d): the blend is the
         // same on deviations).
-        const Real gm = p.gamma[n];
+        const Real gm = gamma[n];
         const Real keep = Real(1) - gm;
         Real mixed[Q];
         static_for<Q>([&](auto DD) {

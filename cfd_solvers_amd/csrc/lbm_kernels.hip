@@ -21,14 +21,15 @@ namespace {
 // ------------------------------------------------------------------ hot kernel
 template <int Q, class T, bool Odd, bool Bounded, bool Forced>
 __device__ __forceinline__ void update_cell(const void* src, void* dst, const Geometry& g,
-                                            const Physics<typename Storage<T>::Real>& p, const Neighbourhood& h,
+                                            const Physics<typename Storage<T>::Real>& p,
+                                            const typename Storage<T>::Real* gamma, const Neighbourhood& h,
                                             uint8_t flag, uint32_t desc, uint32_t n, uint32_t x, uint32_t y,
                                             uint32_t z) {
     using Real = typename Storage<T>::Real;
     Real fin[Q];
     PairSlots<Q> s;
     load_cell<Q, T, Odd, Bounded>(src, g, desc, h, fin, s);
-    const Real rho = collide<Q, Forced, Storage<T>>(p, g, flag, n, y, z, fin);
+    const Real rho = collide<Q, Forced, Storage<T>>(p, gamma, g, flag, n, y, z, fin);
     store_cell<Q, T, Odd, Bounded>(dst, g, p, n, x, y, z, rho, fin, s);
 }
 
@@ -52,21 +53,24 @@ __global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const vo
         // wavefront-uniform: one scalar load, and a branch on it costs no lane masking.  A segment
         // whose cells are all plain fluid (span 0) takes the periodic path: no link loads, no
         // bounce-back selects.  Otherwise only the cells inside the span [lo, hi) can have a nonzero
-        // link descriptor; the others skip its load.
+        // link descriptor; the others skip its load.  Segments without a porous cell skip the
+        // partial bounce-back (its loads of gamma and its blend) as a whole.
         const uint32_t k = __builtin_amdgcn_readfirstlane((z * g.ny + y) * p.seg_x + (x >> 6));
         using ConstantWords = const __attribute__((address_space(4))) uint32_t*;  // scalar loads
-        const uint32_t span = ((ConstantWords)p.segments)[k];
+        const uint32_t word = ((ConstantWords)p.segments)[k];
+        const auto* gamma = (word & kPorousSegment) != 0U ? p.gamma : nullptr;
+        const uint32_t span = word & kSpanBits;
         if (span == 0U) {
-            update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, 0U, n, x, y, z);
+            update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, gamma, h, kFluid, 0U, n, x, y, z);
             return;
         }
         const uint32_t xi = x & 63U;
         const uint32_t desc = (xi >= (span & 0xFFU) && xi < (span >> 8)) ? p.links[n] : 0U;
         const uint32_t type = (desc >> kTypeShift) & 3U;
         if (type == kTypeWall) return;
-        update_cell<Q, T, Odd, true, Forced>(src, dst, g, p, h, static_cast<uint8_t>(type), desc, n, x, y, z);
+        update_cell<Q, T, Odd, true, Forced>(src, dst, g, p, gamma, h, static_cast<uint8_t>(type), desc, n, x, y, z);
     } else {
-        update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, h, kFluid, 0U, n, x, y, z);
+        update_cell<Q, T, Odd, false, Forced>(src, dst, g, p, p.gamma, h, kFluid, 0U, n, x, y, z);
     }
 }
 
@@ -108,6 +112,19 @@ __global__ __launch_bounds__(256) void mark_segments(const uint8_t* __restrict__
         mid = right;
     }
     segments[k] = hi == 0U ? 0U : lo | (hi << 8);
+}
+
+// Sets kPorousSegment in the word of every row segment that holds a cell with gamma != 0 (after
+// mark_segments has written the spans).  Launched on the step's grid: each wavefront covers one
+// segment (bx >= 64), and its first lane, x = 64 * segment, owns the word.
+template <class Real>
+__global__ __launch_bounds__(256) void mark_porous_segments(const Real* __restrict__ gamma, const Geometry g,
+                                                            uint32_t seg_x, uint32_t* __restrict__ segments) {
+    uint32_t x, y, z;
+    const bool inside = cell_coords(g, x, y, z);
+    const bool porous = inside && gamma[(z * g.ny + y) * g.nx + x] != Real(0);
+    const uint64_t any = __ballot(porous);
+    if (inside && (x & 63U) == 0U && any != 0U) segments[(z * g.ny + y) * seg_x + (x >> 6)] |= kPorousSegment;
 }
 
 // Link descriptor of every cell from the flags (layout in lbm_device.hpp).
@@ -611,7 +628,8 @@ template <int Q, class T> struct Ops final : KernelOps {
     template <bool Odd, bool Bounded, bool Forced>
     static void step_t(const LaunchState& st, const void* src, void* dst) {
         hipLaunchKernelGGL((stream_collide<Q, T, Odd, Bounded, Forced>), grid_of(st.geo), block_of(st.geo),
-                           st.step_lds, st.stream, src, dst, st.geo, *static_cast<const Physics<Real>*>(st.physics));
+                           Bounded ? st.step_lds : st.step_lds_periodic, st.stream, src, dst, st.geo,
+                           *static_cast<const Physics<Real>*>(st.physics));
     }
     void step(const LaunchState& st, bool odd, bool bounded, bool forced, const void* src, void* dst) const override {
         per_parity(st, odd, bounded, [&](auto O, auto B) {
@@ -657,6 +675,11 @@ template <int Q, class T> struct Ops final : KernelOps {
         hipLaunchKernelGGL(mark_segments, dim3((total + 255U) / 256U), dim3(256), 0, st.stream, flags, st.geo, seg_x,
                            out);
         check(hipGetLastError(), "mark_segments launch");
+    }
+    void porous_segments(const LaunchState& st, const void* gamma, uint32_t seg_x, uint32_t* segments) const override {
+        hipLaunchKernelGGL(mark_porous_segments<Real>, grid_of(st.geo), block_of(st.geo), 0, st.stream,
+                           static_cast<const Real*>(gamma), st.geo, seg_x, segments);
+        check(hipGetLastError(), "mark_porous_segments launch");
     }
     void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
               const void* solid_rho) const override {

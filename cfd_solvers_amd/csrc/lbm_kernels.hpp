@@ -16,7 +16,8 @@ struct LaunchState {
     const void* physics;     // Physics<float> or Physics<double> (host copy, passed by value)
     hipStream_t stream;
     uint32_t cells;
-    uint32_t step_lds;       // dynamic LDS bytes per step-kernel block (occupancy cap; 0 = none)
+    uint32_t step_lds;       // dynamic LDS bytes per bounded step-kernel block (occupancy cap; 0 = none)
+    uint32_t step_lds_periodic;  // the same for the periodic step
     uint32_t adjoint_lds;    // the same for the transposed step
 };
 
@@ -32,6 +33,9 @@ struct KernelOps {
     virtual void freeze(const LaunchState& st, bool odd, const uint8_t* flags, const uint8_t* old_flags,
                         void* solid_rho) const = 0;
     virtual void segments(const LaunchState& st, const uint8_t* flags, uint32_t seg_x, uint32_t* out) const = 0;
+    // Marks the segments of `segments` (spans already written) that hold a cell with gamma != 0.
+    virtual void porous_segments(const LaunchState& st, const void* gamma, uint32_t seg_x,
+                                 uint32_t* segments) const = 0;
     virtual void thaw(const LaunchState& st, bool odd, const uint8_t* old_flags, const uint8_t* flags,
                       const void* solid_rho) const = 0;
     virtual void macroscopic(const LaunchState& st, bool odd, bool bounded, void* out) const = 0;
