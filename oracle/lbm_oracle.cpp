@@ -27,7 +27,8 @@
 //   * seeded noise on the Taylor-Green initial velocity (SplitMix64 counter hash),
 //   * 3-D velocity inlet / pressure outlet with Zou-He-style densities and equilibrium populations,
 //   * partial bounce-back porosity gamma (Walsh, Burwinkle & Saar 2009 form),
-//   * FP16 shifted-population storage h = half_rne((f - w_d) * 4096).
+//   * FP16 shifted-population storage h = half_rne((f - w_d) * 4096),
+//   * a switch (off by default) for FP64 arithmetic with the weights' exact values.
 //
 // Arithmetic follows the reference operation by operation (same association order) and the
 // library is built with -ffp-contract=off so that it is reproducible bit for bit.
