@@ -99,7 +99,9 @@ CFD_LBM_API int cfd_lbm_get_flags(cfd_lbm_solver* s, uint8_t* flags, char* error
 /* Wall velocity for flag CFD_LBM_WALL|index (index 1..7); src/lbm/d2q9.cpp:109-115,217-223. */
 CFD_LBM_API int cfd_lbm_set_wall_velocity(cfd_lbm_solver* s, int index, float ux, float uy, float uz, char* error,
                                           size_t error_capacity);
-/* Partial bounce-back porosity gamma in [0,1] per cell (cells values; NULL clears). */
+/* Partial bounce-back porosity gamma in [0,1] per cell (cells values; NULL clears).  On a bounded
+ * lattice the steps blend only in the 64-cell row segments that hold a nonzero gamma; the call
+ * marks them (one pass over gamma), and so does every later change of the flags. */
 CFD_LBM_API int cfd_lbm_set_porosity(cfd_lbm_solver* s, const float* gamma, char* error, size_t error_capacity);
 /* set_porosity from DEVICE memory (cells floats on the solver's device), checked on the device: an
  * error, and the field in use unchanged, unless every value lies in [0, 1].  No reference
