@@ -24,7 +24,7 @@ PKG = Path(__file__).resolve().parent
 DEFAULT_LIBRARY = PKG / "libcfd_lbm_hip.so"
 
 LATTICES = {"d2q9": 9, "d3q19": 19, "d3q27": 27}
-STORAGE = {"fp32": 0, "fp16s": 1, "fp64": 2, "fp32s": 3}
+STORAGE = {"fp32": 0, "fp16s": 1, "fp64": 2, "fp32s": 3, "fp64x": 4}
 FLUID, VELOCITY_INLET, PRESSURE_OUTLET, WALL = 0x00, 0x01, 0x02, 0x80
 
 _ERR = 1024

@@ -33,10 +33,10 @@ inline void check(hipError_t e, const char* what) {
 }
 
 // d feq_d / d rho = e and d feq_d / d u = gu for the second-order equilibrium.
-template <int Q, int D, class Real>
+template <class S, int Q, int D, class Real>
 __device__ __forceinline__ void feq_grad(Real rho, Real ux, Real uy, Real uz, Real uu_plain, Real& e, Real (&gu)[3]) {
     using L = Lattice<Q>;
-    const Real w = static_cast<Real>(L::w[D]);
+    const Real w = lattice_weight<S, Q, D, Real>();
     const Real cu = cdot<Q, D>(ux, uy, uz);
     e = w * (Real(1) + Real(3) * cu + Real(4.5) * cu * cu - Real(1.5) * uu_plain);
     const Real wr = w * rho;
@@ -62,7 +62,7 @@ template <int Q, int D, class Real> __device__ __forceinline__ void add_c(Real (
 }
 
 // mu = (d fout / d fin)^T lam for one cell; also d/dgamma and d/du_peak of lam . fout.
-template <int Q, class Real>
+template <int Q, class S, class Real>
 __device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geometry& g, uint8_t flag, uint32_t n,
                                               uint32_t y, uint32_t z, const Real (&f)[Q], const Real (&lam)[Q],
                                               Real (&mu)[Q], double& dgamma, double& dpeak) {
@@ -83,7 +83,7 @@ __device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geom
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
             Real e, gu[3];
-            feq_grad<Q, d>(rho, uin, Real(0), Real(0), uin * uin, e, gu);
+            feq_grad<S, Q, d>(rho, uin, Real(0), Real(0), uin * uin, e, gu);
             g_rho += lam[d] * e;
             g_ux += lam[d] * gu[0];
         });
@@ -102,7 +102,7 @@ __device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geom
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
             Real e, gu[3];
-            feq_grad<Q, d>(rho, ux, Real(0), Real(0), ux * ux, e, gu);
+            feq_grad<S, Q, d>(rho, ux, Real(0), Real(0), ux * ux, e, gu);
             g_ux += lam[d] * gu[0];
         });
         const Real s = g_ux / rho;
@@ -135,7 +135,7 @@ __device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geom
     const Real ok = omega * keep;                  // b_d = ok w_d lam_d
     const Real om1k = (Real(1) - omega) * keep;    // (1 - omega) a_d = om1k lam_d
     const Real quad = Real(1) - Real(1.5) * uu;
-    const Real w0 = static_cast<Real>(L::w[0]);
+    const Real w0 = lattice_weight<S, Q, 0, Real>();
     Real S0 = ok * w0 * lam[0];
     Real S1[3] = {Real(0), Real(0), Real(0)}, S2[3] = {Real(0), Real(0), Real(0)};
     // dJ/dgamma = sum_d lam_d (f_opp(d) - fstar_d), fstar_d = f_d - omega (f_d - feq_d), for porous
@@ -147,7 +147,7 @@ __device__ __forceinline__ void collision_vjp(const Physics<Real>& p, const Geom
     }
     static_for<(Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * decltype(P)::value + 1;
-        const Real w = static_cast<Real>(L::w[i]);
+        const Real w = lattice_weight<S, Q, i, Real>();
         const Real cu = cdot<Q, i>(ux, uy, uz);
         const Real kw = ok * w;
         const Real bs = kw * (lam[i] + lam[i + 1]);
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const void* __restrict__ fst
     });
     Real mu[Q];
     double dgamma, dpeak;
-    collision_vjp<Q>(p, g, flag, n, y, z, f, lam, mu, dgamma, dpeak);
+    collision_vjp<Q, Storage<T>>(p, g, flag, n, y, z, f, lam, mu, dgamma, dpeak);
     if (a.add_objective && a.kind == kPressureDrop) {
         const Real c = plane_weight<Real>(a, x, flag == kFluid);
         if (c != Real(0)) static_for<Q>([&](auto DD) { mu[decltype(DD)::value] += c; });
@@ -523,12 +523,14 @@ template <int Q> const AdjointOps* adj_for_storage(int storage) {
     static const AdjOps<Q, _Float16> f16;
     static const AdjOps<Q, double> f64;
     static const AdjOps<Q, F32Dev> f32s;
+    static const AdjOps<Q, F64Exact> f64x;
     switch (storage) {
     case 0: return &f32;
     case 1: return &f16;
     case 2: return &f64;
     case 3: return &f32s;
-    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2) or FP32S (3)");
+    case 4: return &f64x;
+    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2), FP32S (3) or FP64X (4)");
     }
 }
 

@@ -125,8 +125,8 @@ struct cfd_lbm_solver {
                    int storage_) {
         if (lattice != 9 && lattice != 19 && lattice != 27)
             throw std::invalid_argument("lattice must be D2Q9 (9), D3Q19 (19) or D3Q27 (27)");
-        if (storage_ < 0 || storage_ > 3)
-            throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2) or FP32S (3)");
+        if (storage_ < 0 || storage_ > 4)
+            throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2), FP32S (3) or FP64X (4)");
         if (nx_ < 2 || ny_ < 2) throw std::invalid_argument("LBM grid requires nx, ny >= 2");
         if (lattice == 9) nz_ = 1;
         else if (nz_ < 2) throw std::invalid_argument("3-D LBM grid requires nz >= 2");
@@ -143,8 +143,8 @@ struct cfd_lbm_solver {
         ax = ax_;
         ay = ay_;
         az = az_;
-        elem = storage == 1 ? 2 : (storage == 2 ? 8 : 4);
-        real = storage == 2 ? 8 : 4;
+        elem = storage == 1 ? 2 : (storage == 2 || storage == 4 ? 8 : 4);
+        real = storage == 2 || storage == 4 ? 8 : 4;
 
         // Launch geometry: 256-thread tiles, bx a power of two in [64, 256] covering x, so that a
         // wavefront's 64 lanes lie in one row segment (the bounded step reads the segment's span
@@ -301,7 +301,7 @@ struct cfd_lbm_solver {
     }
 
     LaunchState state() {
-        if (storage == 2) build_physics<double>();
+        if (real == 8) build_physics<double>();
         else build_physics<float>();
         return LaunchState{f.ptr, geo, physics, stream, static_cast<uint32_t>(cells), step_lds, step_lds_periodic,
                            adjoint_lds};

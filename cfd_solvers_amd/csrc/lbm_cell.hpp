@@ -35,6 +35,7 @@ template <class T> __device__ __forceinline__ void store_at(void* base, uint32_t
     // Streaming stores: no slot written in a step is read again within that step, and the whole
     // lattice passes through the caches every step (measured ~1 % faster than plain stores).
     if constexpr (__is_same(T, F32Dev)) __builtin_nontemporal_store(v.v, reinterpret_cast<float*>(p));
+    else if constexpr (__is_same(T, F64Exact)) __builtin_nontemporal_store(v.v, reinterpret_cast<double*>(p));
     else __builtin_nontemporal_store(v, p);
 }
 
@@ -116,14 +117,14 @@ __device__ __forceinline__ void store_cell(void* f, const Geometry& g, const Phy
                     const Real cu = static_cast<Real>(L::cx[i]) * p.wall_u[ka][0] +
                                     static_cast<Real>(L::cy[i]) * p.wall_u[ka][1] +
                                     static_cast<Real>(L::cz[i]) * p.wall_u[ka][2];
-                    fout[i] = fout[i] - wall_term<S>(Real(6) * static_cast<Real>(L::w[i]) * rho_wall * cu);
+                    fout[i] = fout[i] - wall_term<S>(Real(6) * lattice_weight<S, Q, i, Real>() * rho_wall * cu);
                 }
                 const uint32_t kb = (vel[(i - 1) / 10] >> (3 * ((i - 1) % 10))) & 7U;
                 if (kb != 0U) {
                     const Real cu = static_cast<Real>(L::cx[i + 1]) * p.wall_u[kb][0] +
                                     static_cast<Real>(L::cy[i + 1]) * p.wall_u[kb][1] +
                                     static_cast<Real>(L::cz[i + 1]) * p.wall_u[kb][2];
-                    fout[i + 1] = fout[i + 1] - wall_term<S>(Real(6) * static_cast<Real>(L::w[i + 1]) * rho_wall * cu);
+                    fout[i + 1] = fout[i + 1] - wall_term<S>(Real(6) * lattice_weight<S, Q, i + 1, Real>() * rho_wall * cu);
                 }
             });
         }

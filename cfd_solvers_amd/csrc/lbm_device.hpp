@@ -32,6 +32,8 @@ template <> struct Lattice<9> {
     static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     static constexpr float w[9] = {4.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F,  1.0F / 9.0F, 1.0F / 9.0F,
                                    1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+    static constexpr double w_exact[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
+                                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 };
 template <> struct Lattice<19> {
     static constexpr int dims = 3;
@@ -42,6 +44,10 @@ template <> struct Lattice<19> {
                                     1.0F / 18.0F, 1.0F / 18.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
                                     1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F,
                                     1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F, 1.0F / 36.0F};
+    static constexpr double w_exact[19] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                                           1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                           1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                           1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 };
 template <> struct Lattice<27> {
     static constexpr int dims = 3;
@@ -56,6 +62,11 @@ template <> struct Lattice<27> {
         1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,
         1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 54.0F,  1.0F / 216.0F, 1.0F / 216.0F,
         1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F, 1.0F / 216.0F};
+    static constexpr double w_exact[27] = {
+        8.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,
+        1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,
+        1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 216.0, 1.0 / 216.0,
+        1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0};
 };
 
 constexpr int opposite(int i) { return i == 0 ? 0 : ((i & 1) != 0 ? i + 1 : i - 1); }
@@ -92,6 +103,7 @@ template <class T> struct Storage;
 template <> struct Storage<float> {
     using Real = float;
     static constexpr bool kShifted = false;
+    static constexpr bool kExactWeights = false;
     static constexpr float kScale = 1.0F, kInvScale = 1.0F;
     template <int Q, int D> __device__ __forceinline__ static float decode(float v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static float encode(float v) { return v; }
@@ -102,6 +114,7 @@ template <> struct Storage<float> {
 template <> struct Storage<_Float16> {
     using Real = float;
     static constexpr bool kShifted = true;
+    static constexpr bool kExactWeights = false;
     static constexpr float kScale = kHalfScale, kInvScale = kHalfInvScale;
     template <int Q, int D> __device__ __forceinline__ static float decode(_Float16 h) { return static_cast<float>(h); }
     // The barrier keeps the compiler from fusing the producing multiply or fma into the conversion
@@ -120,6 +133,7 @@ struct F32Dev {
 template <> struct Storage<F32Dev> {
     using Real = float;
     static constexpr bool kShifted = true;
+    static constexpr bool kExactWeights = false;
     static constexpr float kScale = 1.0F, kInvScale = 1.0F;
     template <int Q, int D> __device__ __forceinline__ static float decode(F32Dev h) { return h.v; }
     template <int Q, int D> __device__ __forceinline__ static F32Dev encode(float v) { return F32Dev{v}; }
@@ -128,6 +142,7 @@ template <> struct Storage<F32Dev> {
 template <> struct Storage<uint32_t> {
     using Real = uint32_t;
     static constexpr bool kShifted = false;
+    static constexpr bool kExactWeights = false;
     static constexpr float kScale = 1.0F, kInvScale = 1.0F;
     template <int Q, int D> __device__ __forceinline__ static uint32_t decode(uint32_t v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static uint32_t encode(uint32_t v) { return v; }
@@ -135,10 +150,32 @@ template <> struct Storage<uint32_t> {
 template <> struct Storage<double> {
     using Real = double;
     static constexpr bool kShifted = false;
+    static constexpr bool kExactWeights = false;
     static constexpr double kScale = 1.0, kInvScale = 1.0;
     template <int Q, int D> __device__ __forceinline__ static double decode(double v) { return v; }
     template <int Q, int D> __device__ __forceinline__ static double encode(double v) { return v; }
 };
+// fp64x: the FP64 build with the lattice weights' exact values (4/9, 1/3, 1/18, ...) instead of the
+// reference's FP32 constants, which sum to 1 + 1.49e-8 (D3Q19) or 1 + 7.45e-9 (D2Q9, D3Q27) and so
+// add omega times that excess to a cell's density at every collision.  A reference for precision
+// studies; fp64 keeps the reference's constants.
+struct F64Exact {
+    double v;
+};
+template <> struct Storage<F64Exact> {
+    using Real = double;
+    static constexpr bool kShifted = false;
+    static constexpr bool kExactWeights = true;
+    static constexpr double kScale = 1.0, kInvScale = 1.0;
+    template <int Q, int D> __device__ __forceinline__ static double decode(F64Exact v) { return v.v; }
+    template <int Q, int D> __device__ __forceinline__ static F64Exact encode(double v) { return F64Exact{v}; }
+};
+
+// Lattice weight w_D in the arithmetic of storage S.
+template <class S, int Q, int D, class Real> __host__ __device__ constexpr Real lattice_weight() {
+    if constexpr (S::kExactWeights) return static_cast<Real>(Lattice<Q>::w_exact[D]);
+    else return static_cast<Real>(Lattice<Q>::w[D]);
+}
 
 // Population d of the state representation -> the population itself (deviation / scale + w_d
 // when shifted).
@@ -278,10 +315,10 @@ template <int Q, int D, class Real> __device__ __forceinline__ Real cdot(Real ux
     return s;
 }
 
-template <int Q, int D, class Real>
+template <class S, int Q, int D, class Real>
 __device__ __forceinline__ Real equilibrium(Real rho, Real ux, Real uy, Real uz, Real uu) {
     const Real cu = Real(3) * cdot<Q, D>(ux, uy, uz);
-    return static_cast<Real>(Lattice<Q>::w[D]) * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
+    return lattice_weight<S, Q, D, Real>() * rho * (Real(1) + cu + Real(0.5) * cu * cu - uu);
 }
 
 // Shifted arithmetic (fp16s, fp32s; no reference counterpart).  With cu = 3 c.u and
@@ -306,7 +343,7 @@ template <class Real> __device__ __forceinline__ Real shifted_uu(Real ux, Real u
 template <class S, int Q, int D, class Real>
 __device__ __forceinline__ Real equilibrium_state(Real rp, Real rho, Real ux, Real uy, Real uz, Real uu) {
     if constexpr (S::kShifted) return equilibrium_dev<S, Q, D>(rp, rho, ux, uy, uz, uu);
-    else return equilibrium<Q, D>(rho, ux, uy, uz, uu);
+    else return equilibrium<S, Q, D>(rho, ux, uy, uz, uu);
 }
 // 3/2 |u|^2 in the arithmetic of storage S.
 template <class S, class Real> __device__ __forceinline__ Real kinetic_term(Real ux, Real uy, Real uz) {
@@ -314,11 +351,11 @@ template <class S, class Real> __device__ __forceinline__ Real kinetic_term(Real
     else return Real(1.5) * (ux * ux + uy * uy + uz * uz);
 }
 
-template <int Q, int D, class Real>
+template <class S, int Q, int D, class Real>
 __device__ __forceinline__ Real guo(Real ux, Real uy, Real uz, Real fx, Real fy, Real fz, Real uf, Real wfac) {
     const Real cu = cdot<Q, D>(ux, uy, uz);
     const Real cf = cdot<Q, D>(fx, fy, fz);
-    return static_cast<Real>(Lattice<Q>::w[D]) * wfac * (Real(3) * (cf - uf) + Real(9) * cu * cf);
+    return lattice_weight<S, Q, D, Real>() * wfac * (Real(3) * (cf - uf) + Real(9) * cu * cf);
 }
 
 // Density and momentum in direction order 0..q-1 (esoteric_pull.hpp:384-406).
@@ -485,7 +522,7 @@ __device__ __forceinline__ void cell_state(const Physics<Real>& p, const Geometr
 // Q = sum_ab Pi_ab^2 in row-major order, tau_local = (tau + sqrt(tau^2 + C sqrt(Q) / rho)) / 2 with
 // C = 16 sqrt(2) / (3 pi^2) evaluated in float.  The reference's additions of zero terms are
 // skipped (exact: they add +-0).  Unshifted storage only.
-template <int Q, class Real>
+template <int Q, class S, class Real>
 __device__ __forceinline__ Real smagorinsky_tau(const Real (&f)[Q], Real rho, Real ux, Real uy, Real uz, Real uu,
                                                 Real tau) {
     using L = Lattice<Q>;
@@ -494,7 +531,7 @@ __device__ __forceinline__ Real smagorinsky_tau(const Real (&f)[Q], Real rho, Re
     static_for<Q>([&](auto DD) {
         constexpr int d = decltype(DD)::value;
         constexpr int cx = L::cx[d], cy = L::cy[d], cz = L::cz[d];
-        const Real neq = f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu);
+        const Real neq = f[d] - equilibrium<S, Q, d>(rho, ux, uy, uz, uu);
         if constexpr (cx != 0) pxx = pxx + neq;
         if constexpr (cy != 0) pyy = pyy + neq;
         if constexpr (cz != 0) pzz = pzz + neq;
@@ -542,7 +579,7 @@ __device__ __forceinline__ void relax(Real (&f)[Q], Real rp, Real rho, Real ux, 
     } else {
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            f[d] = f[d] - omega * (f[d] - equilibrium<Q, d>(rho, ux, uy, uz, uu));
+            f[d] = f[d] - omega * (f[d] - equilibrium<S, Q, d>(rho, ux, uy, uz, uu));
         });
     }
 }
@@ -598,7 +635,7 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Real* gamm
     const Real uu = kinetic_term<S>(ux, uy, uz);
     Real omega = p.omega;
     if constexpr (Forced && !S::kShifted)
-        if (p.les) omega = Real(1) / smagorinsky_tau<Q>(f, rho, ux, uy, uz, uu, p.tau);
+        if (p.les) omega = Real(1) / smagorinsky_tau<Q, S>(f, rho, ux, uy, uz, uu, p.tau);
     Real fin_copy[Q];
     if (gamma) {
         static_for<Q>([&](auto DD) { fin_copy[decltype(DD)::value] = f[decltype(DD)::value]; });
@@ -612,7 +649,7 @@ __device__ __forceinline__ Real collide(const Physics<Real>& p, const Real* gamm
         relax<Q, S>(f, rp, rho, ux, uy, uz, uu, omega);
         static_for<Q>([&](auto DD) {
             constexpr int d = decltype(DD)::value;
-            Real src = guo<Q, d>(ux, uy, uz, fx, fy, fz, uf, wfac);
+            Real src = guo<S, Q, d>(ux, uy, uz, fx, fy, fz, uf, wfac);
             if constexpr (S::kShifted) src = src * static_cast<Real>(S::kScale);
             f[d] = f[d] + src;
         });

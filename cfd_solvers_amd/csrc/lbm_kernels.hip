@@ -384,7 +384,7 @@ __global__ __launch_bounds__(256) void download_logical(const void* __restrict__
             const Real rho = p.solid_rho[n];
             static_for<Q>([&](auto DD) {
                 constexpr int d = decltype(DD)::value;
-                out[static_cast<uint64_t>(d) * p.cells + n] = static_cast<Real>(Lattice<Q>::w[d]) * rho;
+                out[static_cast<uint64_t>(d) * p.cells + n] = lattice_weight<Storage<T>, Q, d, Real>() * rho;
             });
             return;
         }
@@ -782,12 +782,14 @@ template <int Q> const KernelOps* ops_for_storage(int storage) {
     static const Ops<Q, _Float16> f16;
     static const Ops<Q, double> f64;
     static const Ops<Q, F32Dev> f32s;
+    static const Ops<Q, F64Exact> f64x;
     switch (storage) {
     case 0: return &f32;
     case 1: return &f16;
     case 2: return &f64;
     case 3: return &f32s;
-    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2) or FP32S (3)");
+    case 4: return &f64x;
+    default: throw std::invalid_argument("storage must be FP32 (0), FP16S (1), FP64 (2), FP32S (3) or FP64X (4)");
     }
 }
 
@@ -802,7 +804,9 @@ const KernelOps* kernel_ops(int q, int storage) {
     }
 }
 
-size_t init_args_size(int storage) { return storage == 2 ? sizeof(InitArgs<double>) : sizeof(InitArgs<float>); }
+size_t init_args_size(int storage) {
+    return storage == 2 || storage == 4 ? sizeof(InitArgs<double>) : sizeof(InitArgs<float>);
+}
 
 void fill_init_args(int storage, void* out, int mode, double rho, double ux, double uy, double uz, double amplitude,
                     double noise, uint64_t seed, const void* tables, const void* fields) {
@@ -819,7 +823,7 @@ void fill_init_args(int storage, void* out, int mode, double rho, double ux, dou
         a->tables = static_cast<const Real*>(tables);
         a->fields = static_cast<const Real*>(fields);
     };
-    if (storage == 2) fill(static_cast<InitArgs<double>*>(out));
+    if (storage == 2 || storage == 4) fill(static_cast<InitArgs<double>*>(out));
     else fill(static_cast<InitArgs<float>*>(out));
 }
 

@@ -36,15 +36,18 @@ typedef struct cfd_lbm_solver cfd_lbm_solver;
 /* Lattice ids are the velocity-set sizes q (descriptors.hpp:31-146). */
 enum { CFD_LBM_D2Q9 = 9, CFD_LBM_D3Q19 = 19, CFD_LBM_D3Q27 = 27 };
 
-/* Population storage.  Collision arithmetic is FP32 except for FP64.  FP32 follows the
+/* Population storage.  Collision arithmetic is FP32 except for FP64 and FP64X.  FP32 follows the
  * reference's arithmetic bit for bit; the shifted storages hold the deviation f - w_d and collide
  * on deviations (rho = 1 + sum f'), which keeps the non-equilibrium part at FP32 precision at low
- * velocities. */
+ * velocities.  FP64 keeps the reference's FP32 lattice-weight constants (as its double solver
+ * does), whose sum exceeds 1 by 1.49e-8 (D3Q19) or 7.45e-9 (D2Q9, D3Q27); FP64X uses the weights'
+ * exact values and has no reference counterpart (a reference for precision studies). */
 enum {
     CFD_LBM_STORE_FP32 = 0,  /* 4 B / population (the reference's storage and arithmetic)  */
     CFD_LBM_STORE_FP16S = 1, /* 2 B / population: binary16 of (f - w_d) * 4096, RNE        */
     CFD_LBM_STORE_FP64 = 2,  /* 8 B / population: verification build of the same kernels   */
-    CFD_LBM_STORE_FP32S = 3  /* 4 B / population: f - w_d in FP32                          */
+    CFD_LBM_STORE_FP32S = 3, /* 4 B / population: f - w_d in FP32                          */
+    CFD_LBM_STORE_FP64X = 4  /* 8 B / population: FP64 with the exact lattice weights       */
 };
 
 /* uint8 cell flags (cfd_lbm_set_flags). */
@@ -128,11 +131,11 @@ CFD_LBM_API int cfd_lbm_synchronize(cfd_lbm_solver* s, char* error, size_t error
  * Fluid u = m/rho + a/2; wall cells report (rho_wall, 0, 0, 0) -- esoteric_pull.hpp:180-209. */
 CFD_LBM_API int cfd_lbm_download_macroscopic(cfd_lbm_solver* s, float* rho, float* ux, float* uy, float* uz,
                                              char* error, size_t error_capacity);
-/* Same in double precision (exact for FP64 storage). */
+/* Same in double precision (exact for FP64 and FP64X storage). */
 CFD_LBM_API int cfd_lbm_download_macroscopic_f64(cfd_lbm_solver* s, double* rho, double* ux, double* uy, double* uz,
                                                  char* error, size_t error_capacity);
 /* refresh_device_macroscopic -- esoteric_pull_sycl.hpp:229-307: device pointers to rho, ux, uy, uz
- * (cells each, element type float, or double for FP64 storage), valid until the next refresh. */
+ * (cells each, element type float, or double for FP64 and FP64X storage), valid until the next refresh. */
 CFD_LBM_API int cfd_lbm_refresh_device_macroscopic(cfd_lbm_solver* s, void** rho, void** ux, void** uy, void** uz,
                                                    char* error, size_t error_capacity);
 /* total_mass -- esoteric_pull_sycl.hpp:327-339, accumulated in FP64 (the reference uses FP32). */
@@ -159,7 +162,7 @@ CFD_LBM_API int cfd_lbm_transfer_stats(cfd_lbm_solver* s, uint64_t* stats4, int 
                                        size_t error_capacity);
 
 /* InPlaceLbmConfig::smagorinsky_les (esoteric_pull.hpp:28, closure at :72-97, used at :422-425):
- * local relaxation time from the non-equilibrium momentum flux.  FP32 or FP64 storage. */
+ * local relaxation time from the non-equilibrium momentum flux.  FP32, FP64 or FP64X storage. */
 CFD_LBM_API int cfd_lbm_set_smagorinsky(cfd_lbm_solver* s, int enabled, char* error, size_t error_capacity);
 
 /* Objective kinds of the adjoint entry points. */

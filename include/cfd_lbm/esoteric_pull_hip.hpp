@@ -36,6 +36,7 @@ enum class HipPopulationStorage : int {
     fp16_shifted = CFD_LBM_STORE_FP16S,
     fp64 = CFD_LBM_STORE_FP64,
     fp32_shifted = CFD_LBM_STORE_FP32S,
+    fp64_exact_weights = CFD_LBM_STORE_FP64X,
 };
 
 struct HipMacroscopicFields {

@@ -55,6 +55,29 @@ def oracle_duct(oracle_mod, n, blend, u_peak=0.05, storage="fp64"):
     return o
 
 
+def test_fp64x_adjoint_matches_exact_weight_oracle(hip, oracle_mod):
+    """FP64X (the lattice weights' exact values) against the oracle's FP64 textbook adjoint with the
+    same exact weights, to 1e-9; the reference-weight FP64 gradient differs from it by more."""
+    report("adjoint_vs_oracle", storage="fp64x")
+    n = (16, 8, 8)
+    flags = duct(*n)
+    blend = 0.05 * design(flags)
+    T, W, every = 40, 8, 7
+    g = gpu_duct(hip, n, blend, storage="fp64x")
+    J, grad, du = g.adjoint_run(T, W, every, 1, n[0] - 2)
+    with oracle_mod.exact_fp64_weights():
+        o = oracle_duct(oracle_mod, n, blend, storage="fp64")
+        Jo, grado, duo = o.adjoint_run(T, W, 1, n[0] - 2)
+    assert abs(J - Jo) <= 1e-9 * abs(Jo)
+    scale = np.abs(grado).max()
+    assert np.abs(grad - grado).max() <= 1e-9 * scale
+    assert abs(du - duo) <= 1e-9 * abs(duo)
+    g64 = gpu_duct(hip, n, blend, storage="fp64")
+    J64, _, _ = g64.adjoint_run(T, W, every, 1, n[0] - 2)
+    report("adjoint_fp64x", J_rel_to_fp64=abs(J64 - J) / abs(J))
+    assert J64 != J
+
+
 @pytest.mark.parametrize("storage,tol", [("fp64", 1e-9), ("fp32", 2e-4)])
 def test_adjoint_matches_logical_space_oracle(hip, oracle_mod, storage, tol):
     report("adjoint_vs_oracle", storage=storage)

@@ -25,7 +25,9 @@ PrecisionPullSolver): they sum to 1 + 1.49e-8 (D3Q19) or 1 + 7.45e-9 (D3Q27, D2Q
 collision adds omega times that excess to a cell's density, so the FP64 and FP32 builds gain mass at
 ~2.5e-8 of the total per step (tau 0.6, D3Q19).  The shifted storages apply the weights to
 deviations only and conserve mass to rounding: their max density error against FP64 is the FP64
-build's drift, and the fluctuation error removes it.
+build's drift, and the fluctuation error removes it.  Each workload with an FP64 run also runs
+FP64X, the FP64 build with the weights' exact values, and reports every error against it as well
+(the *_exact_result lines); FP64X conserves mass to rounding.
 
 Collision arithmetic is FP32 except in the FP64 build: the reference's arithmetic on full
 populations for fp32, the same arithmetic on deviations f - w_d for the shifted storages fp32s
@@ -100,7 +102,7 @@ def report_errors(case, storage, e, **extra):
 def test_taylor_green_64_errors_vs_fp64(hip):
     n, steps = 64, 110
     fields, drift = {}, {}
-    for storage in ("fp64",) + SHIFTED_AND_FP32:
+    for storage in ("fp64x", "fp64") + SHIFTED_AND_FP32:
         report("taylor_green_precision", storage=storage, n=n, steps=steps)
         s = hip.EsotericPullHipSolver(19, n, n, n, 0.6, storage=storage)
         s.initialize_taylor_green(0.05, 1e-4, 11)
@@ -112,10 +114,16 @@ def test_taylor_green_64_errors_vs_fp64(hip):
     for k in SHIFTED_AND_FP32:
         report_errors("taylor_green_precision_result", k, e[k], mass_drift=drift[k])
     report("taylor_green_precision_fp64", mass_drift=drift["fp64"])
+    ex = {k: field_errors(fields[k], fields["fp64x"], fluid) for k in ("fp64",) + SHIFTED_AND_FP32}
+    for k in ex:
+        report_errors("taylor_green_precision_exact_result", k, ex[k], vs="fp64x")
+    report("taylor_green_precision_fp64x", mass_drift=drift["fp64x"])
+    assert ex["fp32s"][0] <= 1e-6 and ex["fp32s"][2] <= 1e-5
     for k in ("fp32", "fp32s"):
         assert e[k][0] <= 1e-5 and e[k][1] <= 2e-3 and e[k][2] <= 1e-5
     assert e["fp16s"][0] <= 1e-4 and e["fp16s"][1] <= 3e-2 and e["fp16s"][2] <= 1e-3
     assert abs(drift["fp32s"]) <= 1e-7 and abs(drift["fp16s"]) <= 1e-6
+    assert abs(drift["fp64x"]) <= 1e-12
 
 
 def test_cavity_256_velocity_error_vs_fp64(hip):
@@ -123,7 +131,7 @@ def test_cavity_256_velocity_error_vs_fp64(hip):
     flags = cavity_flags(n)
     fluid = (flags == 0).reshape(n, n, n)
     fields, drift = {}, {}
-    for storage in ("fp64",) + SHIFTED_AND_FP32:
+    for storage in ("fp64x", "fp64") + SHIFTED_AND_FP32:
         report("cavity_precision", storage=storage, n=n, steps=steps)
         s = hip.EsotericPullHipSolver(19, n, n, n, 0.55, storage=storage)
         s.set_flags(flags)
@@ -136,6 +144,11 @@ def test_cavity_256_velocity_error_vs_fp64(hip):
     for k in SHIFTED_AND_FP32:
         report_errors("cavity_precision_result", k, e[k], mass_drift=drift[k])
     report("cavity_precision_fp64", mass_drift=drift["fp64"])
+    ex = {k: field_errors(fields[k], fields["fp64x"], fluid) for k in ("fp64",) + SHIFTED_AND_FP32}
+    for k in ex:
+        report_errors("cavity_precision_exact_result", k, ex[k], vs="fp64x")
+    report("cavity_precision_fp64x", mass_drift=drift["fp64x"])
+    assert ex["fp32s"][0] <= 1e-6 and ex["fp16s"][2] <= 1e-3
     assert e["fp32"][2] <= 5e-4 and e["fp32s"][2] <= 5e-4
     assert e["fp16s"][2] <= 1e-3
     assert max(e[k][0] for k in SHIFTED_AND_FP32) <= 5e-4
@@ -146,7 +159,7 @@ def test_cavity_d3q27_256_errors_vs_fp64(hip):
     flags = cavity_flags(n)
     fluid = (flags == 0).reshape(n, n, n)
     fields, drift = {}, {}
-    for storage in ("fp64",) + SHIFTED_AND_FP32:
+    for storage in ("fp64x", "fp64") + SHIFTED_AND_FP32:
         report("cavity27_precision", storage=storage, n=n, steps=steps)
         s = hip.EsotericPullHipSolver(27, n, n, n, 0.55, storage=storage)
         s.set_flags(flags)
@@ -159,6 +172,11 @@ def test_cavity_d3q27_256_errors_vs_fp64(hip):
     for k in SHIFTED_AND_FP32:
         report_errors("cavity27_precision_result", k, e[k], mass_drift=drift[k])
     report("cavity27_precision_fp64", mass_drift=drift["fp64"])
+    ex = {k: field_errors(fields[k], fields["fp64x"], fluid) for k in ("fp64",) + SHIFTED_AND_FP32}
+    for k in ex:
+        report_errors("cavity27_precision_exact_result", k, ex[k], vs="fp64x")
+    report("cavity27_precision_fp64x", mass_drift=drift["fp64x"])
+    assert ex["fp32s"][0] <= 2e-6 and ex["fp16s"][2] <= 2e-3
     assert e["fp32"][2] <= 1e-3 and e["fp32s"][2] <= 1e-3
     assert e["fp16s"][2] <= 2e-3
     assert max(e[k][0] for k in SHIFTED_AND_FP32) <= 1e-3
@@ -179,7 +197,7 @@ def test_porous_channel_permeability_vs_fp64(hip):
     porosity = fluid.sum() / interior
     nu = (tau - 0.5) / 3.0
     res = {}
-    for storage in ("fp64",) + SHIFTED_AND_FP32:
+    for storage in ("fp64x", "fp64") + SHIFTED_AND_FP32:
         report("porous_precision", storage=storage, spheres=len(radii), porosity=porosity, steps=steps)
         s = hip.EsotericPullHipSolver(19, nx, ny, nz, tau, acceleration=(g, 0.0, 0.0), storage=storage)
         s.set_flags(flags.reshape(-1))
@@ -197,6 +215,13 @@ def test_porous_channel_permeability_vs_fp64(hip):
         report_errors("porous_precision_result", storage, out[storage][2:], mean_u_err=out[storage][0],
                       k_err=out[storage][1], mass_drift=drift)
     report("porous_precision_fp64", mean_u=u64, k=k64, mass_drift=drift64)
+    ux_, kx, mx, driftx = res["fp64x"]
+    for storage in ("fp64",) + SHIFTED_AND_FP32:
+        u, k, m, drift = res[storage]
+        report_errors("porous_precision_exact_result", storage, field_errors(m, mx, fluid), vs="fp64x",
+                      mean_u_err=abs(u - ux_) / abs(ux_), k_err=abs(k - kx) / abs(kx))
+    report("porous_precision_fp64x", mean_u=ux_, k=kx, mass_drift=driftx)
+    assert abs(driftx) <= 1e-10
     assert u64 > 0 and np.isfinite(u64)
     assert out["fp32s"][1] <= 1e-4
     assert out["fp16s"][1] <= 2e-3
@@ -237,7 +262,7 @@ def test_porous_duct_128_objective_and_gradient_vs_fp64(hip):
     flags, blend = porous_duct(n)
     fluid = (flags == 0).reshape(n[::-1])
     runs = {}
-    for storage in ("fp64",) + SHIFTED_AND_FP32:
+    for storage in ("fp64x", "fp64") + SHIFTED_AND_FP32:
         report("duct_precision", storage=storage, n=n, steps=T, window=W)
         s = hip.EsotericPullHipSolver(19, *n, 0.6, storage=storage)
         s.set_flags(flags)
@@ -247,6 +272,12 @@ def test_porous_duct_128_objective_and_gradient_vs_fp64(hip):
         J, grad, du, _ = s.adjoint_run_binomial(T, W, 0, "pressure_drop", 1, n[0] - 2)
         runs[storage] = (J, grad, du, s.download_macroscopic(f64=True))
         s.close()
+    exact = adjoint_errors("duct_precision_exact_result", runs, fluid)
+    # Against the exact weights the shifted FP32 storage is the closest; the reference's FP32
+    # weight constants put FP64 and FP32 ~8e-5 off in J.
+    assert max(exact["fp32s"][i] for i in (0, 1)) <= 5e-5
+    assert max(exact["fp16s"][i] for i in (0, 1)) <= 1e-3
+    runs.pop("fp64x")
     out = adjoint_errors("duct_precision_result", runs, fluid)
     # fp32 shares the FP64 build's weight constants; fp32s differs from it by their mass source.
     assert max(out["fp32"][i] for i in (0, 1, 2, 5)) <= 2e-5
@@ -280,9 +311,12 @@ def test_sphere_cluster_512_drag_and_gradient_vs_fp64(hip):
     cluster = G.sphere_cluster_duct(n, seed=5)
     fluid = (cluster[0] == 0).reshape(n, n, n)
     runs = {}
-    for storage in ("fp64",) + SHIFTED_AND_FP32:
+    for storage in ("fp64x", "fp64") + SHIFTED_AND_FP32:
         report("cluster512_precision", storage=storage, n=n, steps=T)
         runs[storage] = cluster_run(hip, cluster, n, storage, T, adjoint=True)
+    exact = adjoint_errors("cluster512_precision_exact_result", runs, fluid)
+    assert exact["fp32s"][0] <= 1e-5 and exact["fp32s"][1] <= 1e-4
+    runs.pop("fp64x")
     out = adjoint_errors("cluster512_precision_result", runs, fluid)
     for k in SHIFTED_AND_FP32:
         assert np.isfinite(runs[k][0]) and runs[k][0] > 0
