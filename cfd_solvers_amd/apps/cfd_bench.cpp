@@ -4,7 +4,7 @@
 // backends belong to the reference build; asking this binary for them exits with code 2, as the
 // reference does for an SYCL backend it was not compiled with.
 //
-// Added flags: --storage fp32|fp16s|fp64|fp32s (population storage of the HIP backend).
+// Added flags: --storage fp32|fp16s|fp64|fp32s|fp64x (population storage of the HIP backend).
 #include "cfd_lbm/esoteric_pull_hip.hpp"
 
 #include <chrono>
@@ -81,7 +81,7 @@ Options parse_options(int argc, char** argv) {
                       << "  --lattice d2q9|d3q19|d3q27\n"
                       << "  --backend cpu|sycl|hip   (this build runs hip)\n"
                       << "  --streaming in-place|pull|both   (pull/both are CPU-only)\n"
-                      << "  --storage fp32|fp16s|fp64|fp32s  (hip population storage)\n"
+                      << "  --storage fp32|fp16s|fp64|fp32s|fp64x  (hip population storage)\n"
                       << "  --nx N --ny N --nz N\n"
                       << "  --warmup N --steps N --tau T\n"
                       << "  --threads N      OpenMP thread count (CPU)\n"
@@ -113,10 +113,13 @@ cfd::lbm::HipPopulationStorage storage_of(const std::string& s) {
     if (s == "fp16s") return cfd::lbm::HipPopulationStorage::fp16_shifted;
     if (s == "fp64") return cfd::lbm::HipPopulationStorage::fp64;
     if (s == "fp32s") return cfd::lbm::HipPopulationStorage::fp32_shifted;
-    throw std::invalid_argument("storage must be fp32, fp16s, fp64 or fp32s");
+    if (s == "fp64x") return cfd::lbm::HipPopulationStorage::fp64_exact_weights;
+    throw std::invalid_argument("storage must be fp32, fp16s, fp64, fp32s or fp64x");
 }
 
-std::size_t bytes_per_population(const std::string& s) { return s == "fp16s" ? 2U : (s == "fp64" ? 8U : 4U); }
+std::size_t bytes_per_population(const std::string& s) {
+    return s == "fp16s" ? 2U : (s == "fp64" || s == "fp64x" ? 8U : 4U);
+}
 
 template <class D> int run_hip(const Options& o) {
     if (o.streaming != "in-place") throw std::invalid_argument("HIP benchmark supports only --streaming in-place");
