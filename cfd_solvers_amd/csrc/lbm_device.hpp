@@ -288,6 +288,18 @@ template <int Q> struct PairSlots {
     __device__ __forceinline__ bool is_wall_link(int link) const { return ((desc >> link) & 1U) != 0U; }
 };
 
+// Link bits (descriptor layout above) of a fluid cell whose only wall neighbours are the cells of
+// the x column at x + side (side -1 or +1).
+template <int Q> constexpr uint32_t x_wall_links(int side) {
+    uint32_t m = 0U;
+    for (int pair = 0; pair < (Q - 1) / 2; ++pair) {
+        const int cx = Lattice<Q>::cx[2 * pair + 1];
+        if (cx == -side) m |= 1U << (2 * pair);     // population i from n - c_i, at x + side
+        if (cx == side) m |= 1U << (2 * pair + 1);  // population i + 1 from n + c_i, at x + side
+    }
+    return m;
+}
+
 // Direction plane holding fin[i] (a) and fin[i+1] (b) of pair i.
 template <bool Odd> __device__ __forceinline__ int dir_a(int i, bool wall_up) { return (Odd != wall_up) ? i + 1 : i; }
 template <bool Odd> __device__ __forceinline__ int dir_b(int i, bool wall_dn) { return (Odd != wall_dn) ? i : i + 1; }
@@ -378,6 +390,11 @@ __device__ __forceinline__ void moments(const Real (&f)[Q], Real& rho, Real& mx,
 
 constexpr uint32_t kSpanBits = 0xFFFFU;        // lo | hi << 8 of a segment word
 constexpr uint32_t kPorousSegment = 1U << 16;  // the segment holds a cell with gamma != 0
+// Bits 17..18 of a segment word: the span is a resting x wall and its one fluid neighbour, whose
+// 3x3x3 neighbourhood is plain fluid but for the wall's x column.  1: wall at lane lo, fluid at
+// lo + 1; 2: fluid at lo, wall at lo + 1.  The fluid cell's wall links then follow from the
+// lattice alone (x_wall_links), without its link descriptor.
+constexpr uint32_t kXWallShift = 17;
 
 template <class Real> struct Physics {
     Real omega;
@@ -393,9 +410,10 @@ template <class Real> struct Physics {
     const uint32_t* links;      // per-cell link descriptor (bounded lattices)
     const uint32_t* link_vel;   // per cell, (q-1+9)/10 words of 3-bit wall-velocity indices per link
     // Per 64-cell row segment of a bounded lattice: the span lo | hi << 8 of the cells [lo, hi) that
-    // are not plain fluid with a plain-fluid 3x3x3 neighbourhood, 0 if none, and kPorousSegment when
-    // a cell of the segment has a nonzero partial bounce-back fraction.  A 32-bit word, read through
-    // the scalar cache by the wavefront that updates the segment.
+    // are not plain fluid with a plain-fluid 3x3x3 neighbourhood, 0 if none, kPorousSegment when
+    // a cell of the segment has a nonzero partial bounce-back fraction, and the x-wall pattern in
+    // bits kXWallShift..+1.  A 32-bit word, read through the scalar cache by the wavefront that
+    // updates the segment.
     const uint32_t* segments;
     uint32_t seg_x;             // segments per row, ceil(nx / 64)
     uint32_t cells;

@@ -65,6 +65,18 @@ __global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const vo
             return;
         }
         const uint32_t xi = x & 63U;
+        const uint32_t side = (word >> kXWallShift) & 3U;
+        if (!Forced && side != 0U) {  // the forced kernels keep their register budget
+            // A resting x wall and its one fluid neighbour (the segments at the ends of a row in a
+            // walled box): the neighbour's wall links follow from the lattice, with no descriptor
+            // load ahead of the population loads.
+            const uint32_t lo = span & 0xFFU;
+            if (xi == (side == 1U ? lo : lo + 1U)) return;  // the wall cell
+            const uint32_t links = side == 1U ? x_wall_links<Q>(-1) : x_wall_links<Q>(1);
+            const uint32_t desc = xi == (side == 1U ? lo + 1U : lo) ? links : 0U;
+            update_cell<Q, T, Odd, true, Forced>(src, dst, g, p, gamma, h, kFluid, desc, n, x, y, z);
+            return;
+        }
         const uint32_t desc = (xi >= (span & 0xFFU) && xi < (span >> 8)) ? p.links[n] : 0U;
         const uint32_t type = (desc >> kTypeShift) & 3U;
         if (type == kTypeWall) return;
@@ -111,7 +123,20 @@ __global__ __launch_bounds__(256) void mark_segments(const uint8_t* __restrict__
         left = mid;
         mid = right;
     }
-    segments[k] = hi == 0U ? 0U : lo | (hi << 8);
+    // The x-wall pattern (kXWallShift): a resting wall column and a plain-fluid column on the
+    // other side of the span's fluid cell.
+    uint32_t side = 0U;
+    if (hi == lo + 2U) {
+        auto wall_column = [&](uint32_t xx) {
+            for (int i = 0; i < 9; ++i)
+                if (flags[rows[i] + xx] != kWall) return false;
+            return true;
+        };
+        const uint32_t xa = xs + lo, xb = xa + 1U;
+        if (wall_column(xa) && column(xb) && column(xb + 1U == g.nx ? 0U : xb + 1U)) side = 1U;
+        else if (wall_column(xb) && column(xa) && column(xa == 0U ? g.nx - 1U : xa - 1U)) side = 2U;
+    }
+    segments[k] = hi == 0U ? 0U : lo | (hi << 8) | (side << kXWallShift);
 }
 
 // Sets kPorousSegment in the word of every row segment that holds a cell with gamma != 0 (after
