@@ -88,7 +88,8 @@ __global__ __launch_bounds__(256) CFD_LBM_OCCUPANCY void stream_collide(const vo
 
 // The span of each 64-cell row segment: cells [lo, hi) of the segment that are not plain fluid
 // with a plain-fluid 3x3x3 neighbourhood (periodic), stored as lo | hi << 8; 0 when every cell is
-// (the interior segments).  Cells outside the span have link descriptor 0.
+// (the interior segments).  Cells outside the span have link descriptor 0.  A two-cell span that
+// is a resting x wall and its fluid neighbour also records that pattern (kXWallShift).
 __global__ __launch_bounds__(256) void mark_segments(const uint8_t* __restrict__ flags, const Geometry g,
                                                      uint32_t seg_x, uint32_t* __restrict__ segments) {
     const uint32_t total = seg_x * g.ny * g.nz;
