@@ -236,6 +236,41 @@ def test_walled_channel_parity(hip, oracle_mod, q):
     assert np.array_equal(g.logical_populations()[:, fluid].astype(np.float64), o.logical()[:, fluid])
 
 
+@pytest.mark.parametrize("q,storage,moving_x", [(19, "fp32", False), (19, "fp16s", False), (19, "fp32s", False),
+                                                (27, "fp32", False), (9, "fp32", False), (19, "fp32", True)])
+def test_row_end_wall_segments_match_oracle(hip, oracle_mod, q, storage, moving_x):
+    """A walled box 70 cells wide: the first 64-cell segment of a row starts with the x = 0 wall
+    and its fluid neighbour, and the second, 6 cells long, ends with the fluid cell x = 68 and the
+    x = 69 wall.  Rows away from the y and z walls take the row-end path, which derives the fluid
+    cell's wall links from the lattice.  Bit for bit against the oracle; with the x = 0 wall moving
+    (velocity index 2) its rows take the descriptor path instead."""
+    report("row_end", q=q, storage=storage, moving_x=moving_x)
+    nx, ny, nz = 70, 9, 1 if q == 9 else 7
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, :, 0] = f[:, :, -1] = 0x80
+    f[:, 0, :] = 0x80
+    f[:, -1, :] = 0x81
+    if q != 9:
+        f[0, :, :] = f[-1, :, :] = 0x80
+    if moving_x:
+        f[:, 1:-1, 0] = 0x82
+    flags = f.reshape(-1)
+    g, o = make_pair(hip, oracle_mod, q, (nx, ny, nz), 0.55, storage=storage)
+    for s in (g, o):
+        s.set_flags(flags)
+        s.set_wall_velocity(1, (float(np.float32(0.1)), 0.0, 0.0))
+        s.set_wall_velocity(2, (0.0, float(np.float32(0.05)), 0.0))
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    g.initialize_taylor_green(0.03, 1e-4, 7)
+    o.init_taylor_green(f32(0.03), f32(1e-4), 7)
+    g.step(9)
+    o.step(9)
+    fluid = flags == 0
+    assert_fields(gpu_fields(g), o.macroscopic(), mask=fluid)
+    assert np.abs(gpu_fields(g)[1][fluid]).max() > 1e-3
+    g.close()
+
+
 @pytest.mark.parametrize("q", [19, 27])
 def test_moving_lid_cavity_parity(hip, oracle_mod, q):
     """Config-2 geometry at 24^3: walls x=0, x=nx-1, y=0, z=0, z=nz-1, lid y=ny-1 moving at 0.1."""
